@@ -1,0 +1,651 @@
+// cs3d_api.hip - C ABI of libcs3d: context and pool management, systems,
+// stepping (eager or HIP-graph replay), per-kernel entry points and field I/O.
+// Replaces the host driver's device-side orchestration in main()
+// (src/GPUbounce.cu:358-1506) for one or many systems per GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "cs3d_dev.h"
+#include "cs3d_internal.h"
+
+namespace cs3d {
+void launch_integrate(const Pool& p, const Range& r, int flags, int64_t step_override, hipStream_t st);
+void launch_con(const Pool& p, const Range& r, int mode, bool use_va, hipStream_t st);
+void launch_dis(const Pool& p, const Range& r, int mode, int64_t step_override, bool use_va, hipStream_t st);
+void launch_divide(const Pool& p, const Range& r, int mode, hipStream_t st);
+void launch_grid(const Pool& p, const Range& r, int vox_blocks, hipStream_t st);
+void launch_velocity_ab(const Pool& p, const Range& r, hipStream_t st);
+void launch_set_rebuild(const Pool& p, const Range& r, int value, hipStream_t st);
+}  // namespace cs3d
+
+using namespace cs3d;
+
+#define HIPCHK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) return fail(CS3D_HIP_ERROR, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+namespace {
+
+enum { I_KICK = 1, I_PRESSURE = 2, I_PBOX = 4 };
+
+struct Block {
+    int64_t off, len;
+};
+
+// first-fit allocator over [0, total)
+struct RangeAlloc {
+    int64_t total = 0;
+    std::vector<Block> used;  // sorted by off
+    bool alloc(int64_t len, int64_t& off) {
+        int64_t cur = 0;
+        size_t i = 0;
+        for (; i < used.size(); ++i) {
+            if (used[i].off - cur >= len) break;
+            cur = used[i].off + used[i].len;
+        }
+        if (i == used.size() && total - cur < len) return false;
+        off = cur;
+        used.insert(used.begin() + i, Block{off, len});
+        return true;
+    }
+    void release(int64_t off) {
+        for (size_t i = 0; i < used.size(); ++i)
+            if (used[i].off == off) { used.erase(used.begin() + i); return; }
+    }
+    int64_t high() const { return used.empty() ? 0 : used.back().off + used.back().len; }
+};
+
+struct SysHost {
+    bool used = false;
+    int32_t base = 0, cap = 0;
+    int64_t vreg = 0;  // voxel region offset
+    int32_t vbudget = 0, vcap = 0;
+    cs3d_params prm;
+};
+
+}  // namespace
+
+struct cs3d_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t cap_cells = 0;
+    int64_t vox_total = 0;
+    Pool pool;
+    std::vector<void*> allocs;
+    RangeAlloc cells, voxels;
+    std::vector<SysHost> sys;
+    int nsys_high = 0;
+    bool use_graph = true;
+    int graph_steps = 10;
+    std::map<int, std::pair<hipGraphExec_t, int>> graphs;  // target -> (exec, steps)
+};
+
+namespace {
+
+template <class T> cs3d_status dalloc(cs3d_ctx* c, T*& p, int64_t count) {
+    void* q = nullptr;
+    const size_t bytes = (size_t)std::max<int64_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return fail(CS3D_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+    e = hipMemset(q, 0, bytes);
+    if (e != hipSuccess) return fail(CS3D_HIP_ERROR, std::string("hipMemset: ") + hipGetErrorString(e));
+    c->allocs.push_back(q);
+    p = (T*)q;
+    return CS3D_OK;
+}
+
+void drop_graphs(cs3d_ctx* c, int target_only = -2) {
+    for (auto it = c->graphs.begin(); it != c->graphs.end();) {
+        if (target_only == -2 || it->first == target_only || it->first == CS3D_ALL) {
+            (void)hipGraphExecDestroy(it->second.first);
+            it = c->graphs.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
+bool valid_sys(cs3d_ctx* c, cs3d_sys s) { return c && s >= 0 && s < (int)c->sys.size() && c->sys[s].used; }
+
+Range range_of(cs3d_ctx* c, cs3d_sys s) {
+    Range r;
+    if (s == CS3D_ALL) {
+        r.slot_lo = 0;
+        r.slot_hi = (int32_t)c->cells.high();
+        r.sys_lo = 0;
+        r.sys_hi = c->nsys_high;
+    } else {
+        r.slot_lo = c->sys[s].base;
+        r.slot_hi = c->sys[s].base + c->sys[s].cap;
+        r.sys_lo = s;
+        r.sys_hi = s + 1;
+    }
+    return r;
+}
+
+int vox_blocks_of(cs3d_ctx* c, cs3d_sys s) {
+    int64_t vb = 0;
+    if (s == CS3D_ALL) {
+        for (auto& h : c->sys)
+            if (h.used) vb = std::max<int64_t>(vb, h.vbudget);
+    } else {
+        vb = c->sys[s].vbudget;
+    }
+    int64_t b = vb / 4096;
+    return (int)std::min<int64_t>(64, std::max<int64_t>(1, b));
+}
+
+cs3d_status read_meta(cs3d_ctx* c, cs3d_sys s, SysMeta& m) {
+    HIPCHK(hipMemcpyAsync(&m, c->pool.meta + s, sizeof(SysMeta), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CS3D_OK;
+}
+
+cs3d_status write_meta(cs3d_ctx* c, cs3d_sys s, const SysMeta& m) {
+    HIPCHK(hipMemcpyAsync(c->pool.meta + s, &m, sizeof(SysMeta), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CS3D_OK;
+}
+
+void enqueue_step(cs3d_ctx* c, const Range& r, int vb) {
+    const Pool& p = c->pool;
+    launch_integrate(p, r, I_KICK | I_PRESSURE | I_PBOX, -1, c->stream);
+    launch_con(p, r, M_STEP, true, c->stream);
+    launch_dis(p, r, M_STEP, -1, false, c->stream);
+    launch_divide(p, r, M_STEP, c->stream);
+    launch_grid(p, r, vb, c->stream);
+}
+
+cs3d_status run_startup(cs3d_ctx* c, cs3d_sys s) {
+    const Pool& p = c->pool;
+    Range r = range_of(c, s);
+    const int vb = vox_blocks_of(c, s);
+    // CenterOfMass, bounding_boxes, minmaxpre/post, makeNNlist (GPUbounce.cu:739-763)
+    launch_dis(p, r, M_COM, 0, false, c->stream);
+    launch_integrate(p, r, I_PBOX, 0, c->stream);
+    launch_set_rebuild(p, r, 1, c->stream);
+    launch_grid(p, r, vb, c->stream);
+    launch_set_rebuild(p, r, 0, c->stream);
+    // initial CalculateConForce and CalculateDisForce (GPUbounce.cu:978-1004)
+    launch_con(p, r, M_INIT, false, c->stream);
+    launch_dis(p, r, M_DIS_V, 0, false, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CS3D_OK;
+}
+
+struct FieldLoc {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+FieldLoc field_loc(cs3d_ctx* c, cs3d_sys s, cs3d_field f) {
+    FieldLoc L;
+    const SysHost& h = c->sys[s];
+    const Pool& p = c->pool;
+    const int64_t nb = (int64_t)h.base * kSlots;
+    const size_t nbytes = (size_t)h.cap * kSlots * sizeof(float);
+    float* node[] = {p.X, p.Y, p.Z, p.VX, p.VY, p.VZ, p.FCX, p.FCY, p.FCZ, p.FDX, p.FDY, p.FDZ, p.VAX, p.VAY, p.VAZ};
+    if (f >= CS3D_F_X && f <= CS3D_F_VAZ) {
+        L.ptr = node[f] + nb;
+        L.bytes = nbytes;
+        return L;
+    }
+    switch (f) {
+        case CS3D_F_CMX: L.ptr = p.CMX + h.base; L.bytes = h.cap * 4; break;
+        case CS3D_F_CMY: L.ptr = p.CMY + h.base; L.bytes = h.cap * 4; break;
+        case CS3D_F_CMZ: L.ptr = p.CMZ + h.base; L.bytes = h.cap * 4; break;
+        case CS3D_F_BBOX: L.ptr = p.BBOX + (int64_t)h.base * 6; L.bytes = (size_t)h.cap * 24; break;
+        case CS3D_F_PRESSURE: L.ptr = p.P + h.base; L.bytes = h.cap * 4; break;
+        case CS3D_F_STIFFNESS: L.ptr = p.E + h.base; L.bytes = h.cap * 4; break;
+        case CS3D_F_VOLUME: L.ptr = p.VOL + h.base; L.bytes = h.cap * 4; break;
+        case CS3D_F_AREA: L.ptr = p.AREA + h.base; L.bytes = h.cap * 4; break;
+        case CS3D_F_DIVFLAG: L.ptr = p.FLAG + h.base; L.bytes = h.cap; break;
+        case CS3D_F_VOXCOUNT: L.ptr = p.vox + h.vreg; L.bytes = (size_t)h.vbudget * 4; break;
+        case CS3D_F_VOXLIST: L.ptr = p.vox + h.vreg + h.vbudget; L.bytes = (size_t)h.vbudget * h.vcap * 4; break;
+        case CS3D_F_PATCHBOX: L.ptr = p.PBOX + (int64_t)h.base * kPBox; L.bytes = (size_t)h.cap * kPBox * 4; break;
+        case CS3D_F_GROWTH_SCALE: L.ptr = p.GS + h.base; L.bytes = h.cap * 4; break;
+        case CS3D_F_GRID: L.bytes = 32; break;
+        case CS3D_F_RANMAR: L.bytes = sizeof(cs3d_ranmar_state); break;
+        default: break;
+    }
+    return L;
+}
+
+}  // namespace
+
+extern "C" cs3d_status cs3d_ctx_create(int device, const cs3d_mesh* mesh, int64_t capacity_cells, int64_t voxel_budget,
+                                       cs3d_ctx** out) {
+    if (!mesh || !out || capacity_cells <= 0 || capacity_cells > (1ll << 30) / kSlots || voxel_budget <= 0)
+        return fail(CS3D_INVALID_ARG, "bad context arguments");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(CS3D_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= ndev) return fail(CS3D_NO_DEVICE, "device index out of range");
+    // angle gather table and mesh consistency (every bond listed from both ends)
+    MeshDev md;
+    std::memset(&md, 0, sizeof(md));
+    std::memcpy(md.nn, mesh->nn, sizeof(md.nn));
+    std::memcpy(md.sign, mesh->sign, sizeof(md.sign));
+    std::memcpy(md.faces, mesh->faces, sizeof(md.faces));
+    std::memcpy(md.R0, mesh->R0, sizeof(md.R0));
+    std::memcpy(md.theta0, mesh->theta0, sizeof(md.theta0));
+    for (int p = 0; p < kNodes; ++p) {
+        for (int q = 0; q < 3; ++q) {
+            const int n = mesh->nn[q * kSlots + p];
+            if (n < 0 || n >= kNodes) return fail(CS3D_INVALID_ARG, "mesh neighbour out of range");
+            const int a0 = mesh->nn[n], a1 = mesh->nn[kSlots + n], a2 = mesh->nn[2 * kSlots + n];
+            int e0, e1;  // NeighNeighs (propagate.cu:63-78)
+            if (a0 == p) { e0 = (n * 3 + 0) * 6 + 0; e1 = (n * 3 + 2) * 6 + 0; }
+            else if (a1 == p) { e0 = (n * 3 + 0) * 6 + 3; e1 = (n * 3 + 1) * 6 + 0; }
+            else if (a2 == p) { e0 = (n * 3 + 2) * 6 + 3; e1 = (n * 3 + 1) * 6 + 3; }
+            else return fail(CS3D_INVALID_ARG, "mesh neighbour lists are not symmetric");
+            md.gather[p * 6 + 2 * q] = e0;
+            md.gather[p * 6 + 2 * q + 1] = e1;
+        }
+    }
+    for (int i = 0; i < CS3D_FACES * 7; ++i)
+        if (mesh->faces[i] < 0 || mesh->faces[i] >= kNodes) return fail(CS3D_INVALID_ARG, "mesh face index out of range");
+
+    cs3d_ctx* c = new cs3d_ctx();
+    c->device = device;
+    c->cap_cells = capacity_cells;
+    c->vox_total = voxel_budget;
+    c->cells.total = capacity_cells;
+    c->voxels.total = voxel_budget;
+    c->sys.resize(kMaxSys);
+    auto cleanup = [&](cs3d_status st) {
+        cs3d_ctx_destroy(c);
+        return st;
+    };
+    if (hipSetDevice(device) != hipSuccess) return cleanup(fail(CS3D_HIP_ERROR, "hipSetDevice failed"));
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(CS3D_HIP_ERROR, "hipStreamCreate failed"));
+    Pool& p = c->pool;
+    const int64_t ns = capacity_cells * kSlots;
+    cs3d_status st = CS3D_OK;
+    float** nodes[] = {&p.X, &p.Y, &p.Z, &p.VX, &p.VY, &p.VZ, &p.VAX, &p.VAY, &p.VAZ,
+                       &p.FCX, &p.FCY, &p.FCZ, &p.FDX, &p.FDY, &p.FDZ};
+    for (float** a : nodes)
+        if ((st = dalloc(c, *a, ns)) != CS3D_OK) return cleanup(st);
+    float** cells[] = {&p.CMX, &p.CMY, &p.CMZ, &p.P, &p.E, &p.VOL, &p.AREA, &p.GS};
+    for (float** a : cells)
+        if ((st = dalloc(c, *a, capacity_cells)) != CS3D_OK) return cleanup(st);
+    if ((st = dalloc(c, p.FLAG, capacity_cells)) != CS3D_OK) return cleanup(st);
+    if ((st = dalloc(c, p.BBOX, capacity_cells * 6)) != CS3D_OK) return cleanup(st);
+    if ((st = dalloc(c, p.PBOX, capacity_cells * kPBox)) != CS3D_OK) return cleanup(st);
+    if ((st = dalloc(c, p.NCL, capacity_cells * kNCMax)) != CS3D_OK) return cleanup(st);
+    if ((st = dalloc(c, p.NCN, capacity_cells)) != CS3D_OK) return cleanup(st);
+    if ((st = dalloc(c, p.EVL, capacity_cells)) != CS3D_OK) return cleanup(st);
+    if ((st = dalloc(c, p.slot_sys, capacity_cells)) != CS3D_OK) return cleanup(st);
+    if ((st = dalloc(c, p.meta, kMaxSys)) != CS3D_OK) return cleanup(st);
+    cs3d_params* prm = nullptr;
+    if ((st = dalloc(c, prm, kMaxSys)) != CS3D_OK) return cleanup(st);
+    p.prm = prm;
+    if ((st = dalloc(c, p.vox, voxel_budget)) != CS3D_OK) return cleanup(st);
+    MeshDev* dm = nullptr;
+    if ((st = dalloc(c, dm, 1)) != CS3D_OK) return cleanup(st);
+    p.mesh = dm;
+    if (hipMemcpy(dm, &md, sizeof(md), hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(CS3D_HIP_ERROR, "mesh upload failed"));
+    if (hipMemset(p.slot_sys, 0xFF, capacity_cells * sizeof(int32_t)) != hipSuccess)
+        return cleanup(fail(CS3D_HIP_ERROR, "slot table init failed"));
+    *out = c;
+    return CS3D_OK;
+}
+
+extern "C" cs3d_status cs3d_ctx_destroy(cs3d_ctx* c) {
+    if (!c) return CS3D_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graphs(c);
+    for (void* q : c->allocs) (void)hipFree(q);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return CS3D_OK;
+}
+
+extern "C" cs3d_status cs3d_sync(cs3d_ctx* c) {
+    if (!c) return fail(CS3D_INVALID_ARG, "NULL context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipGetLastError());
+    return CS3D_OK;
+}
+
+extern "C" cs3d_status cs3d_system_add_raw(cs3d_ctx* c, const cs3d_params* prm, int32_t n_cells, int32_t max_cells,
+                                           int64_t voxels, cs3d_sys* out) {
+    if (!c || !prm || !out) return fail(CS3D_INVALID_ARG, "NULL argument");
+    cs3d_status st = cs3d_params_check(prm);
+    if (st != CS3D_OK) return st;
+    if (n_cells < 0 || max_cells < 1 || n_cells > max_cells) return fail(CS3D_INVALID_ARG, "bad cell counts");
+    int sid = -1;
+    for (int i = 0; i < kMaxSys; ++i)
+        if (!c->sys[i].used) { sid = i; break; }
+    if (sid < 0) return fail(CS3D_CAPACITY, "too many systems in this context");
+    int64_t base = 0, vreg = 0;
+    if (!c->cells.alloc(max_cells, base)) return fail(CS3D_CAPACITY, "cell pool exhausted");
+    const int32_t vcap = prm->voxel_capacity;
+    int64_t vb = voxels > 0 ? voxels : std::max<int64_t>(4096, 32 * (int64_t)max_cells);
+    if (vb > (1ll << 30)) vb = 1ll << 30;
+    if (!c->voxels.alloc(vb * (1 + vcap), vreg)) {
+        c->cells.release(base);
+        return fail(CS3D_CAPACITY, "voxel pool exhausted");
+    }
+    SysHost& h = c->sys[sid];
+    h.used = true;
+    h.base = (int32_t)base;
+    h.cap = max_cells;
+    h.vreg = vreg;
+    h.vbudget = (int32_t)vb;
+    h.vcap = vcap;
+    h.prm = *prm;
+    c->nsys_high = std::max(c->nsys_high, sid + 1);
+    HIPCHK(hipSetDevice(c->device));
+    const Pool& p = c->pool;
+    const int64_t nb = base * kSlots, nbytes = (int64_t)max_cells * kSlots * 4;
+    float* node[] = {p.X, p.Y, p.Z, p.VX, p.VY, p.VZ, p.VAX, p.VAY, p.VAZ, p.FCX, p.FCY, p.FCZ, p.FDX, p.FDY, p.FDZ};
+    for (float* a : node) HIPCHK(hipMemsetAsync(a + nb, 0, nbytes, c->stream));
+    float* cellz[] = {p.CMX, p.CMY, p.CMZ, p.VOL, p.AREA};
+    for (float* a : cellz) HIPCHK(hipMemsetAsync(a + base, 0, (size_t)max_cells * 4, c->stream));
+    HIPCHK(hipMemsetAsync(p.FLAG + base, 0, max_cells, c->stream));
+    HIPCHK(hipMemsetAsync(p.BBOX + base * 6, 0, (size_t)max_cells * 24, c->stream));
+    HIPCHK(hipMemsetAsync(p.PBOX + base * kPBox, 0, (size_t)max_cells * kPBox * 4, c->stream));
+    HIPCHK(hipMemsetAsync(p.NCN + base, 0, (size_t)max_cells * 4, c->stream));
+    HIPCHK(hipMemsetAsync(p.vox + vreg, 0, (size_t)vb * (1 + vcap) * 4, c->stream));
+    std::vector<float> pv(max_cells, prm->min_pressure), ev(max_cells, prm->stiffness1), gv(max_cells, 1.0f);
+    std::vector<int32_t> owner(max_cells, sid);
+    HIPCHK(hipMemcpyAsync(p.P + base, pv.data(), max_cells * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(p.E + base, ev.data(), max_cells * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(p.GS + base, gv.data(), max_cells * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(p.slot_sys + base, owner.data(), max_cells * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync((void*)(p.prm + sid), prm, sizeof(cs3d_params), hipMemcpyHostToDevice, c->stream));
+    SysMeta m;
+    std::memset(&m, 0, sizeof(m));
+    m.n = n_cells;
+    m.cap = max_cells;
+    m.base = (int32_t)base;
+    m.active = 1;
+    m.step = 0;
+    m.horizon = -1;
+    m.vbase = vreg;
+    m.lbase = vreg + vb;
+    m.vbudget = (int32_t)vb;
+    m.vcap = vcap;
+    m.dims[0] = m.dims[1] = m.dims[2] = 1;
+    for (int a = 0; a < 3; ++a) {  // minmaxpre sentinels (propagatebound.cu:161-166)
+        float lo = 1.0E8f, hi = -1.0E8f;
+        int32_t il, ih;
+        std::memcpy(&il, &lo, 4);
+        std::memcpy(&ih, &hi, 4);
+        m.ext[2 * a] = il >= 0 ? il : il ^ 0x7FFFFFFF;
+        m.ext[2 * a + 1] = ih >= 0 ? ih : ih ^ 0x7FFFFFFF;
+    }
+    cs3d_ranmar_init(&m.rng, prm->ranmar_ij, prm->ranmar_kl);
+    st = write_meta(c, sid, m);
+    if (st != CS3D_OK) return st;
+    drop_graphs(c, CS3D_ALL);
+    *out = sid;
+    return CS3D_OK;
+}
+
+extern "C" cs3d_status cs3d_system_add(cs3d_ctx* c, const cs3d_params* prm, int32_t n_cells, int32_t max_cells,
+                                       int64_t voxels, const float* x, const float* y, const float* z,
+                                       const float* vx, const float* vy, const float* vz, cs3d_sys* out) {
+    if (!x || !y || !z) return fail(CS3D_INVALID_ARG, "positions are required");
+    cs3d_sys s;
+    cs3d_status st = cs3d_system_add_raw(c, prm, n_cells, max_cells, voxels, &s);
+    if (st != CS3D_OK) return st;
+    const size_t bytes = (size_t)n_cells * kSlots * 4;
+    const int64_t nb = (int64_t)c->sys[s].base * kSlots;
+    const Pool& p = c->pool;
+    auto up = [&](float* d, const float* h) -> cs3d_status {
+        if (!h || bytes == 0) return CS3D_OK;
+        HIPCHK(hipMemcpyAsync(d + nb, h, bytes, hipMemcpyHostToDevice, c->stream));
+        return CS3D_OK;
+    };
+    if ((st = up(p.X, x)) || (st = up(p.Y, y)) || (st = up(p.Z, z)) || (st = up(p.VX, vx)) || (st = up(p.VY, vy)) ||
+        (st = up(p.VZ, vz))) {
+        cs3d_system_remove(c, s);
+        return st;
+    }
+    if ((st = run_startup(c, s)) != CS3D_OK) {
+        cs3d_system_remove(c, s);
+        return st;
+    }
+    *out = s;
+    return CS3D_OK;
+}
+
+extern "C" cs3d_status cs3d_system_remove(cs3d_ctx* c, cs3d_sys s) {
+    if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    SysHost& h = c->sys[s];
+    std::vector<int32_t> owner(h.cap, -1);
+    HIPCHK(hipMemcpy(c->pool.slot_sys + h.base, owner.data(), h.cap * 4, hipMemcpyHostToDevice));
+    SysMeta m;
+    std::memset(&m, 0, sizeof(m));
+    HIPCHK(hipMemcpy(c->pool.meta + s, &m, sizeof(m), hipMemcpyHostToDevice));
+    c->cells.release(h.base);
+    c->voxels.release(h.vreg);
+    drop_graphs(c, s);
+    h = SysHost();
+    int hi = 0;
+    for (int i = 0; i < (int)c->sys.size(); ++i)
+        if (c->sys[i].used) hi = i + 1;
+    c->nsys_high = hi;
+    return CS3D_OK;
+}
+
+extern "C" cs3d_status cs3d_system_set_horizon(cs3d_ctx* c, cs3d_sys s, int64_t horizon) {
+    if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
+    SysMeta m;
+    cs3d_status st = read_meta(c, s, m);
+    if (st != CS3D_OK) return st;
+    m.horizon = horizon;
+    return write_meta(c, s, m);
+}
+
+extern "C" cs3d_status cs3d_system_info(cs3d_ctx* c, cs3d_sys s, int32_t* n_cells, int64_t* step, uint32_t* status) {
+    if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
+    SysMeta m;
+    cs3d_status st = read_meta(c, s, m);
+    if (st != CS3D_OK) return st;
+    if (n_cells) *n_cells = m.n;
+    if (step) *step = m.step;
+    if (status) *status = m.status;
+    return CS3D_OK;
+}
+
+extern "C" cs3d_status cs3d_system_set_counts(cs3d_ctx* c, cs3d_sys s, int32_t n_cells, int64_t step) {
+    if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
+    if (n_cells < 0 || n_cells > c->sys[s].cap) return fail(CS3D_INVALID_ARG, "cell count out of range");
+    SysMeta m;
+    cs3d_status st = read_meta(c, s, m);
+    if (st != CS3D_OK) return st;
+    m.n = n_cells;
+    m.step = step;
+    return write_meta(c, s, m);
+}
+
+extern "C" size_t cs3d_field_bytes(cs3d_ctx* c, cs3d_sys s, cs3d_field f) {
+    if (!valid_sys(c, s)) return 0;
+    return field_loc(c, s, f).bytes;
+}
+
+extern "C" cs3d_status cs3d_get(cs3d_ctx* c, cs3d_sys s, cs3d_field f, void* host, size_t bytes) {
+    if (!valid_sys(c, s) || !host) return fail(CS3D_INVALID_ARG, "bad get arguments");
+    FieldLoc L = field_loc(c, s, f);
+    if (L.bytes == 0) return fail(CS3D_INVALID_ARG, "unknown field");
+    if (bytes > L.bytes) return fail(CS3D_INVALID_ARG, "requested more bytes than the field holds");
+    if (f == CS3D_F_GRID || f == CS3D_F_RANMAR) {
+        SysMeta m;
+        cs3d_status st = read_meta(c, s, m);
+        if (st != CS3D_OK) return st;
+        if (f == CS3D_F_GRID) {
+            int32_t g[8];
+            std::memcpy(g, m.gmin, 12);
+            g[3] = m.dims[0]; g[4] = m.dims[1]; g[5] = m.dims[2]; g[6] = m.vbudget; g[7] = m.vcap;
+            std::memcpy(host, g, bytes);
+        } else {
+            std::memcpy(host, &m.rng, bytes);
+        }
+        return CS3D_OK;
+    }
+    HIPCHK(hipMemcpyAsync(host, L.ptr, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CS3D_OK;
+}
+
+extern "C" cs3d_status cs3d_set(cs3d_ctx* c, cs3d_sys s, cs3d_field f, const void* host, size_t bytes) {
+    if (!valid_sys(c, s) || !host) return fail(CS3D_INVALID_ARG, "bad set arguments");
+    FieldLoc L = field_loc(c, s, f);
+    if (L.bytes == 0) return fail(CS3D_INVALID_ARG, "unknown field");
+    if (bytes > L.bytes) return fail(CS3D_INVALID_ARG, "more bytes than the field holds");
+    if (f == CS3D_F_GRID || f == CS3D_F_RANMAR) {
+        SysMeta m;
+        cs3d_status st = read_meta(c, s, m);
+        if (st != CS3D_OK) return st;
+        if (f == CS3D_F_GRID) {
+            if (bytes < 24) return fail(CS3D_INVALID_ARG, "grid needs origin and dims");
+            int32_t g[8];
+            std::memcpy(g, host, std::min<size_t>(bytes, 32));
+            std::memcpy(m.gmin, g, 12);
+            m.dims[0] = g[3]; m.dims[1] = g[4]; m.dims[2] = g[5];
+            if ((int64_t)g[3] * g[4] * g[5] > m.vbudget || g[3] < 1 || g[4] < 1 || g[5] < 1)
+                return fail(CS3D_INVALID_ARG, "grid dims exceed the system's voxel budget");
+        } else {
+            std::memcpy(&m.rng, host, bytes);
+        }
+        return write_meta(c, s, m);
+    }
+    HIPCHK(hipMemcpyAsync(L.ptr, host, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CS3D_OK;
+}
+
+extern "C" cs3d_status cs3d_set_graph(cs3d_ctx* c, int32_t enable) {
+    if (!c) return fail(CS3D_INVALID_ARG, "NULL context");
+    c->use_graph = enable != 0;
+    if (!c->use_graph) drop_graphs(c);
+    return CS3D_OK;
+}
+
+extern "C" cs3d_status cs3d_step(cs3d_ctx* c, cs3d_sys s, int32_t n_steps) {
+    if (!c) return fail(CS3D_INVALID_ARG, "NULL context");
+    if (s != CS3D_ALL && !valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
+    if (n_steps <= 0) return CS3D_OK;
+    const Range r = range_of(c, s);
+    if (r.slot_hi <= r.slot_lo || r.sys_hi <= r.sys_lo) return CS3D_OK;
+    const int vb = vox_blocks_of(c, s);
+    int done = 0;
+    if (c->use_graph && n_steps >= c->graph_steps) {
+        auto it = c->graphs.find(s);
+        if (it == c->graphs.end()) {
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < c->graph_steps; ++k) enqueue_step(c, r, vb);
+            HIPCHK(hipStreamEndCapture(c->stream, &g));
+            hipGraphExec_t ex = nullptr;
+            hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return fail(CS3D_HIP_ERROR, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+            it = c->graphs.emplace(s, std::make_pair(ex, c->graph_steps)).first;
+        }
+        const int per = it->second.second;
+        for (; done + per <= n_steps; done += per) HIPCHK(hipGraphLaunch(it->second.first, c->stream));
+    }
+    for (; done < n_steps; ++done) enqueue_step(c, r, vb);
+    HIPCHK(hipGetLastError());
+    return CS3D_OK;
+}
+
+// ---- per-kernel entry points -------------------------------------------------
+namespace {
+cs3d_status finish(cs3d_ctx* c) {
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CS3D_OK;
+}
+}  // namespace
+
+extern "C" cs3d_status cs3d_k_integrate(cs3d_ctx* c, cs3d_sys s) {
+    if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
+    launch_integrate(c->pool, range_of(c, s), I_KICK | I_PBOX, 0, c->stream);
+    return finish(c);
+}
+
+extern "C" cs3d_status cs3d_k_pressure_update(cs3d_ctx* c, cs3d_sys s, int64_t step) {
+    if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
+    launch_integrate(c->pool, range_of(c, s), I_PRESSURE, step, c->stream);
+    return finish(c);
+}
+
+extern "C" cs3d_status cs3d_k_con_force(cs3d_ctx* c, cs3d_sys s) {
+    if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
+    const Range r = range_of(c, s);
+    launch_integrate(c->pool, r, I_PBOX, 0, c->stream);
+    launch_con(c->pool, r, M_CON, false, c->stream);
+    return finish(c);
+}
+
+extern "C" cs3d_status cs3d_k_con_force_bonded(cs3d_ctx* c, cs3d_sys s) {
+    if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
+    launch_con(c->pool, range_of(c, s), M_BONDED, false, c->stream);
+    return finish(c);
+}
+
+extern "C" cs3d_status cs3d_k_dis_force(cs3d_ctx* c, cs3d_sys s, int32_t which) {
+    if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
+    const Range r = range_of(c, s);
+    launch_integrate(c->pool, r, I_PBOX, 0, c->stream);
+    launch_dis(c->pool, r, which ? M_DIS_VA : M_DIS_V, 0, which != 0, c->stream);
+    return finish(c);
+}
+
+extern "C" cs3d_status cs3d_k_velocity_ab(cs3d_ctx* c, cs3d_sys s) {
+    if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
+    launch_velocity_ab(c->pool, range_of(c, s), c->stream);
+    return finish(c);
+}
+
+extern "C" cs3d_status cs3d_k_com(cs3d_ctx* c, cs3d_sys s) {
+    if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
+    launch_dis(c->pool, range_of(c, s), M_COM, 0, false, c->stream);
+    return finish(c);
+}
+
+extern "C" cs3d_status cs3d_k_volumes(cs3d_ctx* c, cs3d_sys s, int64_t step) {
+    if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
+    launch_dis(c->pool, range_of(c, s), M_VOLUMES, step, false, c->stream);
+    return finish(c);
+}
+
+extern "C" cs3d_status cs3d_k_grid(cs3d_ctx* c, cs3d_sys s) {
+    if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
+    const Range r = range_of(c, s);
+    launch_integrate(c->pool, r, I_PBOX, 0, c->stream);
+    launch_set_rebuild(c->pool, r, 1, c->stream);
+    launch_grid(c->pool, r, vox_blocks_of(c, s), c->stream);
+    launch_set_rebuild(c->pool, r, 0, c->stream);
+    return finish(c);
+}
+
+extern "C" cs3d_status cs3d_k_divide(cs3d_ctx* c, cs3d_sys s, int32_t* n_events) {
+    if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
+    launch_divide(c->pool, range_of(c, s), M_INIT, c->stream);
+    cs3d_status st = finish(c);
+    if (st != CS3D_OK) return st;
+    if (n_events) {
+        SysMeta m;
+        st = read_meta(c, s, m);
+        if (st != CS3D_OK) return st;
+        *n_events = m.n_events;
+    }
+    return CS3D_OK;
+}

@@ -1,0 +1,256 @@
+/*
+ * cs3d.h - C ABI of the MI355X-native CellSim3D engine (libcs3d.so).
+ *
+ * The reference (SoftSimu/CellSim3D) has no library API: its hot path is a
+ * set of CUDA kernels launched from main() in src/GPUbounce.cu with raw
+ * device pointers.  Each entry point below names the reference code it
+ * replaces.  Conventions:
+ *   - plain C, no exceptions cross the boundary; every call returns a
+ *     cs3d_status and cs3d_last_error() holds a message for the calling thread;
+ *   - node arrays are SoA float32 with a 192-slot stride per cell
+ *     (x[cell*192 + node], nodes 180..191 are padding), exactly the layout of
+ *     d_X/d_Y/d_Z in src/GPUbounce.cu:365-373;
+ *   - the context owns all device memory; caller buffers are host memory and
+ *     are copied, never retained;
+ *   - a context is used from one host thread and runs on one HIP stream.
+ */
+#ifndef CS3D_H
+#define CS3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CS3D_NODES 180   /* real nodes per C180 cell */
+#define CS3D_SLOTS 192   /* node slots per cell (stride), GPUbounce.cu:365 */
+#define CS3D_FACES 92    /* 12 pentagons + 80 hexagons, GPUbounce.cu:1858-1886 */
+#define CS3D_ALL (-1)    /* cs3d_step target: every active system in one batch */
+
+typedef enum cs3d_status {
+    CS3D_OK = 0,
+    CS3D_INVALID_ARG = 1,
+    CS3D_OUT_OF_MEMORY = 2,
+    CS3D_CAPACITY = 3,       /* population or pool capacity exceeded (GPUbounce.cu:1264-1270 returns -69) */
+    CS3D_NAN_DETECTED = 4,   /* reference traps: propagate.cu:241-248, volume.cu:134-139 */
+    CS3D_LIST_OVERFLOW = 5,  /* voxel list / candidate list capacity exceeded */
+    CS3D_HIP_ERROR = 6,
+    CS3D_NO_DEVICE = 7,
+    CS3D_UNSUPPORTED = 8     /* e.g. add_rands = 1 (cuRAND stream, not reproduced) */
+} cs3d_status;
+
+/* Sticky per-system status bits (replace the reference's device traps). */
+#define CS3D_SYS_NAN              (1u << 0)  /* NaN node position (propagate.cu:241-248) */
+#define CS3D_SYS_VOXEL_OVERFLOW   (1u << 1)  /* > voxel_capacity cells in a voxel (propagatebound.cu:247-259 is unchecked) */
+#define CS3D_SYS_GRID_TOO_LARGE   (1u << 2)  /* Xdiv*Ydiv*Zdiv beyond the system's voxel budget */
+#define CS3D_SYS_CAPACITY         (1u << 3)  /* division past max_cells (GPUbounce.cu:1264-1270) */
+#define CS3D_SYS_ANGLE_NONFINITE  (1u << 4)  /* propagate.cu:48-52 */
+#define CS3D_SYS_VOLUME_NONFINITE (1u << 5)  /* volume.cu:134-139 */
+#define CS3D_SYS_BAD_NORMAL       (1u << 6)  /* postscriptinit.cu:215-219 */
+#define CS3D_SYS_NN_OVER_10       (1u << 7)  /* diagnostic: > 10 candidate cells for a node; the reference
+                                                writes past localNNs[10] there (propagate.cu:222,398-403) */
+#define CS3D_SYS_CAND_OVERFLOW    (1u << 8)  /* a cell touches more candidate cells or voxels than the engine's
+                                                per-cell lists hold */
+#define CS3D_SYS_FATAL_MASK       (0x17Fu)   /* every bit except the NN_OVER_10 diagnostic */
+
+/* Mesh topology and rest state shared by all cells.
+ * nn[k*192+i] (k=0..2) and sign[i]: C180NN, 0-based (GPUbounce.cu:1826-1833).
+ * faces[f*7+j]: C180_pentahexa rows; pentagons padded N1..N5,N1,N1, hexagons
+ *   N1..N6,N1 (GPUbounce.cu:1861-1886).
+ * R0[k*192+i]: rest length of bond (i, nn[k][i]) (GPUbounce.cu:1892-1908).
+ * theta0[3*i+{0,1,2}]: rest angles aij, ajk, aik of node i (GPUbounce.cu:909-938,
+ *   struct angles3 in inc/VectorFunctions.hpp:10-12). */
+typedef struct cs3d_mesh {
+    int32_t nn[3 * CS3D_SLOTS];
+    int32_t sign[CS3D_SLOTS];
+    int32_t faces[CS3D_FACES * 7];
+    float R0[3 * CS3D_SLOTS];
+    float theta0[3 * CS3D_SLOTS];
+} cs3d_mesh;
+
+/* Physical and numerical parameters of one system (tissue).  Names follow the
+ * inp.json keys read by read_json_params (GPUbounce.cu:1914-2247). */
+typedef struct cs3d_params {
+    float mass;                 /* core.particle_mass */
+    float repulsion_range;      /* core.repulsion_range */
+    float attraction_range;     /* core.attraction_range */
+    float repulsion_strength;   /* core.repulsion_strength */
+    float attraction_strength;  /* core.attraction_strength */
+    float youngs_mod;           /* core.Youngs_mod (contact and wall forces use this global value) */
+    float stiffness1;           /* core.stiffFactor1 * Youngs_mod (GPUbounce.cu:1953) */
+    float stiffness2;           /* stiffnessParams.softStiffFactor * Youngs_mod (GPUbounce.cu:2048) */
+    float viscotic_damping;     /* gamma_ext, inter-cell friction */
+    float internal_damping;     /* gamma_int, bond damping */
+    float gamma_visc;           /* gamma_o, drag */
+    float division_vol;         /* core.division_Vol */
+    float dt;                   /* core.time_interval */
+    float max_pressure;         /* core.maxPressure */
+    float min_pressure;         /* core.minPressure */
+    float growth_rate;          /* core.growth_rate (rMax) */
+    int32_t check_sphericity;   /* core.checkSphericity */
+    int32_t constrain_angles;   /* core.constrainAngles */
+    int32_t div_time_steps;     /* core.div_time_steps (Time_steps) */
+    int32_t non_div_time_steps; /* core.non_div_time_steps */
+    int32_t phase_count;        /* core.phase_count */
+    int32_t use_rigid_box;      /* boxParams.useRigidSimulationBox */
+    float box_max[3];           /* boxParams.box_len_x/y/z */
+    float thresh_dist;          /* walls.threshDist */
+    int32_t use_different_stiffnesses; /* stiffnessParams.useDifferentStiffnesses */
+    int32_t recalc_r0;          /* stiffnessParams.recalc_r0 */
+    int32_t daught_same_stiffness; /* stiffnessParams.daughtSameStiffness */
+    int32_t use_div_plane_basis;   /* divParams.useDivPlaneBasis */
+    float div_plane_basis[3];      /* divParams.divPlaneBasisX/Y/Z */
+    int32_t ranmar_ij, ranmar_kl;  /* RANMAR seeds (GPUbounce.cu:1777-1810: ij = rand_seed, kl = 100*rand_seed) */
+    int32_t do_pop_model;          /* population.doPopModel */
+    float max_pop;                 /* population.max_pop */
+    int32_t add_rands;             /* rand_params.add_rands: must be 0 (cuRAND is not reproduced) */
+    float angle_k;                 /* bending constant, 1000 in the reference (propagate.cu:335) */
+    float grid_dl;                 /* voxel edge DL, 2.9 in the reference (GPUbounce.cu:756) */
+    int32_t voxel_capacity;        /* cells per voxel list, 32 in the reference (propagatebound.cu:259) */
+    int32_t grid_interval;         /* rebuild bbox + voxel grid every k steps (1 = reference) */
+    int32_t daughter_com_fix;      /* 0 = reference: a new daughter's centre of mass stays 0 until the next
+                                      CenterOfMass; 1 = set it from the daughter's nodes at division */
+    int32_t reserved[7];
+} cs3d_params;
+
+/* Fields readable/writable with cs3d_get / cs3d_set.  Sizes are per system
+ * with n = max_cells of the system (node fields: n*192 floats). */
+typedef enum cs3d_field {
+    CS3D_F_X = 0, CS3D_F_Y, CS3D_F_Z,          /* positions d_X/d_Y/d_Z */
+    CS3D_F_VX, CS3D_F_VY, CS3D_F_VZ,           /* velocities d_velListX/Y/Z (end of step) */
+    CS3D_F_FCX, CS3D_F_FCY, CS3D_F_FCZ,        /* conservative force d_fConList */
+    CS3D_F_FDX, CS3D_F_FDY, CS3D_F_FDZ,        /* dissipative force d_fDisList */
+    CS3D_F_VAX, CS3D_F_VAY, CS3D_F_VAZ,        /* velocity after Integrate (read by the first DisForce) */
+    CS3D_F_CMX, CS3D_F_CMY, CS3D_F_CMZ,        /* per-cell centre of mass d_CMx/y/z */
+    CS3D_F_BBOX,                               /* 6 floats per cell: minx maxx miny maxy minz maxz */
+    CS3D_F_PRESSURE,                           /* d_pressList */
+    CS3D_F_STIFFNESS,                          /* d_Youngs_mod (per-cell spring constant) */
+    CS3D_F_VOLUME, CS3D_F_AREA,                /* d_volume, d_area */
+    CS3D_F_DIVFLAG,                            /* int8 per cell, d_cell_div */
+    CS3D_F_GRID,                               /* 8 x int32: float minx, miny, minz (bit patterns), xdiv, ydiv, zdiv,
+                                                  voxel budget, voxel capacity */
+    CS3D_F_VOXCOUNT,                           /* int32 per voxel (budget entries), d_NoofNNlist */
+    CS3D_F_VOXLIST,                            /* int32 budget*capacity, d_NNlist (each list ascending) */
+    CS3D_F_RANMAR,                             /* cs3d_ranmar_state */
+    CS3D_F_PATCHBOX,                           /* 12 patches x 6 floats per cell (engine-specific) */
+    CS3D_F_GROWTH_SCALE,                       /* per-cell multiplier of the pressure increment (default 1) */
+    CS3D_F__COUNT
+} cs3d_field;
+
+/* RANMAR generator state (src/marsaglia.h:50-52). */
+typedef struct cs3d_ranmar_state {
+    float u[98];
+    float c, cd, cm;
+    int32_t i97, j97;
+    int32_t ready;
+} cs3d_ranmar_state;
+
+typedef struct cs3d_ctx cs3d_ctx;
+typedef int32_t cs3d_sys;
+
+/* ---- diagnostics -------------------------------------------------------- */
+const char* cs3d_status_str(cs3d_status s);
+const char* cs3d_last_error(void);
+const char* cs3d_version(void);
+
+/* ---- host-side helpers (no device needed) -------------------------------- */
+/* Defaults of inp.json (the file shipped with the reference, inp.json:1-108). */
+cs3d_status cs3d_params_default(cs3d_params* p);
+/* Validation rules of read_json_params (GPUbounce.cu:2168-2244) that apply to the kernel path. */
+cs3d_status cs3d_params_check(const cs3d_params* p);
+/* Builds R0 and theta0 from the 180 rest positions x/y/z (one cell, as placed),
+ * replacing read_fullerene_nn's R0 loop (GPUbounce.cu:1892-1908) and the theta0
+ * loop (GPUbounce.cu:909-938). nn is C180_nn (3*192, 0-based), sign 180 ints,
+ * faces 92*7 ints. */
+cs3d_status cs3d_mesh_build(const int32_t* nn, const int32_t* sign, const int32_t* faces,
+                            const float* x, const float* y, const float* z, cs3d_mesh* out);
+/* RANMAR: rmarin (marsaglia.h:54-113) and ranmar (marsaglia.h:115-144). */
+cs3d_status cs3d_ranmar_init(cs3d_ranmar_state* st, int32_t ij, int32_t kl);
+cs3d_status cs3d_ranmar(cs3d_ranmar_state* st, float* out, int32_t n);
+
+/* ---- context ------------------------------------------------------------ */
+/* Allocates a pool of capacity_cells cell slots and voxel_budget voxels on
+ * `device`.  Replaces the cudaMalloc block of main (GPUbounce.cu:358-615). */
+cs3d_status cs3d_ctx_create(int device, const cs3d_mesh* mesh, int64_t capacity_cells,
+                            int64_t voxel_budget, cs3d_ctx** out);
+cs3d_status cs3d_ctx_destroy(cs3d_ctx* ctx);
+cs3d_status cs3d_sync(cs3d_ctx* ctx);
+
+/* ---- systems ------------------------------------------------------------ */
+/* Adds a system of n_cells cells (room for max_cells), uploads positions
+ * (n_cells*192 per component; velocities nullable = 0), sets pressures to
+ * min_pressure and stiffness to stiffness1, then runs the reference's start-up
+ * sequence: CenterOfMass, bounding_boxes, minmaxpre/post, makeNNlist
+ * (GPUbounce.cu:739-763), initial CalculateConForce and CalculateDisForce
+ * (GPUbounce.cu:978-1004).  voxels = voxel budget of this system (0 = auto). */
+cs3d_status cs3d_system_add(cs3d_ctx* ctx, const cs3d_params* p, int32_t n_cells, int32_t max_cells,
+                            int64_t voxels, const float* x, const float* y, const float* z,
+                            const float* vx, const float* vy, const float* vz, cs3d_sys* out);
+/* Adds a system without running the start-up sequence (state injection for
+ * per-kernel parity); fields are then set with cs3d_set. */
+cs3d_status cs3d_system_add_raw(cs3d_ctx* ctx, const cs3d_params* p, int32_t n_cells, int32_t max_cells,
+                                int64_t voxels, cs3d_sys* out);
+cs3d_status cs3d_system_remove(cs3d_ctx* ctx, cs3d_sys sys);
+/* Limits a system to `horizon` total steps (stepping stops there; -1 = none). */
+cs3d_status cs3d_system_set_horizon(cs3d_ctx* ctx, cs3d_sys sys, int64_t horizon);
+cs3d_status cs3d_system_info(cs3d_ctx* ctx, cs3d_sys sys, int32_t* n_cells, int64_t* step, uint32_t* status);
+/* Sets the live cell count and step counter (state injection). */
+cs3d_status cs3d_system_set_counts(cs3d_ctx* ctx, cs3d_sys sys, int32_t n_cells, int64_t step);
+cs3d_status cs3d_get(cs3d_ctx* ctx, cs3d_sys sys, cs3d_field f, void* host, size_t bytes);
+cs3d_status cs3d_set(cs3d_ctx* ctx, cs3d_sys sys, cs3d_field f, const void* host, size_t bytes);
+size_t cs3d_field_bytes(cs3d_ctx* ctx, cs3d_sys sys, cs3d_field f);
+
+/* ---- stepping ----------------------------------------------------------- */
+/* Runs n_steps of the reference step loop (GPUbounce.cu:1055-1506) on one
+ * system (plain per-system path) or on every active system (sys = CS3D_ALL,
+ * batched path: one launch per kernel for all systems, captured in a HIP
+ * graph).  Asynchronous: returns after enqueueing; cs3d_sync waits. */
+cs3d_status cs3d_step(cs3d_ctx* ctx, cs3d_sys sys, int32_t n_steps);
+/* Enables (1) or disables (0) HIP graph capture for cs3d_step. */
+cs3d_status cs3d_set_graph(cs3d_ctx* ctx, int32_t enable);
+
+/* ---- per-kernel entry points (state injection / parity) ------------------
+ * Each runs one reference kernel (or fixed kernel group) on the device state
+ * of one system and returns after it completes. */
+/* Integrate + ForwardTime (propagate.cu:681-712, 738-763). */
+cs3d_status cs3d_k_integrate(cs3d_ctx* ctx, cs3d_sys sys);
+/* PressureUpdate (PressureKernels.cu:4-41) for step `step` with rGrowth from
+ * the population model (GPUbounce.cu:1077-1088). */
+cs3d_status cs3d_k_pressure_update(cs3d_ctx* ctx, cs3d_sys sys, int64_t step);
+/* CalculateConForce (propagate.cu:183-520) -> FC. */
+cs3d_status cs3d_k_con_force(cs3d_ctx* ctx, cs3d_sys sys);
+/* Spring + pressure (+ angle) part of CalculateConForce only (propagate.cu:253-337) -> FC. */
+cs3d_status cs3d_k_con_force_bonded(cs3d_ctx* ctx, cs3d_sys sys);
+/* CalculateDisForce (propagate.cu:523-665) with velocities `which` (0 = V, 1 = VA) -> FD. */
+cs3d_status cs3d_k_dis_force(cs3d_ctx* ctx, cs3d_sys sys, int32_t which);
+/* VelocityUpdateA then VelocityUpdateB (propagate.cu:714-736) on V. */
+cs3d_status cs3d_k_velocity_ab(cs3d_ctx* ctx, cs3d_sys sys);
+/* CenterOfMass (centermass.cu:8-50). */
+cs3d_status cs3d_k_com(cs3d_ctx* ctx, cs3d_sys sys);
+/* volumes (volume.cu:5-163) for step `step`: VOLUME, AREA, DIVFLAG. */
+cs3d_status cs3d_k_volumes(cs3d_ctx* ctx, cs3d_sys sys, int64_t step);
+/* bounding_boxes + minmaxpre/post + grid dims + makeNNlist
+ * (propagatebound.cu:10-323, GPUbounce.cu:1312-1338) with ascending voxel lists. */
+cs3d_status cs3d_k_grid(cs3d_ctx* ctx, cs3d_sys sys);
+/* Division phase: count_and_get_div, RANMAR normals, cell_division,
+ * PressureReset and stiffness bookkeeping (GPUbounce.cu:1222-1286,
+ * postscriptinit.cu:148-256, PressureKernels.cu:45-65). Returns events via
+ * n_events (nullable). */
+cs3d_status cs3d_k_divide(cs3d_ctx* ctx, cs3d_sys sys, int32_t* n_events);
+
+/* ---- trajectory (binary format of GPUbounce.cu:1007-1018, 2446-2477) ----- */
+typedef struct cs3d_traj cs3d_traj;
+cs3d_status cs3d_traj_open(const char* path, int32_t max_cells, int32_t variable_stiffness,
+                           int32_t max_frames, cs3d_traj** out);
+/* Appends one frame of n_cells cells (x/y/z with 192 stride; types nullable). */
+cs3d_status cs3d_traj_append(cs3d_traj* t, int32_t step, int32_t frame, int32_t n_cells,
+                             const float* x, const float* y, const float* z, const int32_t* types);
+/* Rewrites the header's first int with final_cells (GPUbounce.cu:1508-1511) and closes. */
+cs3d_status cs3d_traj_close(cs3d_traj* t, int32_t final_cells);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CS3D_H */
