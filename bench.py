@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Headline benchmark: node-updates/s of the C180 cell-mechanics step at 2000
+cells on one MI355X.
+
+Workload (single tissue): 2000 unit-radius C180 cells (360,000 nodes) on a
+seeded jittered simple-cubic lattice at volume fraction 0.40, FP32, grid
+rebuilt every 10 steps, division off.  A step is one full pass of the step loop
+(integrate, conservative + dissipative forces, velocity updates, second
+dissipative force, centre of mass, grid rebuild on its schedule) over all
+cells; the state is device-resident before the timed region starts.
+
+Prints ONE JSON line.  `roofline` covers the dominant kernel (the fused
+contact/friction kernel), timed live with HIP events on the engine's stream;
+`cpu_baseline` is the CPU oracle (a C++/OpenMP restatement of the reference
+kernels) timed on this host on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# algorithmic HBM bytes per node and launch of each phase (FP32 SoA):
+#   integrate: reads x, v, F_con, F_dis (48 B), writes x, v_after_integrate (24 B)
+#   con_dis:   reads x, v_after_integrate (24 B), writes F_con, v (24 B)
+#   dis_post:  reads x, v (24 B), writes F_dis (12 B)
+ALG_BYTES = {"integrate": 72.0, "con_dis": 48.0, "dis_post": 36.0}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def load_traffic():
+    """HBM bytes per launch measured with rocprofv3 counters (committed summary)."""
+    path = os.path.join(ROOT, "profiles", "traffic_single_tissue_2000.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return None
+
+
+def cpu_baseline(cfg, mesh_tables, rest, budget_s=20.0):
+    from oracle import oracle as O
+    om = O.build_mesh(mesh_tables, rest)
+    n = cfg["n"]
+    sim = O.Sim(cfg["params"], om, cfg["X"], cfg["Y"], cfg["Z"], n, max_cells=n, budget=1 << 16)
+    sim.init()
+    sim.step(2)  # warm-up
+    t0 = time.perf_counter()
+    sim.step(1)
+    one = time.perf_counter() - t0
+    k = max(1, min(200, int(budget_s / max(one, 1e-6))))
+    t0 = time.perf_counter()
+    sim.step(k)
+    dt = time.perf_counter() - t0
+    return {
+        "value": n * 180.0 * k / dt,
+        "unit": "node-updates/s",
+        "cores": O.get_threads(),
+        "kind": "port",
+        "sample": "oracle (C++/OpenMP restatement of the reference kernels), same 2000-cell input, "
+                  "%d steps after 2 warm-up steps (%.1f s)" % (k, dt),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--cells", type=int, default=2000)
+    ap.add_argument("--phi", type=float, default=0.40)
+    ap.add_argument("--grid-interval", type=int, default=10)
+    ap.add_argument("--timed-steps", type=int, default=100, help="steps with per-phase HIP events")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="also run the known-size copy kernel (for HBM counter scaling)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every system of this engine runs on a single MI355X "
+              "(multi-GPU sharding is a design note, not built); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from cellsim3d_amd import abi, configs, engine, mesh
+
+    cfg = configs.single_tissue(args.cells, args.phi, seed=1, grid_interval=args.grid_interval,
+                                steps=args.steps + args.warmup)
+    tables = mesh.tables(cfg["mesh"])
+    rest = configs.rest_positions(cfg["mesh"])
+    gm = engine.build_mesh(tables, rest)
+    n = cfg["n"]
+    log("bench: %d cells, phi %.2f, grid every %d steps" % (n, args.phi, args.grid_interval))
+    ctx = engine.Context(gm, capacity_cells=n, voxel_budget=1 << 24)
+    s = ctx.add_system(cfg["params"], cfg["X"], cfg["Y"], cfg["Z"], n, max_cells=n, voxels=1 << 18)
+    ctx.step(args.warmup, s)
+    ctx.sync()
+    log("bench: warm-up done")
+    t0 = time.perf_counter()
+    ctx.step(args.steps, s)
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    n_now, step_now, status = s.info()
+    if status & abi.SYS_FATAL_MASK:
+        raise SystemExit("system status 0x%x after the timed run" % status)
+    value = n * 180.0 * args.steps / elapsed
+    log("bench: %d steps in %.4f s -> %.4g node-updates/s" % (args.steps, elapsed, value))
+
+    # per-phase timing with HIP events on the engine's stream (eager launches)
+    ph = ctx.step_timed(max(1, args.timed_steps), s) / max(1, args.timed_steps)
+    phases = {name: float(ms) for name, ms in zip(engine.PHASE_NAMES, ph)}
+    dom = max(("con_dis", "dis_post", "integrate"), key=lambda k: phases[k])
+    dom_ms = phases[dom]
+    alg = ALG_BYTES[dom] * n * 180
+    achieved = alg / (dom_ms * 1e-3) / 1e9
+    traffic = None
+    tr = load_traffic()
+    if tr and dom in tr.get("bytes_per_launch", {}):
+        traffic = tr["bytes_per_launch"][dom]
+    roofline = {
+        "kernel": {"con_dis": "k_con_dis", "dis_post": "k_dis_post", "integrate": "k_integrate"}[dom],
+        "bound": "hbm",
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 5),
+        "traffic": traffic,
+        "avg_launch_ms": round(dom_ms, 5),
+        "alg_bytes_per_launch": alg,
+    }
+    log("bench: phases ms/step %s" % json.dumps({k: round(v, 4) for k, v in phases.items()}))
+
+    calib = None
+    if args.calib_copy_mb > 0:
+        nbytes = args.calib_copy_mb << 20
+        cms = ctx.copy_calibrate(nbytes, reps=5)
+        calib = {"bytes_read": nbytes, "bytes_written": nbytes, "avg_ms": cms,
+                 "GBs": round(2 * nbytes / (cms * 1e-3) / 1e9, 1)}
+        log("bench: copy kernel %d MiB: %.4f ms (%.0f GB/s read+write)" % (args.calib_copy_mb, cms, calib["GBs"]))
+    cpu = None
+    if not args.no_cpu:
+        log("bench: CPU oracle baseline (~%.0f s)" % args.cpu_seconds)
+        cpu = cpu_baseline(cfg, tables, rest, args.cpu_seconds)
+    ms_per_step = elapsed / args.steps * 1e3
+    out = {
+        "metric": "node-updates/s at 2k cells",
+        "value": value,
+        "unit": "node-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "single tissue: %d C180 cells (%d nodes), volume fraction %.2f, FP32, "
+                        "grid rebuilt every %d steps, division off" % (n, n * 180, args.phi, args.grid_interval),
+            "cells": n,
+            "nodes": n * 180,
+            "grid_interval": args.grid_interval,
+        },
+        "phases_ms_per_step": {k: round(v, 5) for k, v in phases.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if calib:
+        out["copy_calibration"] = calib
+    if cpu:
+        out["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

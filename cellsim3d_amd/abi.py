@@ -28,7 +28,7 @@ SYS_FATAL_MASK = 0x17F
 # cs3d_field
 (F_X, F_Y, F_Z, F_VX, F_VY, F_VZ, F_FCX, F_FCY, F_FCZ, F_FDX, F_FDY, F_FDZ, F_VAX, F_VAY, F_VAZ,
  F_CMX, F_CMY, F_CMZ, F_BBOX, F_PRESSURE, F_STIFFNESS, F_VOLUME, F_AREA, F_DIVFLAG, F_GRID,
- F_VOXCOUNT, F_VOXLIST, F_RANMAR, F_PATCHBOX, F_GROWTH_SCALE, F__COUNT) = range(31)
+ F_VOXCOUNT, F_VOXLIST, F_RANMAR, F_PAIRCOUNT, F_PAIRS, F_GROWTH_SCALE, F__COUNT) = range(32)
 
 NODE_FIELDS = {F_X, F_Y, F_Z, F_VX, F_VY, F_VZ, F_FCX, F_FCY, F_FCZ, F_FDX, F_FDY, F_FDZ, F_VAX, F_VAY, F_VAZ}
 CELL_FIELDS = {F_CMX, F_CMY, F_CMZ, F_PRESSURE, F_STIFFNESS, F_VOLUME, F_AREA, F_GROWTH_SCALE}
@@ -85,7 +85,8 @@ class Params(C.Structure):
         ("voxel_capacity", C.c_int32),
         ("grid_interval", C.c_int32),
         ("daughter_com_fix", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("pair_skin", C.c_float),
+        ("reserved", C.c_int32 * 6),
     ]
 
     def copy(self):
@@ -160,4 +161,5 @@ def default_params():
     p.voxel_capacity = 32
     p.grid_interval = 1
     p.daughter_com_fix = 0
+    p.pair_skin = 0.05
     return p

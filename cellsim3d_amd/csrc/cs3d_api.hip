@@ -15,12 +15,15 @@
 
 namespace cs3d {
 void launch_integrate(const Pool& p, const Range& r, int flags, int64_t step_override, hipStream_t st);
+void launch_search(const Pool& p, const Range& r, int always, hipStream_t st);
 void launch_con(const Pool& p, const Range& r, int mode, bool use_va, hipStream_t st);
-void launch_dis(const Pool& p, const Range& r, int mode, int64_t step_override, bool use_va, hipStream_t st);
+void launch_dis(const Pool& p, const Range& r, int mode, int64_t step_override, int clear_bins, bool use_va,
+                hipStream_t st);
 void launch_divide(const Pool& p, const Range& r, int mode, hipStream_t st);
 void launch_grid(const Pool& p, const Range& r, int vox_blocks, hipStream_t st);
 void launch_velocity_ab(const Pool& p, const Range& r, hipStream_t st);
-void launch_set_rebuild(const Pool& p, const Range& r, int value, hipStream_t st);
+void launch_set_flags(const Pool& p, const Range& r, int rebuild, int search, hipStream_t st);
+void launch_copy_calibrate(const void* src, void* dst, int64_t n4, hipStream_t st);
 }  // namespace cs3d
 
 using namespace cs3d;
@@ -32,8 +35,6 @@ using namespace cs3d;
     } while (0)
 
 namespace {
-
-enum { I_KICK = 1, I_PRESSURE = 2, I_PBOX = 4 };
 
 struct Block {
     int64_t off, len;
@@ -67,6 +68,8 @@ struct SysHost {
     int32_t base = 0, cap = 0;
     int64_t vreg = 0;  // voxel region offset
     int32_t vbudget = 0, vcap = 0;
+    int64_t breg = 0;  // search-bin region offset
+    int32_t bbudget = 0;
     cs3d_params prm;
 };
 
@@ -77,9 +80,10 @@ struct cs3d_ctx {
     hipStream_t stream = nullptr;
     int64_t cap_cells = 0;
     int64_t vox_total = 0;
+    int64_t bin_total = 0;
     Pool pool;
     std::vector<void*> allocs;
-    RangeAlloc cells, voxels;
+    RangeAlloc cells, voxels, bins;
     std::vector<SysHost> sys;
     int nsys_high = 0;
     bool use_graph = true;
@@ -156,9 +160,11 @@ cs3d_status write_meta(cs3d_ctx* c, cs3d_sys s, const SysMeta& m) {
 
 void enqueue_step(cs3d_ctx* c, const Range& r, int vb) {
     const Pool& p = c->pool;
-    launch_integrate(p, r, I_KICK | I_PRESSURE | I_PBOX, -1, c->stream);
+    launch_integrate(p, r, I_KICK | I_PRESSURE | I_DISP, -1, c->stream);
+    launch_integrate(p, r, I_BINS | I_IFSEARCH, -1, c->stream);
+    launch_search(p, r, 0, c->stream);
     launch_con(p, r, M_STEP, true, c->stream);
-    launch_dis(p, r, M_STEP, -1, false, c->stream);
+    launch_dis(p, r, M_STEP, -1, 1, false, c->stream);
     launch_divide(p, r, M_STEP, c->stream);
     launch_grid(p, r, vb, c->stream);
 }
@@ -168,14 +174,16 @@ cs3d_status run_startup(cs3d_ctx* c, cs3d_sys s) {
     Range r = range_of(c, s);
     const int vb = vox_blocks_of(c, s);
     // CenterOfMass, bounding_boxes, minmaxpre/post, makeNNlist (GPUbounce.cu:739-763)
-    launch_dis(p, r, M_COM, 0, false, c->stream);
-    launch_integrate(p, r, I_PBOX, 0, c->stream);
-    launch_set_rebuild(p, r, 1, c->stream);
+    launch_dis(p, r, M_COM, 0, 0, false, c->stream);
+    launch_set_flags(p, r, 1, -1, c->stream);
     launch_grid(p, r, vb, c->stream);
-    launch_set_rebuild(p, r, 0, c->stream);
+    launch_set_flags(p, r, 0, -1, c->stream);
     // initial CalculateConForce and CalculateDisForce (GPUbounce.cu:978-1004)
+    launch_integrate(p, r, I_BINS, 0, c->stream);
+    launch_search(p, r, 1, c->stream);
     launch_con(p, r, M_INIT, false, c->stream);
-    launch_dis(p, r, M_DIS_V, 0, false, c->stream);
+    launch_dis(p, r, M_DIS_V, 0, 1, false, c->stream);
+    launch_set_flags(p, r, -1, 0, c->stream);  // candidate lists are current
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     return CS3D_OK;
@@ -210,7 +218,8 @@ FieldLoc field_loc(cs3d_ctx* c, cs3d_sys s, cs3d_field f) {
         case CS3D_F_DIVFLAG: L.ptr = p.FLAG + h.base; L.bytes = h.cap; break;
         case CS3D_F_VOXCOUNT: L.ptr = p.vox + h.vreg; L.bytes = (size_t)h.vbudget * 4; break;
         case CS3D_F_VOXLIST: L.ptr = p.vox + h.vreg + h.vbudget; L.bytes = (size_t)h.vbudget * h.vcap * 4; break;
-        case CS3D_F_PATCHBOX: L.ptr = p.PBOX + (int64_t)h.base * kPBox; L.bytes = (size_t)h.cap * kPBox * 4; break;
+        case CS3D_F_PAIRCOUNT: L.ptr = p.NPAIR + nb; L.bytes = nbytes; break;
+        case CS3D_F_PAIRS: L.ptr = p.PAIRS + (int64_t)h.base * kPMax * kSlots; L.bytes = (size_t)h.cap * kPMax * kSlots * 4; break;
         case CS3D_F_GROWTH_SCALE: L.ptr = p.GS + h.base; L.bytes = h.cap * 4; break;
         case CS3D_F_GRID: L.bytes = 32; break;
         case CS3D_F_RANMAR: L.bytes = sizeof(cs3d_ranmar_state); break;
@@ -222,8 +231,9 @@ FieldLoc field_loc(cs3d_ctx* c, cs3d_sys s, cs3d_field f) {
 }  // namespace
 
 extern "C" cs3d_status cs3d_ctx_create(int device, const cs3d_mesh* mesh, int64_t capacity_cells, int64_t voxel_budget,
-                                       cs3d_ctx** out) {
-    if (!mesh || !out || capacity_cells <= 0 || capacity_cells > (1ll << 30) / kSlots || voxel_budget <= 0)
+                                       int64_t bin_budget, cs3d_ctx** out) {
+    if (!mesh || !out || capacity_cells <= 0 || capacity_cells > (1ll << 30) / kSlots || voxel_budget <= 0 ||
+        bin_budget <= 0)
         return fail(CS3D_INVALID_ARG, "bad context arguments");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(CS3D_NO_DEVICE, "no HIP device visible");
@@ -257,8 +267,10 @@ extern "C" cs3d_status cs3d_ctx_create(int device, const cs3d_mesh* mesh, int64_
     c->device = device;
     c->cap_cells = capacity_cells;
     c->vox_total = voxel_budget;
+    c->bin_total = bin_budget;
     c->cells.total = capacity_cells;
     c->voxels.total = voxel_budget;
+    c->bins.total = bin_budget;
     c->sys.resize(kMaxSys);
     auto cleanup = [&](cs3d_status st) {
         cs3d_ctx_destroy(c);
@@ -279,9 +291,14 @@ extern "C" cs3d_status cs3d_ctx_create(int device, const cs3d_mesh* mesh, int64_
         if ((st = dalloc(c, *a, capacity_cells)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.FLAG, capacity_cells)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.BBOX, capacity_cells * 6)) != CS3D_OK) return cleanup(st);
-    if ((st = dalloc(c, p.PBOX, capacity_cells * kPBox)) != CS3D_OK) return cleanup(st);
-    if ((st = dalloc(c, p.NCL, capacity_cells * kNCMax)) != CS3D_OK) return cleanup(st);
-    if ((st = dalloc(c, p.NCN, capacity_cells)) != CS3D_OK) return cleanup(st);
+    if ((st = dalloc(c, p.PAIRS, capacity_cells * kPMax * kSlots)) != CS3D_OK) return cleanup(st);
+    if ((st = dalloc(c, p.NPAIR, capacity_cells * kSlots)) != CS3D_OK) return cleanup(st);
+    if ((st = dalloc(c, p.CAND, capacity_cells * kPMax * kSlots)) != CS3D_OK) return cleanup(st);
+    if ((st = dalloc(c, p.NCAND, capacity_cells * kSlots)) != CS3D_OK) return cleanup(st);
+    if ((st = dalloc(c, p.XBX, ns)) != CS3D_OK) return cleanup(st);
+    if ((st = dalloc(c, p.XBY, ns)) != CS3D_OK) return cleanup(st);
+    if ((st = dalloc(c, p.XBZ, ns)) != CS3D_OK) return cleanup(st);
+    if ((st = dalloc(c, p.bins, bin_budget)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.EVL, capacity_cells)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.slot_sys, capacity_cells)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.meta, kMaxSys)) != CS3D_OK) return cleanup(st);
@@ -337,6 +354,18 @@ extern "C" cs3d_status cs3d_system_add_raw(cs3d_ctx* c, const cs3d_params* prm, 
         c->cells.release(base);
         return fail(CS3D_CAPACITY, "voxel pool exhausted");
     }
+    // search bins: enough for a tissue at moderate density; coarser bins beyond that
+    // (counts, then float4 entries: 1 + 4 * kBCap words per bin; entries 16-byte aligned)
+    int64_t bb = std::max<int64_t>(16384, 640 * (int64_t)max_cells);
+    bb = std::min<int64_t>(bb, c->bin_total / (1 + 4 * kBCap));
+    bb = bb / 4 * 4;
+    int64_t breg = 0;
+    while (bb >= 4096 && !c->bins.alloc(bb * (1 + 4 * kBCap) + 4, breg)) bb /= 2;
+    if (bb < 4096) {
+        c->cells.release(base);
+        c->voxels.release(vreg);
+        return fail(CS3D_CAPACITY, "search-bin pool exhausted");
+    }
     SysHost& h = c->sys[sid];
     h.used = true;
     h.base = (int32_t)base;
@@ -344,6 +373,8 @@ extern "C" cs3d_status cs3d_system_add_raw(cs3d_ctx* c, const cs3d_params* prm, 
     h.vreg = vreg;
     h.vbudget = (int32_t)vb;
     h.vcap = vcap;
+    h.breg = breg;
+    h.bbudget = (int32_t)bb;
     h.prm = *prm;
     c->nsys_high = std::max(c->nsys_high, sid + 1);
     HIPCHK(hipSetDevice(c->device));
@@ -355,9 +386,9 @@ extern "C" cs3d_status cs3d_system_add_raw(cs3d_ctx* c, const cs3d_params* prm, 
     for (float* a : cellz) HIPCHK(hipMemsetAsync(a + base, 0, (size_t)max_cells * 4, c->stream));
     HIPCHK(hipMemsetAsync(p.FLAG + base, 0, max_cells, c->stream));
     HIPCHK(hipMemsetAsync(p.BBOX + base * 6, 0, (size_t)max_cells * 24, c->stream));
-    HIPCHK(hipMemsetAsync(p.PBOX + base * kPBox, 0, (size_t)max_cells * kPBox * 4, c->stream));
-    HIPCHK(hipMemsetAsync(p.NCN + base, 0, (size_t)max_cells * 4, c->stream));
+    HIPCHK(hipMemsetAsync(p.NPAIR + nb, 0, (size_t)max_cells * kSlots * 4, c->stream));
     HIPCHK(hipMemsetAsync(p.vox + vreg, 0, (size_t)vb * (1 + vcap) * 4, c->stream));
+    HIPCHK(hipMemsetAsync(p.bins + breg, 0, (size_t)bb * 4, c->stream));
     std::vector<float> pv(max_cells, prm->min_pressure), ev(max_cells, prm->stiffness1), gv(max_cells, 1.0f);
     std::vector<int32_t> owner(max_cells, sid);
     HIPCHK(hipMemcpyAsync(p.P + base, pv.data(), max_cells * 4, hipMemcpyHostToDevice, c->stream));
@@ -378,6 +409,12 @@ extern "C" cs3d_status cs3d_system_add_raw(cs3d_ctx* c, const cs3d_params* prm, 
     m.vbudget = (int32_t)vb;
     m.vcap = vcap;
     m.dims[0] = m.dims[1] = m.dims[2] = 1;
+    m.bbudget = (int32_t)bb;
+    m.bbase = breg;
+    m.ebase = (breg + bb + 3) / 4 * 4;
+    m.bh = prm->attraction_range > 0 ? prm->attraction_range * 1.001f : 1.0f;
+    m.bdims[0] = m.bdims[1] = m.bdims[2] = 1;
+    m.need_search = 1;
     for (int a = 0; a < 3; ++a) {  // minmaxpre sentinels (propagatebound.cu:161-166)
         float lo = 1.0E8f, hi = -1.0E8f;
         int32_t il, ih;
@@ -433,6 +470,7 @@ extern "C" cs3d_status cs3d_system_remove(cs3d_ctx* c, cs3d_sys s) {
     HIPCHK(hipMemcpy(c->pool.meta + s, &m, sizeof(m), hipMemcpyHostToDevice));
     c->cells.release(h.base);
     c->voxels.release(h.vreg);
+    c->bins.release(h.breg);
     drop_graphs(c, s);
     h = SysHost();
     int hi = 0;
@@ -526,6 +564,13 @@ extern "C" cs3d_status cs3d_set(cs3d_ctx* c, cs3d_sys s, cs3d_field f, const voi
     }
     HIPCHK(hipMemcpyAsync(L.ptr, host, bytes, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    if (f == CS3D_F_X || f == CS3D_F_Y || f == CS3D_F_Z || f == CS3D_F_BBOX) {  // stored candidates are stale
+        SysMeta m;
+        cs3d_status st = read_meta(c, s, m);
+        if (st != CS3D_OK) return st;
+        m.need_search = 1;
+        return write_meta(c, s, m);
+    }
     return CS3D_OK;
 }
 
@@ -534,6 +579,49 @@ extern "C" cs3d_status cs3d_set_graph(cs3d_ctx* c, int32_t enable) {
     c->use_graph = enable != 0;
     if (!c->use_graph) drop_graphs(c);
     return CS3D_OK;
+}
+
+extern "C" cs3d_status cs3d_set_graph_steps(cs3d_ctx* c, int32_t steps) {
+    if (!c || steps < 1 || steps > 1000) return fail(CS3D_INVALID_ARG, "graph steps must be in [1, 1000]");
+    c->graph_steps = steps;
+    drop_graphs(c);
+    return CS3D_OK;
+}
+
+extern "C" cs3d_status cs3d_step_timed(cs3d_ctx* c, cs3d_sys s, int32_t n_steps, double* phase_ms) {
+    if (!c || !phase_ms) return fail(CS3D_INVALID_ARG, "NULL argument");
+    if (s != CS3D_ALL && !valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
+    const Range r = range_of(c, s);
+    if (r.slot_hi <= r.slot_lo || r.sys_hi <= r.sys_lo) return CS3D_OK;
+    const int vb = vox_blocks_of(c, s);
+    const Pool& p = c->pool;
+    hipEvent_t ev[CS3D_PHASES + 1];
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    cs3d_status st = CS3D_OK;
+    for (int k = 0; k < n_steps && st == CS3D_OK; ++k) {
+        (void)hipEventRecord(ev[0], c->stream);
+        launch_integrate(p, r, I_KICK | I_PRESSURE | I_DISP, -1, c->stream);
+        launch_integrate(p, r, I_BINS | I_IFSEARCH, -1, c->stream);
+        launch_search(p, r, 0, c->stream);
+        (void)hipEventRecord(ev[1], c->stream);
+        launch_con(p, r, M_STEP, true, c->stream);
+        (void)hipEventRecord(ev[2], c->stream);
+        launch_dis(p, r, M_STEP, -1, 1, false, c->stream);
+        (void)hipEventRecord(ev[3], c->stream);
+        launch_divide(p, r, M_STEP, c->stream);
+        (void)hipEventRecord(ev[4], c->stream);
+        launch_grid(p, r, vb, c->stream);
+        (void)hipEventRecord(ev[5], c->stream);
+        hipError_t e = hipEventSynchronize(ev[5]);
+        if (e != hipSuccess) { st = fail(CS3D_HIP_ERROR, hipGetErrorString(e)); break; }
+        for (int q = 0; q < CS3D_PHASES; ++q) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, ev[q], ev[q + 1]);
+            phase_ms[q] += ms;
+        }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return st;
 }
 
 extern "C" cs3d_status cs3d_step(cs3d_ctx* c, cs3d_sys s, int32_t n_steps) {
@@ -576,7 +664,9 @@ cs3d_status finish(cs3d_ctx* c) {
 
 extern "C" cs3d_status cs3d_k_integrate(cs3d_ctx* c, cs3d_sys s) {
     if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
-    launch_integrate(c->pool, range_of(c, s), I_KICK | I_PBOX, 0, c->stream);
+    const Range r = range_of(c, s);
+    launch_integrate(c->pool, r, I_KICK, 0, c->stream);
+    launch_set_flags(c->pool, r, -1, 1, c->stream);
     return finish(c);
 }
 
@@ -589,8 +679,11 @@ extern "C" cs3d_status cs3d_k_pressure_update(cs3d_ctx* c, cs3d_sys s, int64_t s
 extern "C" cs3d_status cs3d_k_con_force(cs3d_ctx* c, cs3d_sys s) {
     if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
     const Range r = range_of(c, s);
-    launch_integrate(c->pool, r, I_PBOX, 0, c->stream);
+    launch_integrate(c->pool, r, I_BINS, 0, c->stream);
+    launch_search(c->pool, r, 1, c->stream);
     launch_con(c->pool, r, M_CON, false, c->stream);
+    launch_integrate(c->pool, r, I_CLEAR, 0, c->stream);
+    launch_set_flags(c->pool, r, -1, 1, c->stream);
     return finish(c);
 }
 
@@ -603,8 +696,11 @@ extern "C" cs3d_status cs3d_k_con_force_bonded(cs3d_ctx* c, cs3d_sys s) {
 extern "C" cs3d_status cs3d_k_dis_force(cs3d_ctx* c, cs3d_sys s, int32_t which) {
     if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
     const Range r = range_of(c, s);
-    launch_integrate(c->pool, r, I_PBOX, 0, c->stream);
-    launch_dis(c->pool, r, which ? M_DIS_VA : M_DIS_V, 0, which != 0, c->stream);
+    launch_integrate(c->pool, r, I_BINS, 0, c->stream);
+    launch_search(c->pool, r, 1, c->stream);
+    launch_con(c->pool, r, M_PAIRS, false, c->stream);
+    launch_dis(c->pool, r, which ? M_DIS_VA : M_DIS_V, 0, 1, which != 0, c->stream);
+    launch_set_flags(c->pool, r, -1, 1, c->stream);
     return finish(c);
 }
 
@@ -616,23 +712,22 @@ extern "C" cs3d_status cs3d_k_velocity_ab(cs3d_ctx* c, cs3d_sys s) {
 
 extern "C" cs3d_status cs3d_k_com(cs3d_ctx* c, cs3d_sys s) {
     if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
-    launch_dis(c->pool, range_of(c, s), M_COM, 0, false, c->stream);
+    launch_dis(c->pool, range_of(c, s), M_COM, 0, 0, false, c->stream);
     return finish(c);
 }
 
 extern "C" cs3d_status cs3d_k_volumes(cs3d_ctx* c, cs3d_sys s, int64_t step) {
     if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
-    launch_dis(c->pool, range_of(c, s), M_VOLUMES, step, false, c->stream);
+    launch_dis(c->pool, range_of(c, s), M_VOLUMES, step, 0, false, c->stream);
     return finish(c);
 }
 
 extern "C" cs3d_status cs3d_k_grid(cs3d_ctx* c, cs3d_sys s) {
     if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
     const Range r = range_of(c, s);
-    launch_integrate(c->pool, r, I_PBOX, 0, c->stream);
-    launch_set_rebuild(c->pool, r, 1, c->stream);
+    launch_set_flags(c->pool, r, 1, -1, c->stream);
     launch_grid(c->pool, r, vox_blocks_of(c, s), c->stream);
-    launch_set_rebuild(c->pool, r, 0, c->stream);
+    launch_set_flags(c->pool, r, 0, -1, c->stream);
     return finish(c);
 }
 
@@ -647,5 +742,34 @@ extern "C" cs3d_status cs3d_k_divide(cs3d_ctx* c, cs3d_sys s, int32_t* n_events)
         if (st != CS3D_OK) return st;
         *n_events = m.n_events;
     }
+    return CS3D_OK;
+}
+
+extern "C" cs3d_status cs3d_copy_calibrate(cs3d_ctx* c, int64_t bytes, int32_t reps, double* avg_ms) {
+    if (!c || bytes < 16 || reps < 1 || !avg_ms) return fail(CS3D_INVALID_ARG, "bad calibration arguments");
+    const int64_t n4 = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc(&a, n4 * 16));
+    if (hipMalloc(&b, n4 * 16) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(CS3D_OUT_OF_MEMORY, "calibration buffer");
+    }
+    (void)hipMemsetAsync(a, 0, n4 * 16, c->stream);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    launch_copy_calibrate(a, b, n4, c->stream);  // warm-up
+    (void)hipEventRecord(e0, c->stream);
+    for (int r = 0; r < reps; ++r) launch_copy_calibrate(a, b, n4, c->stream);
+    (void)hipEventRecord(e1, c->stream);
+    hipError_t e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return fail(CS3D_HIP_ERROR, hipGetErrorString(e));
+    *avg_ms = ms / reps;
     return CS3D_OK;
 }

@@ -5,12 +5,20 @@
 //                (SoA, 192-slot stride per cell as the reference's d_X, GPUbounce.cu:365-373)
 //   cell arrays  float[cap_cells]        CM x/y/z, pressure, stiffness, volume, area, growth scale
 //                float[cap_cells * 6]    bbox used by the contact candidate test (rebuilt with the grid)
-//                float[cap_cells * 72]   12 patch boxes of 16 consecutive nodes (rebuilt every step)
-//                int32[cap_cells * NCMAX] per-cell list of candidate neighbour cells (ascending)
+//   pair lists   int32[cap_cells * kPMax * 192]  contact partners of each node, (cell, node)
+//                ascending, k-major per cell so lanes access consecutive words
+//                int32[cap_cells * 192]  partner count per node
+//                int32[cap_cells * kPMax * 192]  contact candidates (within attraction_range + skin
+//                at the last search), same layout, + count per node
+//                float[cap_cells * 192] x3  node positions at the last search
 //   voxel pool   int32                   per system: counts[budget] then lists[budget * capacity]
+//                                        (the reference's 2.9-wide cell grid)
+//   bin pool     int32                   per system: counts[bbudget] then entries[bbudget * kBCap]
+//                                        float4 (x, y, z, cell*192 + node) (node bins of edge >=
+//                                        attraction_range for the pair search)
 //   slot table   int32[cap_cells]        system owning each cell slot (-1 = free)
-// A system (tissue) owns a contiguous range of cell slots [base, base + cap) and a
-// voxel region; its live cells are base .. base + n - 1 (local index 0 .. n - 1).
+// A system (tissue) owns a contiguous range of cell slots [base, base + cap), a voxel
+// region and a bin region; its live cells are base .. base + n - 1 (local 0 .. n - 1).
 #pragma once
 #include <stdint.h>
 #include "../../include/cs3d.h"
@@ -19,12 +27,8 @@ namespace cs3d {
 
 constexpr int kSlots = CS3D_SLOTS;   // 192
 constexpr int kNodes = CS3D_NODES;   // 180
-constexpr int kPatch = 16;           // nodes per contact patch
-constexpr int kNPatch = 12;          // ceil(180 / 16)
-constexpr int kPBox = kNPatch * 6;   // floats of patch boxes per cell
-constexpr int kNCMax = 128;          // candidate neighbour cells per cell
-constexpr int kNVMax = 64;           // voxels overlapped by one cell's box
-constexpr int kMaxEnt = 2048;        // voxel-list entries gathered per cell
+constexpr int kPMax = 40;            // contact candidates / partners stored per node
+constexpr int kBCap = 16;            // nodes per search bin
 constexpr int kMaxSys = 4096;        // systems per context
 
 // Per-system state kept on the device.
@@ -41,11 +45,17 @@ struct SysMeta {
     int64_t lbase;      // offset of lists[] in the voxel pool
     int32_t vbudget;    // voxels available
     int32_t vcap;       // entries per voxel list
-    float gmin[3];      // grid origin (minx, miny, minz)
+    float gmin[3];      // voxel grid origin (minx, miny, minz)
     int32_t dims[3];    // Xdiv, Ydiv, Zdiv
     int32_t ext[6];     // extent accumulators (order-preserving int encoding)
     int32_t n_events;   // divisions in the last division phase
-    int32_t pad0;
+    int32_t bbudget;    // search bins available
+    int64_t bbase;      // offset of bin counts in the bin pool
+    int64_t ebase;      // offset of bin entries in the bin pool
+    float bmin[3];      // bin grid origin
+    float bh;           // bin edge
+    int32_t bdims[3];   // bins per axis
+    int32_t need_search;  // 1 = search contact candidates this step (a node moved pair_skin/2)
     cs3d_ranmar_state rng;
 };
 
@@ -71,14 +81,17 @@ struct Pool {
     float *P, *E, *VOL, *AREA, *GS;
     int8_t* FLAG;
     float* BBOX;
-    float* PBOX;
-    int32_t* NCL;
-    int32_t* NCN;
+    int32_t* PAIRS;
+    int32_t* NPAIR;
+    int32_t* CAND;
+    int32_t* NCAND;
+    float *XBX, *XBY, *XBZ;
     int32_t* EVL;
     int32_t* slot_sys;
     SysMeta* meta;
     const cs3d_params* prm;
     int32_t* vox;
+    int32_t* bins;
     const MeshDev* mesh;
 };
 
@@ -97,7 +110,18 @@ enum : int32_t {
     M_DIS_VA = 4,      // CalculateDisForce with VA -> FD
     M_COM = 5,         // CenterOfMass only
     M_VOLUMES = 6,     // volumes only (explicit step)
-    M_INIT = 7,        // start-up: no pressure update, forces from state
+    M_INIT = 7,        // start-up: forces from the injected state
+    M_PAIRS = 8,       // contact-partner lists only (for a following CalculateDisForce)
+};
+
+// k_integrate flags
+enum : int32_t {
+    I_KICK = 1,       // Integrate + ForwardTime
+    I_PRESSURE = 2,   // PressureUpdate
+    I_BINS = 4,       // insert the nodes into the search bins
+    I_CLEAR = 8,      // clear the search-bin counters of the nodes
+    I_DISP = 16,      // flag a candidate search when a node moved pair_skin/2 since the last one
+    I_IFSEARCH = 32,  // I_BINS only on systems that search this step
 };
 
 }  // namespace cs3d

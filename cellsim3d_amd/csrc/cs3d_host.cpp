@@ -78,6 +78,7 @@ extern "C" cs3d_status cs3d_params_default(cs3d_params* p) {
     p->grid_dl = 2.9f;
     p->voxel_capacity = 32;
     p->grid_interval = 1;
+    p->pair_skin = 0.05f;
     return CS3D_OK;
 }
 
@@ -92,6 +93,7 @@ extern "C" cs3d_status cs3d_params_check(const cs3d_params* p) {
     if (p->voxel_capacity < 1 || p->voxel_capacity > 1024)
         return fail(CS3D_INVALID_ARG, "voxel_capacity must be in [1, 1024]");
     if (p->grid_interval < 1) return fail(CS3D_INVALID_ARG, "grid_interval must be >= 1");
+    if (!(p->pair_skin >= 0) || p->pair_skin > 10) return fail(CS3D_INVALID_ARG, "pair_skin must be in [0, 10]");
     if (p->ranmar_ij < 0 || p->ranmar_ij > 31328 || p->ranmar_kl < 0 || p->ranmar_kl > 30081)
         return fail(CS3D_INVALID_ARG, "RANMAR seeds out of range (marsaglia.h:79-83)");
     return CS3D_OK;

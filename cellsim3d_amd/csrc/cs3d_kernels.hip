@@ -1,24 +1,27 @@
 // cs3d_kernels.hip - CDNA4 (gfx950) kernels of the CellSim3D hot path.
 //
-// One cell per 192-lane workgroup (3 x wave64; lanes 180..191 are the padding
-// slots), grid-stride over cell slots so a fixed launch covers a dynamic cell
-// count.  Every kernel reproduces the floating-point operation order of the
-// reference kernel it replaces (cited per function) so that results match the
-// reference's arithmetic; compiled with -ffp-contract=off.
+// Cell kernels run one cell per 192-lane workgroup (3 x wave64; lanes 180..191
+// are the padding slots) and grid-stride over cell slots, so a fixed launch
+// covers a dynamic cell count.  Every kernel reproduces the floating-point
+// operation order of the reference kernel it replaces (cited per function), so
+// results match the reference arithmetic; the library is compiled with
+// -ffp-contract=off and correctly rounded fp32 division and square root.
 //
-// Contact / friction: the reference scans, per node, the cells listed in its
-// voxel and tests all 180 nodes of each kept cell (propagate.cu:375-442,
-// 603-655).  Here each workgroup first builds the ascending list of cells that
-// any of its nodes can keep (union of the voxels under the cell's current box,
-// filtered by a box-to-box distance bound that is never larger than a node's),
-// then walks it wave-uniformly: neighbour node data are read at uniform
-// addresses (scalar loads into SGPRs), each lane applies the reference's own
-// per-node test (voxel-list membership via a merge cursor + bbox distance), and
-// 16-node patches whose box is farther than attraction_range from the node are
-// skipped.  A skipped patch contains no pair the reference would accept (its
-// float distance bound is <= every pair distance), and accepted pairs are summed
-// in the reference order (ascending cell, ascending node), so the result equals
-// the brute-force loop.
+// Contact pairs.  The reference scans, per node, the cells listed in its
+// voxel (2.9-wide grid), keeps those whose bbox lies within attraction_range of
+// the node, and tests all 180 nodes of each kept cell (propagate.cu:375-442,
+// 603-655).  Here each node keeps a sorted list of candidate partners: the nodes
+// of other cells within attraction_range + pair_skin, found through search bins
+// of that edge (find_candidates).  The list is rebuilt only on steps where some
+// node has moved pair_skin/2 since the last search, so it always contains every
+// node now within attraction_range.  Every step then applies the reference's own
+// tests to the candidates in (cell, node) order -- the order of the reference's
+// loops: the distance test, voxel-list membership and the bbox test
+// (accept_forces).  Voxel-list membership is evaluated arithmetically: makeNNlist
+// puts cell b into voxel v exactly when v lies in the voxel range of b's bbox
+// widened by attraction_range (propagatebound.cu:224-259), so the same range test
+// on the same bbox and grid answers "b in list(v)".  The accepted pairs are
+// stored for the second friction pass.
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -52,6 +55,21 @@ __device__ __forceinline__ int vox1(float x, float mn, float DL, int div) {
     return s;
 }
 
+// bin coordinate of one axis (clamping keeps |i - j| <= 1 for nodes within one bin edge)
+__device__ __forceinline__ int bin1(float x, float mn, float h, int div) {
+    int s = (int)floorf((x - mn) / h);
+    if (s < 0) s = 0;
+    if (s >= div) s = div - 1;
+    return s;
+}
+
+__device__ __forceinline__ int bin_of(const SysMeta* M, float x, float y, float z) {
+    const int bx = bin1(x, M->bmin[0], M->bh, M->bdims[0]);
+    const int by = bin1(y, M->bmin[1], M->bh, M->bdims[1]);
+    const int bz = bin1(z, M->bmin[2], M->bh, M->bdims[2]);
+    return (bz * M->bdims[1] + by) * M->bdims[0] + bx;
+}
+
 // order-preserving int encoding of floats for atomic min/max
 __device__ __forceinline__ int f2o(float f) {
     int i = __float_as_int(f);
@@ -61,10 +79,9 @@ __device__ __forceinline__ float o2f(int i) { return __int_as_float(i >= 0 ? i :
 
 // ------------------------------------------------------------------------------
 // Integrate + ForwardTime (propagate.cu:681-712, 738-763), PressureUpdate
-// (PressureKernels.cu:4-41) and the 16-node patch boxes of the new positions.
+// (PressureKernels.cu:4-41) and insertion of the new positions into the
+// search bins.
 // ------------------------------------------------------------------------------
-enum : int32_t { I_KICK = 1, I_PRESSURE = 2, I_PBOX = 4 };
-
 __global__ __launch_bounds__(192) void k_integrate(Pool p, Range r, int32_t flags, int64_t step_override) {
     const int tid = threadIdx.x;
     const bool node = tid < kNodes;
@@ -75,6 +92,7 @@ __global__ __launch_bounds__(192) void k_integrate(Pool p, Range r, int32_t flag
         if (!sys_live(M)) continue;
         const int c = slot - M->base;
         if (c >= M->n) continue;
+        if ((flags & I_IFSEARCH) && !M->need_search) continue;
         const cs3d_params* P = p.prm + s;
         const int64_t gi = (int64_t)slot * kSlots + tid;
         float x = p.X[gi], y = p.Y[gi], z = p.Z[gi];
@@ -92,19 +110,26 @@ __global__ __launch_bounds__(192) void k_integrate(Pool p, Range r, int32_t flag
             p.VAX[gi] = vx; p.VAY[gi] = vy; p.VAZ[gi] = vz;
             p.X[gi] = x; p.Y[gi] = y; p.Z[gi] = z;
         }
-        if (flags & I_PBOX) {
-            float mnx = node ? x : INFINITY, mxx = node ? x : -INFINITY;
-            float mny = node ? y : INFINITY, mxy = node ? y : -INFINITY;
-            float mnz = node ? z : INFINITY, mxz = node ? z : -INFINITY;
-#pragma unroll
-            for (int o = 8; o > 0; o >>= 1) {
-                mnx = fminf(mnx, __shfl_xor(mnx, o, 16)); mxx = fmaxf(mxx, __shfl_xor(mxx, o, 16));
-                mny = fminf(mny, __shfl_xor(mny, o, 16)); mxy = fmaxf(mxy, __shfl_xor(mxy, o, 16));
-                mnz = fminf(mnz, __shfl_xor(mnz, o, 16)); mxz = fmaxf(mxz, __shfl_xor(mxz, o, 16));
+        if (flags & I_DISP) {  // moved pair_skin/2 since the last candidate search?
+            bool far = false;
+            if (node) {
+                const float h = 0.5f * P->pair_skin;
+                const float dx = x - p.XBX[gi], dy = y - p.XBY[gi], dz = z - p.XBZ[gi];
+                far = !(dx * dx + dy * dy + dz * dz <= h * h);
             }
-            if ((tid & 15) == 0) {
-                float* pb = p.PBOX + (int64_t)slot * kPBox + (tid >> 4) * 6;
-                pb[0] = mnx; pb[1] = mxx; pb[2] = mny; pb[3] = mxy; pb[4] = mnz; pb[5] = mxz;
+            if (__any(far) && (tid & 63) == __ffsll((unsigned long long)__ballot(far)) - 1)
+                atomicOr(&M->need_search, 1);
+        }
+        if (node && (flags & I_CLEAR)) p.bins[M->bbase + bin_of(M, x, y, z)] = 0;
+        if (node && (flags & I_BINS)) {
+            const int b = bin_of(M, x, y, z);
+            int32_t* counts = p.bins + M->bbase;
+            const int k = atomicAdd(&counts[b], 1);
+            if (k < kBCap) {
+                float4* ent = reinterpret_cast<float4*>(p.bins + M->ebase);
+                ent[(int64_t)b * kBCap + k] = make_float4(x, y, z, __int_as_float(c * kSlots + tid));
+            } else {
+                flag(M, CS3D_SYS_CAND_OVERFLOW);
             }
         }
         if (tid == 0 && (flags & I_PRESSURE)) {
@@ -129,217 +154,116 @@ __global__ __launch_bounds__(192) void k_integrate(Pool p, Range r, int32_t flag
 }
 
 // ------------------------------------------------------------------------------
-// Candidate neighbour cells of one cell (workgroup-cooperative).
+// Contact candidates of one node (called per lane by a cell workgroup): every
+// node j of another cell within attraction_range + pair_skin, found by scanning
+// the 27 search bins around the node.  Keys b*192 + j (local cell ids) are left
+// ascending in lst[0 .. n)[tid]; returns n.
 // ------------------------------------------------------------------------------
-struct NCShared {
-    int32_t ent[1024];
-    int8_t keep[1024];
-    int32_t vid[kNVMax];
-    int32_t pre[kNVMax];
-    int32_t list[kNCMax];
-    int32_t ne;
-    int32_t nlist;
-    int32_t bad;
-};
-
-// Builds nc.list (ascending local ids) of cells b != c listed in any voxel under
-// [cb0..cb5] whose bbox lies within attraction_range of that box.
-__device__ void build_nc(const Pool& p, SysMeta* M, const cs3d_params* P, int c, const float* cb,
-                         const float* __restrict__ BB, NCShared& nc) {
-    const int tid = threadIdx.x;
-    const float DL = P->grid_dl, ar = P->attraction_range;
-    const float ar2 = ar * ar;
-    const int xd = M->dims[0], yd = M->dims[1], zd = M->dims[2];
-    const int vx0 = vox1(cb[0], M->gmin[0], DL, xd), vx1 = vox1(cb[1], M->gmin[0], DL, xd);
-    const int vy0 = vox1(cb[2], M->gmin[1], DL, yd), vy1 = vox1(cb[3], M->gmin[1], DL, yd);
-    const int vz0 = vox1(cb[4], M->gmin[2], DL, zd), vz1 = vox1(cb[5], M->gmin[2], DL, zd);
-    const int nx = vx1 - vx0 + 1, ny = vy1 - vy0 + 1, nz = vz1 - vz0 + 1;
-    const int nv = nx * ny * nz;
-    const int32_t* counts = p.vox + M->vbase;
-    const int32_t* lists = p.vox + M->lbase;
-    const int cap = M->vcap;
-    if (tid < 64) {
-        int cnt = 0, v = 0;
-        if (tid < nv && nv <= kNVMax) {
-            const int ix = tid % nx, iy = (tid / nx) % ny, iz = tid / (nx * ny);
-            v = (vz0 + iz) * xd * yd + (vy0 + iy) * xd + (vx0 + ix);
-            cnt = min(counts[v], cap);
-        }
-        int inc = cnt;
+__device__ int find_candidates(const Pool& p, const SysMeta* M, const cs3d_params* P, int c, float X, float Y,
+                               float Z, int32_t (*lst)[kSlots], int tid, uint32_t& st) {
+    int nc = 0;
+    const float rc = P->attraction_range + P->pair_skin;
+    const float rc2 = rc * rc;
+    const int bx = bin1(X, M->bmin[0], M->bh, M->bdims[0]);
+    const int by = bin1(Y, M->bmin[1], M->bh, M->bdims[1]);
+    const int bz = bin1(Z, M->bmin[2], M->bh, M->bdims[2]);
+    const int bdx = M->bdims[0], bdy = M->bdims[1], bdz = M->bdims[2];
+    const int32_t* counts = p.bins + M->bbase;
+    const float4* ents = reinterpret_cast<const float4*>(p.bins + M->ebase);
+    auto consider = [&](const float4 en) {
+        const int key = __float_as_int(en.w);
+        if (key / kSlots == c) return;
+        const float dX = X - en.x, dY = Y - en.y, dZ = Z - en.z;
+        if (dX * dX + dY * dY + dZ * dZ > rc2) return;
+        if (nc < kPMax) lst[nc][tid] = key;
+        ++nc;
+    };
+    // 9 rows of 3 x-adjacent bins; the first 4 entries of each bin of a row are
+    // loaded together (independent loads), longer bins finish in a tail loop
+    constexpr int kHead = 4;
+    for (int row = 0; row < 9; ++row) {
+        const int y3 = by + row % 3 - 1, z3 = bz + row / 3 - 1;
+        const bool rin = y3 >= 0 && y3 < bdy && z3 >= 0 && z3 < bdz;
+        int bi[3], cn[3];
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            int t = __shfl_up(inc, o);
-            if (tid >= o) inc += t;
+        for (int q = 0; q < 3; ++q) {
+            const int x3 = bx + q - 1;
+            const bool in = rin && x3 >= 0 && x3 < bdx;
+            bi[q] = in ? (z3 * bdy + y3) * bdx + x3 : 0;
+            cn[q] = in ? min(counts[bi[q]], kBCap) : 0;
         }
-        if (tid < kNVMax) { nc.vid[tid] = v; nc.pre[tid] = inc - cnt; }
-        if (tid == 63) {
-            nc.ne = inc;
-            nc.nlist = 0;
-            nc.bad = (nv > kNVMax || inc > 1024) ? 1 : 0;
+        float4 head[3][kHead];
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int e = 0; e < kHead; ++e) head[q][e] = ents[(int64_t)bi[q] * kBCap + e];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+#pragma unroll
+            for (int e = 0; e < kHead; ++e)
+                if (e < cn[q]) consider(head[q][e]);
+            for (int e = kHead; e < cn[q]; ++e) consider(ents[(int64_t)bi[q] * kBCap + e]);
         }
     }
-    __syncthreads();
-    if (nc.bad) {
-        if (tid == 0) { flag(M, CS3D_SYS_CAND_OVERFLOW); nc.nlist = 0; }
-        __syncthreads();
-        return;
+    if (nc > kPMax) {
+        st |= CS3D_SYS_CAND_OVERFLOW;
+        nc = kPMax;
     }
-    const int ne = nc.ne;
-    for (int e = tid; e < ne; e += blockDim.x) {
-        int t = 0;
-        while (t + 1 < nv && nc.pre[t + 1] <= e) ++t;
-        nc.ent[e] = lists[(int64_t)nc.vid[t] * cap + (e - nc.pre[t])];
-    }
-    __syncthreads();
-    for (int e = tid; e < ne; e += blockDim.x) {
-        const int b = nc.ent[e];
-        bool keep = b != c;
-        for (int e2 = 0; keep && e2 < e; ++e2) keep = nc.ent[e2] != b;
-        if (keep) {  // box-to-box bound of the node test of propagate.cu:382-392
-            const float* bb = BB + (int64_t)(M->base + b) * 6;
-            float dx = (float)(cb[0] - bb[1] > 0.0f) * (cb[0] - bb[1]);
-            dx += (float)(bb[0] - cb[1] > 0.0f) * (bb[0] - cb[1]);
-            float dy = (float)(cb[2] - bb[3] > 0.0f) * (cb[2] - bb[3]);
-            dy += (float)(bb[2] - cb[3] > 0.0f) * (bb[2] - cb[3]);
-            float dz = (float)(cb[4] - bb[5] > 0.0f) * (cb[4] - bb[5]);
-            dz += (float)(bb[4] - cb[5] > 0.0f) * (bb[4] - cb[5]);
-            keep = !(dx * dx + dy * dy + dz * dz > ar2);
+    for (int i = 1; i < nc; ++i) {  // insertion sort of a short list
+        const int key = lst[i][tid];
+        int j = i - 1;
+        while (j >= 0 && lst[j][tid] > key) {
+            lst[j + 1][tid] = lst[j][tid];
+            --j;
         }
-        nc.keep[e] = keep ? 1 : 0;
+        lst[j + 1][tid] = key;
     }
-    __syncthreads();
-    for (int e = tid; e < ne; e += blockDim.x) {
-        if (!nc.keep[e]) continue;
-        const int b = nc.ent[e];
-        int rank = 0;
-        for (int e2 = 0; e2 < ne; ++e2) rank += (nc.keep[e2] && nc.ent[e2] < b) ? 1 : 0;
-        if (rank < kNCMax) nc.list[rank] = b;
-        atomicAdd(&nc.nlist, 1);
-    }
-    __syncthreads();
-    if (nc.nlist > kNCMax) {
-        __syncthreads();
-        if (tid == 0) { flag(M, CS3D_SYS_CAND_OVERFLOW); nc.nlist = kNCMax; }
-        __syncthreads();
-    }
+    return nc;
 }
 
-// Per-lane state of the reference's candidate test for one node.
-struct LaneCand {
-    const int32_t* list;
-    int cnt;
-    int cur;
-    int ncand;
+// The reference's candidate rule for node (X, Y, Z) in voxel (vx, vy, vz) and
+// cell b: b is in the node's voxel list -- makeNNlist puts b into exactly the
+// voxels of its bbox widened by attraction_range (propagatebound.cu:224-259) --
+// and the node lies within attraction_range of b's bbox (propagate.cu:382-392).
+struct RuleCtx {
+    float ar, ar2, DL, gx, gy, gz;
+    int xd, yd, zd, vx, vy, vz;
 };
 
-__device__ __forceinline__ LaneCand lane_cand(const Pool& p, const SysMeta* M, const cs3d_params* P, float X,
-                                              float Y, float Z, bool node) {
-    LaneCand L;
-    L.cur = 0;
-    L.ncand = 0;
-    L.cnt = 0;
-    L.list = p.vox;
-    if (node) {
-        const float DL = P->grid_dl;
-        const int xd = M->dims[0], yd = M->dims[1], zd = M->dims[2];
-        const int sx = vox1(X, M->gmin[0], DL, xd), sy = vox1(Y, M->gmin[1], DL, yd), sz = vox1(Z, M->gmin[2], DL, zd);
-        const int index = sz * xd * yd + sy * xd + sx;
-        L.cnt = min(p.vox[M->vbase + index], M->vcap);
-        L.list = p.vox + M->lbase + (int64_t)index * M->vcap;
-    }
-    return L;
+__device__ __forceinline__ bool rule_ok(const RuleCtx& R, const float* __restrict__ bb, float X, float Y, float Z) {
+    const float b0 = bb[0], b1 = bb[1], b2 = bb[2], b3 = bb[3], b4 = bb[4], b5 = bb[5];
+    int sx = (int)((b0 - R.ar - R.gx) / R.DL); if (sx < 0) sx = 0;
+    int ex = (int)((b1 + R.ar - R.gx) / R.DL); if (ex >= R.xd) ex = R.xd - 1;
+    int sy = (int)((b2 - R.ar - R.gy) / R.DL); if (sy < 0) sy = 0;
+    int ey = (int)((b3 + R.ar - R.gy) / R.DL); if (ey >= R.yd) ey = R.yd - 1;
+    int sz = (int)((b4 - R.ar - R.gz) / R.DL); if (sz < 0) sz = 0;
+    int ez = (int)((b5 + R.ar - R.gz) / R.DL); if (ez >= R.zd) ez = R.zd - 1;
+    const bool inlist = sx <= R.vx && R.vx <= ex && sy <= R.vy && R.vy <= ey && sz <= R.vz && R.vz <= ez;
+    float qx = (float)(X - b1 > 0.0f) * (X - b1);
+    qx += (float)(b0 - X > 0.0f) * (b0 - X);
+    float qy = (float)(Y - b3 > 0.0f) * (Y - b3);
+    qy += (float)(b2 - Y > 0.0f) * (b2 - Y);
+    float qz = (float)(Z - b5 > 0.0f) * (Z - b5);
+    qz += (float)(b4 - Z > 0.0f) * (b4 - Z);
+    return inlist && !(qx * qx + qy * qy + qz * qz > R.ar2);
 }
 
-// Reference node test (propagate.cu:375-392): b in the node's voxel list and the
-// node within attraction_range of b's bbox.  b values arrive ascending.
-__device__ __forceinline__ bool lane_test(LaneCand& L, int b, float X, float Y, float Z, const float* bb,
-                                          float ar2) {
-    while (L.cur < L.cnt && L.list[L.cur] < b) ++L.cur;
-    if (!(L.cur < L.cnt && L.list[L.cur] == b)) return false;
-    ++L.cur;
-    float dX = (float)(X - bb[1] > 0.0f) * (X - bb[1]);
-    dX += (float)(bb[0] - X > 0.0f) * (bb[0] - X);
-    float dY = (float)(Y - bb[3] > 0.0f) * (Y - bb[3]);
-    dY += (float)(bb[2] - Y > 0.0f) * (bb[2] - Y);
-    float dZ = (float)(Z - bb[5] > 0.0f) * (Z - bb[5]);
-    dZ += (float)(bb[4] - Z > 0.0f) * (bb[4] - Z);
-    if (dX * dX + dY * dY + dZ * dZ > ar2) return false;
-    ++L.ncand;
-    return true;
+__device__ __forceinline__ RuleCtx rule_ctx(const SysMeta* M, const cs3d_params* P, float X, float Y, float Z) {
+    RuleCtx R;
+    R.ar = P->attraction_range;
+    R.ar2 = R.ar * R.ar;
+    R.DL = P->grid_dl;
+    R.gx = M->gmin[0]; R.gy = M->gmin[1]; R.gz = M->gmin[2];
+    R.xd = M->dims[0]; R.yd = M->dims[1]; R.zd = M->dims[2];
+    R.vx = vox1(X, R.gx, R.DL, R.xd);  // the node's voxel, propagate.cu:360-372
+    R.vy = vox1(Y, R.gy, R.DL, R.yd);
+    R.vz = vox1(Z, R.gz, R.DL, R.zd);
+    return R;
 }
 
-__device__ __forceinline__ bool patch_near(float X, float Y, float Z, const float* pb, float ar2) {
-    float gx = (float)(X - pb[1] > 0.0f) * (X - pb[1]);
-    gx += (float)(pb[0] - X > 0.0f) * (pb[0] - X);
-    float gy = (float)(Y - pb[3] > 0.0f) * (Y - pb[3]);
-    gy += (float)(pb[2] - Y > 0.0f) * (pb[2] - Y);
-    float gz = (float)(Z - pb[5] > 0.0f) * (Z - pb[5]);
-    gz += (float)(pb[4] - Z > 0.0f) * (pb[4] - Z);
-    return !(gx * gx + gy * gy + gz * gz > ar2);
-}
-
-struct PairConsts {
-    float ar, ar2, rr, ka, kr, gext;
-};
-
-// Contact (CON: propagate.cu:406-442) and friction (DIS: propagate.cu:629-655)
-// over all kept neighbour nodes, in reference order.
-template <bool CON, bool DIS>
-__device__ __forceinline__ void pair_loop(const NCShared& nc, int base, const SysMeta* M, LaneCand& L, bool node,
-                                          float X, float Y, float Z, float vix, float viy, float viz, float nx,
-                                          float ny, float nz, const PairConsts& k,
-                                          const float* __restrict__ rX, const float* __restrict__ rY,
-                                          const float* __restrict__ rZ, const float* __restrict__ rVX,
-                                          const float* __restrict__ rVY, const float* __restrict__ rVZ,
-                                          const float* __restrict__ rPB, const float* __restrict__ rBB,
-                                          float& cfx, float& cfy, float& cfz, float& fdx, float& fdy,
-                                          float& fdz) {
-    const int nl = nc.nlist;
-    for (int kk = 0; kk < nl; ++kk) {
-        const int b = __builtin_amdgcn_readfirstlane(nc.list[kk]);
-        const int64_t gb = (int64_t)(base + b);
-        const float* bb = rBB + gb * 6;
-        const bool cand = node && lane_test(L, b, X, Y, Z, bb, k.ar2);
-        if (!__any(cand)) continue;
-        const float* pbs = rPB + gb * kPBox;
-        const int64_t nb = gb * kSlots;
-        for (int pt = 0; pt < kNPatch; ++pt) {
-            const bool act = cand && patch_near(X, Y, Z, pbs + pt * 6, k.ar2);
-            if (!__any(act)) continue;
-            const int j0 = pt * kPatch;
-            const int j1 = min(j0 + kPatch, kNodes);
-            for (int j = j0; j < j1; ++j) {
-                const float xj = rX[nb + j], yj = rY[nb + j], zj = rZ[nb + j];
-                float vjx = 0.f, vjy = 0.f, vjz = 0.f;
-                if (DIS) { vjx = rVX[nb + j]; vjy = rVY[nb + j]; vjz = rVZ[nb + j]; }
-                if (act) {
-                    const float dX = X - xj, dY = Y - yj, dZ = Z - zj;
-                    const float R2 = dX * dX + dY * dY + dZ * dZ;
-                    if (CON && R2 < k.ar2) {
-                        const float R = sqrtf(R2);
-                        if (R < k.ar) {
-                            cfx += k.ka * (k.ar - R) / R * dX;
-                            cfy += k.ka * (k.ar - R) / R * dY;
-                            cfz += k.ka * (k.ar - R) / R * dZ;
-                        }
-                        if (R <= k.rr) {
-                            cfx += k.kr * (k.rr - R) / R * dX;
-                            cfy += k.kr * (k.rr - R) / R * dY;
-                            cfz += k.kr * (k.rr - R) / R * dZ;
-                        }
-                    }
-                    if (DIS && !(R2 > k.ar2)) {
-                        const float vx = vix - vjx, vy = viy - vjy, vz = viz - vjz;
-                        const float d = vx * nx + vy * ny + vz * nz;
-                        const float tx = vx - d * nx, ty = vy - d * ny, tz = vz - d * nz;
-                        fdx = fdx - k.gext * tx;
-                        fdy = fdy - k.gext * ty;
-                        fdz = fdz - k.gext * tz;
-                    }
-                }
-            }
-        }
-    }
+// clear the search-bin counter a node incremented (each node clears its own bin)
+__device__ __forceinline__ void clear_own_bin(const Pool& p, const SysMeta* M, float x, float y, float z) {
+    p.bins[M->bbase + bin_of(M, x, y, z)] = 0;
 }
 
 // Shared GetAngleForce pair (propagate.cu:26-60): Fa = F(a-v, b-v), Fb = F(b-v, a-v).
@@ -363,39 +287,171 @@ __device__ __forceinline__ uint32_t angle_pair(float ax, float ay, float az, flo
     return ok ? 0u : (uint32_t)CS3D_SYS_ANGLE_NONFINITE;
 }
 
-// own box of a cell from its 12 patch boxes
-__device__ __forceinline__ void cell_box_from_patches(const float* __restrict__ pb, float* cb) {
-    const int tid = threadIdx.x;
-    if (tid < 6) {
-        float v = pb[tid];
-        for (int q = 1; q < kNPatch; ++q) {
-            const float w = pb[q * 6 + tid];
-            v = (tid & 1) ? fmaxf(v, w) : fminf(v, w);
+// friction normal of node tid (propagate.cu:558-572), positions in LDS
+__device__ __forceinline__ void node_normal(const float* sx, const float* sy, const float* sz, const MeshDev* mesh,
+                                            int tid, int N1, int N2, int N3, float& nx, float& ny, float& nz) {
+    const float Ax = sx[N2] - sx[N1], Ay = sy[N2] - sy[N1], Az = sz[N2] - sz[N1];
+    const float Bx = sx[N3] - sx[N1], By = sy[N3] - sy[N1], Bz = sz[N3] - sz[N1];
+    const float cx = Ay * Bz - Az * By, cy = Az * Bx - Ax * Bz, cz = Ax * By - Ay * Bx;
+    const float inv = 1.0f / sqrtf(cx * cx + cy * cy + cz * cz);
+    const float sg = (float)mesh->sign[tid];
+    nx = sg * (inv * cx);
+    ny = sg * (inv * cy);
+    nz = sg * (inv * cz);
+}
+
+struct PairConsts {
+    float ar, ar2, rr, ka, kr, gext;
+};
+
+// Contact (CON: propagate.cu:411-440) and friction (DIS: propagate.cu:633-653)
+// over the node's stored partners, in reference order.
+template <bool CON, bool DIS>
+__device__ __forceinline__ void pair_forces(const int32_t* lst, int base, int np, float X, float Y, float Z,
+                                            float vix, float viy, float viz, float nx, float ny, float nz,
+                                            const PairConsts& k, const float* __restrict__ rX,
+                                            const float* __restrict__ rY, const float* __restrict__ rZ,
+                                            const float* __restrict__ rVX, const float* __restrict__ rVY,
+                                            const float* __restrict__ rVZ, float& cfx, float& cfy, float& cfz,
+                                            float& fdx, float& fdy, float& fdz) {
+    for (int i = 0; i < np; ++i) {
+        const int key = lst[i * kSlots];
+        const int b = key / kSlots;
+        const int64_t gj = (int64_t)(base + b) * kSlots + (key - b * kSlots);
+        const float dX = X - rX[gj], dY = Y - rY[gj], dZ = Z - rZ[gj];
+        const float R2 = dX * dX + dY * dY + dZ * dZ;
+        if (CON && R2 < k.ar2) {
+            const float R = sqrtf(R2);
+            if (R < k.ar) {
+                cfx += k.ka * (k.ar - R) / R * dX;
+                cfy += k.ka * (k.ar - R) / R * dY;
+                cfz += k.ka * (k.ar - R) / R * dZ;
+            }
+            if (R <= k.rr) {
+                cfx += k.kr * (k.rr - R) / R * dX;
+                cfy += k.kr * (k.rr - R) / R * dY;
+                cfz += k.kr * (k.rr - R) / R * dZ;
+            }
         }
-        cb[tid] = v;
+        if (DIS) {
+            const float vx = vix - rVX[gj], vy = viy - rVY[gj], vz = viz - rVZ[gj];
+            const float d = vx * nx + vy * ny + vz * nz;
+            const float tx = vx - d * nx, ty = vy - d * ny, tz = vz - d * nz;
+            fdx = fdx - k.gext * tx;
+            fdy = fdy - k.gext * ty;
+            fdz = fdz - k.gext * tz;
+        }
+    }
+}
+
+// Reference tests over the node's candidates in stored (= reference) order:
+// distance (ConForce skips R^2 >= ar^2, DisForce R^2 > ar^2, propagate.cu:421,641)
+// and the candidate rule per partner cell.  Accepted keys go to rec (for the
+// second friction pass); contact (CON) and friction (DIS) are summed as in
+// propagate.cu:411-440 and 633-653.  Returns the number accepted.
+template <bool CON, bool DIS>
+__device__ __forceinline__ int accept_forces(const int32_t* cand, int nc, int32_t* rec, int base, float X, float Y,
+                                             float Z, const RuleCtx& R, const float* __restrict__ rBB, float vix,
+                                             float viy, float viz, float nx, float ny, float nz, const PairConsts& k,
+                                             const float* __restrict__ rX, const float* __restrict__ rY,
+                                             const float* __restrict__ rZ, const float* __restrict__ rVX,
+                                             const float* __restrict__ rVY, const float* __restrict__ rVZ,
+                                             float& cfx, float& cfy, float& cfz, float& fdx, float& fdy,
+                                             float& fdz) {
+    int np = 0;
+    int lastb = -1;
+    bool lastok = false;
+    for (int i = 0; i < nc; ++i) {
+        const int key = cand[i * kSlots];
+        const int b = key / kSlots;
+        const int64_t gj = (int64_t)(base + b) * kSlots + (key - b * kSlots);
+        const float dX = X - rX[gj], dY = Y - rY[gj], dZ = Z - rZ[gj];
+        const float R2 = dX * dX + dY * dY + dZ * dZ;
+        if (R2 > k.ar2) continue;
+        if (b != lastb) {
+            lastb = b;
+            lastok = rule_ok(R, rBB + (int64_t)(base + b) * 6, X, Y, Z);
+        }
+        if (!lastok) continue;
+        rec[np * kSlots] = key;
+        ++np;
+        if (CON && R2 < k.ar2) {
+            const float Rl = sqrtf(R2);
+            if (Rl < k.ar) {
+                cfx += k.ka * (k.ar - Rl) / Rl * dX;
+                cfy += k.ka * (k.ar - Rl) / Rl * dY;
+                cfz += k.ka * (k.ar - Rl) / Rl * dZ;
+            }
+            if (Rl <= k.rr) {
+                cfx += k.kr * (k.rr - Rl) / Rl * dX;
+                cfy += k.kr * (k.rr - Rl) / Rl * dY;
+                cfz += k.kr * (k.rr - Rl) / Rl * dZ;
+            }
+        }
+        if (DIS) {
+            const float vx = vix - rVX[gj], vy = viy - rVY[gj], vz = viz - rVZ[gj];
+            const float d = vx * nx + vy * ny + vz * nz;
+            const float tx = vx - d * nx, ty = vy - d * ny, tz = vz - d * nz;
+            fdx = fdx - k.gext * tx;
+            fdy = fdy - k.gext * ty;
+            fdz = fdz - k.gext * tz;
+        }
+    }
+    return np;
+}
+
+// ------------------------------------------------------------------------------
+// Contact-candidate search (one cell per workgroup): on systems whose nodes
+// moved pair_skin/2 since the last search (or always, outside the fused step),
+// rebuild each node's sorted candidate list and remember the positions.
+// ------------------------------------------------------------------------------
+__global__ __launch_bounds__(192) void k_search(Pool p, Range r, int32_t always) {
+    __shared__ int32_t lst[kPMax][kSlots];
+    const int tid = threadIdx.x;
+    for (int slot = r.slot_lo + blockIdx.x; slot < r.slot_hi; slot += gridDim.x) {
+        const int s = p.slot_sys[slot];
+        if (s < 0) continue;
+        SysMeta* M = p.meta + s;
+        if (!sys_live(M) || !(always || M->need_search)) continue;
+        const int c = slot - M->base;
+        if (c >= M->n) continue;
+        if (tid >= kNodes) continue;
+        const cs3d_params* P = p.prm + s;
+        const int64_t gi = (int64_t)slot * kSlots + tid;
+        const float X = p.X[gi], Y = p.Y[gi], Z = p.Z[gi];
+        uint32_t st = 0;
+        const int nc = find_candidates(p, M, P, c, X, Y, Z, lst, tid, st);
+        int32_t* cstore = p.CAND + (int64_t)slot * kPMax * kSlots + tid;
+        for (int i = 0; i < nc; ++i) cstore[i * kSlots] = lst[i][tid];
+        p.NCAND[gi] = nc;
+        p.XBX[gi] = X; p.XBY[gi] = Y; p.XBZ[gi] = Z;
+        if (st) flag(M, st);
     }
 }
 
 // ------------------------------------------------------------------------------
 // Conservative force (CalculateConForce, propagate.cu:183-520) fused with the
 // first dissipative force (CalculateDisForce with the post-Integrate
-// velocities, propagate.cu:523-665) and VelocityUpdateA/B (propagate.cu:714-736).
-// Modes: M_STEP (FC, V), M_CON / M_INIT (FC only), M_BONDED (spring + pressure
-// + angle -> FC).
+// velocities, propagate.cu:523-665) and VelocityUpdateA/B (propagate.cu:714-736),
+// over the stored contact candidates.  Modes: M_STEP (FC, V, accepted
+// partners), M_CON / M_INIT (FC, accepted partners), M_BONDED (spring +
+// pressure + angle -> FC), M_PAIRS (accepted partners only).
 // ------------------------------------------------------------------------------
 __global__ __launch_bounds__(192) void k_con_dis(Pool p, Range r, int32_t mode, const float* __restrict__ rX,
                                                  const float* __restrict__ rY, const float* __restrict__ rZ,
                                                  const float* __restrict__ rVX, const float* __restrict__ rVY,
-                                                 const float* __restrict__ rVZ, const float* __restrict__ rPB,
-                                                 const float* __restrict__ rBB) {
+                                                 const float* __restrict__ rVZ, const float* __restrict__ rBB) {
     __shared__ float sx[kSlots], sy[kSlots], sz[kSlots];
     __shared__ float svx[kSlots], svy[kSlots], svz[kSlots];
-    __shared__ float angF[kNodes * 18];
-    __shared__ float cbox[6];
-    __shared__ NCShared nc;
+    __shared__ struct {
+        float angF[kNodes * 18];
+    } u;
     const int tid = threadIdx.x;
     const bool node = tid < kNodes;
     const MeshDev* mesh = p.mesh;
+    const bool fused = mode == M_STEP;
+    const bool forces = mode != M_PAIRS;
+    const bool search = mode != M_BONDED;
     for (int slot = r.slot_lo + blockIdx.x; slot < r.slot_hi; slot += gridDim.x) {
         const int s = p.slot_sys[slot];
         if (s < 0) continue;
@@ -404,80 +460,77 @@ __global__ __launch_bounds__(192) void k_con_dis(Pool p, Range r, int32_t mode, 
         const int c = slot - M->base;
         if (c >= M->n) continue;
         const cs3d_params* P = p.prm + s;
-        const int64_t cb0 = (int64_t)slot * kSlots;
-        const int64_t gi = cb0 + tid;
+        const int64_t gi = (int64_t)slot * kSlots + tid;
         const float X = rX[gi], Y = rY[gi], Z = rZ[gi];
         sx[tid] = X; sy[tid] = Y; sz[tid] = Z;
-        const bool fused = mode == M_STEP;
         if (fused) { svx[tid] = rVX[gi]; svy[tid] = rVY[gi]; svz[tid] = rVZ[gi]; }
-        if (mode != M_BONDED) cell_box_from_patches(rPB + (int64_t)slot * kPBox, cbox);
         __syncthreads();
         uint32_t st = 0;
-        if (node && (isnan(X) || isnan(Y) || isnan(Z))) st |= CS3D_SYS_NAN;  // propagate.cu:241-248
-        // ---- springs, propagate.cu:298-314
-        const float stiffness = p.E[slot];
         float FX = 0.f, FY = 0.f, FZ = 0.f;
         int nn3[3] = {0, 0, 0};
-        if (node) {
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                const int N1 = mesh->nn[i * kSlots + tid];
-                nn3[i] = N1;
-                const float dX = sx[N1] - X, dY = sy[N1] - Y, dZ = sz[N1] - Z;
-                const float R = sqrtf(dX * dX + dY * dY + dZ * dZ);
-                const float R0 = mesh->R0[i * kSlots + tid];
-                FX += stiffness * (R - R0) / R0 * dX / R;
-                FY += stiffness * (R - R0) / R0 * dY / R;
-                FZ += stiffness * (R - R0) / R0 * dZ / R;
-            }
-            // ---- pressure, propagate.cu:318-330 (centre of mass from the last CenterOfMass)
-            const float rx = X - p.CMX[slot], ry = Y - p.CMY[slot], rz = Z - p.CMZ[slot];
-            const float inv = 1.0f / sqrtf(rx * rx + ry * ry + rz * rz);
-            const float g = 3.0f * p.P[slot];
-            FX += g * (inv * rx);
-            FY += g * (inv * ry);
-            FZ += g * (inv * rz);
-        }
-        // ---- angles, propagate.cu:81-181 (each angle once, gathered in reference order)
-        if (P->constrain_angles) {
-            const float k = P->angle_k;
+        if (forces) {
+            if (node && (isnan(X) || isnan(Y) || isnan(Z))) st |= CS3D_SYS_NAN;  // propagate.cu:241-248
+            // ---- springs, propagate.cu:298-314
+            const float stiffness = p.E[slot];
             if (node) {
-                const float* th = mesh->theta0 + 3 * tid;
-                const int A[3] = {nn3[0], nn3[1], nn3[0]};
-                const int B[3] = {nn3[1], nn3[2], nn3[2]};
 #pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    st |= angle_pair(sx[A[q]] - X, sy[A[q]] - Y, sz[A[q]] - Z, sx[B[q]] - X, sy[B[q]] - Y,
-                                     sz[B[q]] - Z, th[q], k, angF + (tid * 3 + q) * 6);
+                for (int i = 0; i < 3; ++i) {
+                    const int N1 = mesh->nn[i * kSlots + tid];
+                    nn3[i] = N1;
+                    const float dX = sx[N1] - X, dY = sy[N1] - Y, dZ = sz[N1] - Z;
+                    const float R = sqrtf(dX * dX + dY * dY + dZ * dZ);
+                    const float R0 = mesh->R0[i * kSlots + tid];
+                    FX += stiffness * (R - R0) / R0 * dX / R;
+                    FY += stiffness * (R - R0) / R0 * dY / R;
+                    FZ += stiffness * (R - R0) / R0 * dZ / R;
                 }
+                // ---- pressure, propagate.cu:318-330 (centre of mass from the last CenterOfMass)
+                const float rx = X - p.CMX[slot], ry = Y - p.CMY[slot], rz = Z - p.CMZ[slot];
+                const float inv = 1.0f / sqrtf(rx * rx + ry * ry + rz * rz);
+                const float g = 3.0f * p.P[slot];
+                FX += g * (inv * rx);
+                FY += g * (inv * ry);
+                FZ += g * (inv * rz);
             }
-            __syncthreads();
-            if (node) {
-                const float* a = angF + tid * 18;
-                float ax = 0.f, ay = 0.f, az = 0.f;
+            // ---- angles, propagate.cu:81-181 (each angle once, gathered in reference order)
+            if (P->constrain_angles) {
+                const float k = P->angle_k;
+                if (node) {
+                    const float* th = mesh->theta0 + 3 * tid;
+                    const int A[3] = {nn3[0], nn3[1], nn3[0]};
+                    const int B[3] = {nn3[1], nn3[2], nn3[2]};
 #pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    ax = ax - (a[q * 6 + 0] + a[q * 6 + 3]);
-                    ay = ay - (a[q * 6 + 1] + a[q * 6 + 4]);
-                    az = az - (a[q * 6 + 2] + a[q * 6 + 5]);
+                    for (int q = 0; q < 3; ++q) {
+                        st |= angle_pair(sx[A[q]] - X, sy[A[q]] - Y, sz[A[q]] - Z, sx[B[q]] - X, sy[B[q]] - Y,
+                                         sz[B[q]] - Z, th[q], k, u.angF + (tid * 3 + q) * 6);
+                    }
                 }
-                const int32_t* gt = mesh->gather + tid * 6;
+                __syncthreads();
+                if (node) {
+                    const float* a = u.angF + tid * 18;
+                    float ax = 0.f, ay = 0.f, az = 0.f;
 #pragma unroll
-                for (int q = 0; q < 6; ++q) {
-                    const float* f = angF + gt[q];
-                    ax = ax + f[0];
-                    ay = ay + f[1];
-                    az = az + f[2];
+                    for (int q = 0; q < 3; ++q) {
+                        ax = ax - (a[q * 6 + 0] + a[q * 6 + 3]);
+                        ay = ay - (a[q * 6 + 1] + a[q * 6 + 4]);
+                        az = az - (a[q * 6 + 2] + a[q * 6 + 5]);
+                    }
+                    const int32_t* gt = mesh->gather + tid * 6;
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) {
+                        const float* f = u.angF + gt[q];
+                        ax = ax + f[0];
+                        ay = ay + f[1];
+                        az = az + f[2];
+                    }
+                    FX += ax; FY += ay; FZ += az;
                 }
-                FX += ax; FY += ay; FZ += az;
             }
         }
-        float fdx = 0.f, fdy = 0.f, fdz = 0.f;
-        float vix = 0.f, viy = 0.f, viz = 0.f, nx = 0.f, ny = 0.f, nz = 0.f;
-        if (mode != M_BONDED) {
-            build_nc(p, M, P, c, cbox, rBB, nc);
-            if (tid < kNCMax) p.NCL[(int64_t)slot * kNCMax + tid] = tid < nc.nlist ? nc.list[tid] : -1;
-            if (tid == 0) p.NCN[slot] = nc.nlist;
+        if (search && node) {
+            const int nc = p.NCAND[gi];
+            const int32_t* cand = p.CAND + (int64_t)slot * kPMax * kSlots + tid;
+            const RuleCtx R = rule_ctx(M, P, X, Y, Z);
             PairConsts k;
             k.ar = P->attraction_range;
             k.ar2 = k.ar * k.ar;
@@ -485,8 +538,12 @@ __global__ __launch_bounds__(192) void k_con_dis(Pool p, Range r, int32_t mode, 
             k.ka = -P->attraction_strength * P->youngs_mod;
             k.kr = +P->repulsion_strength * P->youngs_mod;
             k.gext = P->viscotic_damping;
-            if (fused && node) {
-                vix = svx[tid]; viy = svy[tid]; viz = svz[tid];
+            float cfx = 0.f, cfy = 0.f, cfz = 0.f;
+            float fdx = 0.f, fdy = 0.f, fdz = 0.f;
+            int32_t* rec = p.PAIRS + (int64_t)slot * kPMax * kSlots + tid;
+            int np;
+            if (fused) {
+                const float vix = svx[tid], viy = svy[tid], viz = svz[tid];
                 const float gint = P->internal_damping;
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {  // bond damping, propagate.cu:547-554
@@ -495,46 +552,41 @@ __global__ __launch_bounds__(192) void k_con_dis(Pool p, Range r, int32_t mode, 
                     fdy = fdy - gint * (viy - svy[N]);
                     fdz = fdz - gint * (viz - svz[N]);
                 }
-                const int N1 = nn3[0], N2 = nn3[1], N3 = nn3[2];  // normal, propagate.cu:558-572
-                const float Ax = sx[N2] - sx[N1], Ay = sy[N2] - sy[N1], Az = sz[N2] - sz[N1];
-                const float Bx = sx[N3] - sx[N1], By = sy[N3] - sy[N1], Bz = sz[N3] - sz[N1];
-                const float cx = Ay * Bz - Az * By, cy = Az * Bx - Ax * Bz, cz = Ax * By - Ay * Bx;
-                const float inv = 1.0f / sqrtf(cx * cx + cy * cy + cz * cz);
-                const float sg = (float)mesh->sign[tid];
-                nx = sg * (inv * cx); ny = sg * (inv * cy); nz = sg * (inv * cz);
-            }
-            LaneCand L = lane_cand(p, M, P, X, Y, Z, node);
-            float cfx = 0.f, cfy = 0.f, cfz = 0.f;
-            if (fused)
-                pair_loop<true, true>(nc, M->base, M, L, node, X, Y, Z, vix, viy, viz, nx, ny, nz, k, rX, rY, rZ,
-                                      rVX, rVY, rVZ, rPB, rBB, cfx, cfy, cfz, fdx, fdy, fdz);
-            else
-                pair_loop<true, false>(nc, M->base, M, L, node, X, Y, Z, vix, viy, viz, nx, ny, nz, k, rX, rY, rZ,
-                                       rVX, rVY, rVZ, rPB, rBB, cfx, cfy, cfz, fdx, fdy, fdz);
-            if (L.ncand > 10) st |= CS3D_SYS_NN_OVER_10;
-            FX += cfx; FY += cfy; FZ += cfz;
-            if (P->use_rigid_box) {  // propagate.cu:473-510 (lower wall at 0)
-                const float E = P->youngs_mod, thr = P->thresh_dist;
-                float g1 = X, g2 = P->box_max[0] - X;
-                if (g1 < thr) FX += -100.0f * E * (g1 - thr);
-                if (g2 < thr) FX += 100.0f * E * (g2 - thr);
-                g1 = Y; g2 = P->box_max[1] - Y;
-                if (g1 < thr) FY += -100.0f * E * (g1 - thr);
-                if (g2 < thr) FY += 100.0f * E * (g2 - thr);
-                g1 = Z; g2 = P->box_max[2] - Z;
-                if (g1 < thr) FZ += -100.0f * E * (g1 - thr);
-                if (g2 < thr) FZ += 100.0f * E * (g2 - thr);
-            }
-        }
-        if (node) {
-            p.FCX[gi] = FX; p.FCY[gi] = FY; p.FCZ[gi] = FZ;
-            if (fused) {
+                float nx, ny, nz;
+                node_normal(sx, sy, sz, mesh, tid, nn3[0], nn3[1], nn3[2], nx, ny, nz);
+                np = accept_forces<true, true>(cand, nc, rec, M->base, X, Y, Z, R, rBB, vix, viy, viz, nx, ny, nz, k,
+                                               rX, rY, rZ, rVX, rVY, rVZ, cfx, cfy, cfz, fdx, fdy, fdz);
                 fdx = fdx - P->gamma_visc * vix;  // drag, propagate.cu:658
                 fdy = fdy - P->gamma_visc * viy;
                 fdz = fdz - P->gamma_visc * viz;
+            } else if (forces) {
+                np = accept_forces<true, false>(cand, nc, rec, M->base, X, Y, Z, R, rBB, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, k,
+                                                rX, rY, rZ, rVX, rVY, rVZ, cfx, cfy, cfz, fdx, fdy, fdz);
+            } else {
+                np = accept_forces<false, false>(cand, nc, rec, M->base, X, Y, Z, R, rBB, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
+                                                 k, rX, rY, rZ, rVX, rVY, rVZ, cfx, cfy, cfz, fdx, fdy, fdz);
+            }
+            p.NPAIR[gi] = np;
+            if (forces) {
+                FX += cfx; FY += cfy; FZ += cfz;
+                if (P->use_rigid_box) {  // propagate.cu:473-510 (lower wall at 0)
+                    const float E = P->youngs_mod, thr = P->thresh_dist;
+                    float g1 = X, g2 = P->box_max[0] - X;
+                    if (g1 < thr) FX += -100.0f * E * (g1 - thr);
+                    if (g2 < thr) FX += 100.0f * E * (g2 - thr);
+                    g1 = Y; g2 = P->box_max[1] - Y;
+                    if (g1 < thr) FY += -100.0f * E * (g1 - thr);
+                    if (g2 < thr) FY += 100.0f * E * (g2 - thr);
+                    g1 = Z; g2 = P->box_max[2] - Z;
+                    if (g1 < thr) FZ += -100.0f * E * (g1 - thr);
+                    if (g2 < thr) FZ += 100.0f * E * (g2 - thr);
+                }
+            }
+            if (fused) {
                 const float dt = P->dt, m = P->mass;
                 const float rdt = sqrtf(dt);
                 const float fr = 0.0f;
+                const float vix = svx[tid], viy = svy[tid], viz = svz[tid];
                 // VelocityUpdateA (propagate.cu:721-723) then VelocityUpdateB (:732-734)
                 float vx = (float)((double)vix + 0.5 * (double)(dt * FX + rdt * fr) / (double)m);
                 float vy = (float)((double)viy + 0.5 * (double)(dt * FY + rdt * fr) / (double)m);
@@ -545,6 +597,7 @@ __global__ __launch_bounds__(192) void k_con_dis(Pool p, Range r, int32_t mode, 
                 p.VX[gi] = vx; p.VY[gi] = vy; p.VZ[gi] = vz;
             }
         }
+        if (node && forces) { p.FCX[gi] = FX; p.FCY[gi] = FY; p.FCZ[gi] = FZ; }
         if (st) flag(M, st);
         __syncthreads();
     }
@@ -557,16 +610,14 @@ __global__ __launch_bounds__(192) void k_con_dis(Pool p, Range r, int32_t mode, 
 // M_VOLUMES.
 // ------------------------------------------------------------------------------
 __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode, int64_t step_override,
-                                                  const float* __restrict__ rX, const float* __restrict__ rY,
-                                                  const float* __restrict__ rZ, const float* __restrict__ rVX,
-                                                  const float* __restrict__ rVY, const float* __restrict__ rVZ,
-                                                  const float* __restrict__ rPB, const float* __restrict__ rBB) {
+                                                  int32_t clear_bins, const float* __restrict__ rX,
+                                                  const float* __restrict__ rY, const float* __restrict__ rZ,
+                                                  const float* __restrict__ rVX, const float* __restrict__ rVY,
+                                                  const float* __restrict__ rVZ) {
     __shared__ float sx[kSlots], sy[kSlots], sz[kSlots];
     __shared__ float svx[kSlots], svy[kSlots], svz[kSlots];
     __shared__ float red[3][128];
     __shared__ float cm[3];
-    __shared__ float cbox[6];
-    __shared__ NCShared nc;
     const int tid = threadIdx.x;
     const bool node = tid < kNodes;
     const MeshDev* mesh = p.mesh;
@@ -583,37 +634,24 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
         const float X = rX[gi], Y = rY[gi], Z = rZ[gi];
         sx[tid] = X; sy[tid] = Y; sz[tid] = Z;
         if (do_dis) { svx[tid] = rVX[gi]; svy[tid] = rVY[gi]; svz[tid] = rVZ[gi]; }
-        if (do_dis && mode != M_STEP) cell_box_from_patches(rPB + (int64_t)slot * kPBox, cbox);
-        if (do_dis && mode == M_STEP) {
-            if (tid == 0) nc.nlist = p.NCN[slot];
-            if (tid < kNCMax) nc.list[tid] = p.NCL[(int64_t)slot * kNCMax + tid];
-        }
+        const int np = (do_dis && node) ? p.NPAIR[gi] : 0;
         __syncthreads();
         uint32_t st = 0;
-        if (do_dis) {
-            if (mode != M_STEP) build_nc(p, M, P, c, cbox, rBB, nc);
+        if (do_dis && node) {
             float fx = 0.f, fy = 0.f, fz = 0.f;
-            float vix = 0.f, viy = 0.f, viz = 0.f, nx = 0.f, ny = 0.f, nz = 0.f;
-            int nn3[3] = {0, 0, 0};
-            if (node) {
-                vix = svx[tid]; viy = svy[tid]; viz = svz[tid];
-                const float gint = P->internal_damping;
+            const float vix = svx[tid], viy = svy[tid], viz = svz[tid];
+            const float gint = P->internal_damping;
+            int nn3[3];
 #pragma unroll
-                for (int i = 0; i < 3; ++i) {  // bond damping, propagate.cu:547-554
-                    const int N = mesh->nn[i * kSlots + tid];
-                    nn3[i] = N;
-                    fx = fx - gint * (vix - svx[N]);
-                    fy = fy - gint * (viy - svy[N]);
-                    fz = fz - gint * (viz - svz[N]);
-                }
-                const int N1 = nn3[0], N2 = nn3[1], N3 = nn3[2];  // normal, propagate.cu:558-572
-                const float Ax = sx[N2] - sx[N1], Ay = sy[N2] - sy[N1], Az = sz[N2] - sz[N1];
-                const float Bx = sx[N3] - sx[N1], By = sy[N3] - sy[N1], Bz = sz[N3] - sz[N1];
-                const float cx = Ay * Bz - Az * By, cy = Az * Bx - Ax * Bz, cz = Ax * By - Ay * Bx;
-                const float inv = 1.0f / sqrtf(cx * cx + cy * cy + cz * cz);
-                const float sg = (float)mesh->sign[tid];
-                nx = sg * (inv * cx); ny = sg * (inv * cy); nz = sg * (inv * cz);
+            for (int i = 0; i < 3; ++i) {  // bond damping, propagate.cu:547-554
+                const int N = mesh->nn[i * kSlots + tid];
+                nn3[i] = N;
+                fx = fx - gint * (vix - svx[N]);
+                fy = fy - gint * (viy - svy[N]);
+                fz = fz - gint * (viz - svz[N]);
             }
+            float nx, ny, nz;
+            node_normal(sx, sy, sz, mesh, tid, nn3[0], nn3[1], nn3[2], nx, ny, nz);
             PairConsts k;
             k.ar = P->attraction_range;
             k.ar2 = k.ar * k.ar;
@@ -621,24 +659,22 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
             k.ka = 0.f;
             k.kr = 0.f;
             k.gext = P->viscotic_damping;
-            LaneCand L = lane_cand(p, M, P, X, Y, Z, node);
             float cfx = 0.f, cfy = 0.f, cfz = 0.f;
-            pair_loop<false, true>(nc, M->base, M, L, node, X, Y, Z, vix, viy, viz, nx, ny, nz, k, rX, rY, rZ, rVX,
-                                   rVY, rVZ, rPB, rBB, cfx, cfy, cfz, fx, fy, fz);
-            if (node) {
-                fx = fx - P->gamma_visc * vix;  // drag, propagate.cu:658
-                fy = fy - P->gamma_visc * viy;
-                fz = fz - P->gamma_visc * viz;
-                p.FDX[gi] = fx; p.FDY[gi] = fy; p.FDZ[gi] = fz;
-            }
-            if (L.ncand > 10) st |= CS3D_SYS_NN_OVER_10;
+            const int32_t* lst = p.PAIRS + (int64_t)slot * kPMax * kSlots + tid;
+            pair_forces<false, true>(lst, M->base, np, X, Y, Z, vix, viy, viz, nx, ny, nz, k, rX, rY, rZ, rVX, rVY,
+                                     rVZ, cfx, cfy, cfz, fx, fy, fz);
+            fx = fx - P->gamma_visc * vix;  // drag, propagate.cu:658
+            fy = fy - P->gamma_visc * viy;
+            fz = fz - P->gamma_visc * viz;
+            p.FDX[gi] = fx; p.FDY[gi] = fy; p.FDZ[gi] = fz;
+            if (clear_bins && (mode != M_STEP || M->need_search)) clear_own_bin(p, M, X, Y, Z);
         }
         const bool do_com = mode == M_STEP || mode == M_COM;
         const int64_t stepno = step_override >= 0 ? step_override : M->step + 1;
         const bool do_vol =
             mode == M_VOLUMES || (mode == M_STEP && stepno <= P->div_time_steps && rgrowth(P, M->n) > 0.0f);
         if (do_com) {
-            // 256-slot tree of centermass.cu:32-41: level 128 and 64 through LDS, 32..1 by shuffles
+            // 256-slot tree of centermass.cu:32-41: levels 128 and 64 through LDS, 32..1 by shuffles
             if (tid < 128) {
                 const bool hi = tid + 128 < kNodes;
                 red[0][tid] = sx[tid] + (hi ? sx[tid + 128] : 0.0f);
@@ -861,9 +897,7 @@ __global__ __launch_bounds__(256) void k_sys_divide(Pool p, Range r, int32_t mod
             const int64_t pi = (int64_t)(base + rank) * kSlots + tid;
             const int64_t di = (int64_t)(base + d) * kSlots + tid;
             const float cmx = p.CMX[base + rank], cmy = p.CMY[base + rank], cmz = p.CMZ[base + rank];
-            float px = 0, py = 0, pz = 0, dx = 0, dy = 0, dz = 0;
-            const bool nd = tid < kNodes;
-            if (nd) {
+            if (tid < kNodes) {
                 const float x0 = p.X[pi], y0 = p.Y[pi], z0 = p.Z[pi];
 #pragma unroll
                 for (int side = 0; side < 2; ++side) {  // CalcAndUpdateDaughtPos, postscriptinit.cu:148-182
@@ -874,37 +908,17 @@ __global__ __launch_bounds__(256) void k_sys_divide(Pool p, Range r, int32_t mod
                     const float pd = x * Nx + y * Ny + z * Nz;
                     if (pd < 0.0f || pd == 0) { x = x - pd * Nx; y = y - pd * Ny; z = z - pd * Nz; }
                     x = x + (cmx + half * Nx); y = y + (cmy + half * Ny); z = z + (cmz + half * Nz);
-                    if (side) { dx = x; dy = y; dz = z; } else { px = x; py = y; pz = z; }
+                    const int64_t o = side ? di : pi;
+                    p.X[o] = x; p.Y[o] = y; p.Z[o] = z;
                 }
-                p.X[pi] = px; p.Y[pi] = py; p.Z[pi] = pz;
-                p.X[di] = dx; p.Y[di] = dy; p.Z[di] = dz;
                 p.VX[di] = p.VX[pi]; p.VY[di] = p.VY[pi]; p.VZ[di] = p.VZ[pi];
             } else if (tid < kSlots) {
                 p.X[di] = 0.f; p.Y[di] = 0.f; p.Z[di] = 0.f;
                 p.VX[di] = 0.f; p.VY[di] = 0.f; p.VZ[di] = 0.f;
             }
-            if (tid < kSlots) {  // a new cell's forces start at zero (never-written slots, GPUbounce.cu:563-572)
+            if (tid < kSlots) {  // a new cell's forces start at zero (never-written slots in the reference)
                 p.FCX[di] = 0.f; p.FCY[di] = 0.f; p.FCZ[di] = 0.f;
                 p.FDX[di] = 0.f; p.FDY[di] = 0.f; p.FDZ[di] = 0.f;
-            }
-            // patch boxes of the parent and the daughter
-            if (tid < kSlots) {
-                for (int side = 0; side < 2; ++side) {
-                    const float x = side ? dx : px, y = side ? dy : py, z = side ? dz : pz;
-                    float mnx = nd ? x : INFINITY, mxx = nd ? x : -INFINITY;
-                    float mny = nd ? y : INFINITY, mxy = nd ? y : -INFINITY;
-                    float mnz = nd ? z : INFINITY, mxz = nd ? z : -INFINITY;
-#pragma unroll
-                    for (int o = 8; o > 0; o >>= 1) {
-                        mnx = fminf(mnx, __shfl_xor(mnx, o, 16)); mxx = fmaxf(mxx, __shfl_xor(mxx, o, 16));
-                        mny = fminf(mny, __shfl_xor(mny, o, 16)); mxy = fmaxf(mxy, __shfl_xor(mxy, o, 16));
-                        mnz = fminf(mnz, __shfl_xor(mnz, o, 16)); mxz = fmaxf(mxz, __shfl_xor(mxz, o, 16));
-                    }
-                    if ((tid & 15) == 0) {
-                        float* pb = p.PBOX + (int64_t)(base + (side ? d : rank)) * kPBox + (tid >> 4) * 6;
-                        pb[0] = mnx; pb[1] = mxx; pb[2] = mny; pb[3] = mxy; pb[4] = mnz; pb[5] = mxz;
-                    }
-                }
             }
             if (tid == 0) {
                 const int gd = base + d, gp = base + rank;
@@ -961,44 +975,80 @@ __global__ __launch_bounds__(256) void k_sys_divide(Pool p, Range r, int32_t mod
             const int gi = P->grid_interval > 0 ? P->grid_interval : 1;
             M->rebuild = (events > 0 || stepno % gi == 0) ? 1 : 0;
             M->step = stepno;
+            M->need_search = events > 0 ? 1 : 0;  // divided cells moved: search again next step
+        } else if (events > 0) {
+            M->need_search = 1;
         }
     }
 }
 
 // ------------------------------------------------------------------------------
-// Grid rebuild: bounding_boxes (propagatebound.cu:10-141, from the patch
-// boxes), minmaxpre/minmaxpost (:145-204, 266-323) as order-free atomic min/max,
-// host Xdiv/Ydiv/Zdiv (GPUbounce.cu:1331-1334) on the device, makeNNlist
-// (:209-263) with atomic slots and a per-voxel sort into ascending order.
+// Grid rebuild: bounding_boxes (propagatebound.cu:10-141), minmaxpre/minmaxpost
+// (:145-204, 266-323) as order-free min/max (one atomic per workgroup and
+// system), host Xdiv/Ydiv/Zdiv (GPUbounce.cu:1331-1334) and the search-bin
+// geometry on the device, makeNNlist (:209-263) with atomic slots and a
+// per-voxel sort into ascending order.
 // ------------------------------------------------------------------------------
-__global__ void k_cellbox(Pool p, Range r) {
-    for (int slot = r.slot_lo + blockIdx.x * blockDim.x + threadIdx.x; slot < r.slot_hi;
-         slot += gridDim.x * blockDim.x) {
+constexpr int kBoxCells = 8;  // cells per workgroup of k_cellbox
+
+__global__ __launch_bounds__(192) void k_cellbox(Pool p, Range r) {
+    __shared__ float part[3][6];
+    __shared__ int32_t acc[6];
+    __shared__ int32_t accsys;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
+    if (tid == 0) accsys = -1;
+    __syncthreads();
+    const int first = r.slot_lo + blockIdx.x * kBoxCells;
+    for (int q = 0; q < kBoxCells; ++q) {
+        const int slot = first + q;
+        if (slot >= r.slot_hi) break;
         const int s = p.slot_sys[slot];
-        if (s < 0) continue;
-        SysMeta* M = p.meta + s;
-        if (!M->rebuild) continue;
-        const int c = slot - M->base;
-        if (c >= M->n) continue;
-        const float* pb = p.PBOX + (int64_t)slot * kPBox;
-        float b[6];
+        const SysMeta* M = s >= 0 ? p.meta + s : nullptr;
+        const bool use = M && M->rebuild && (slot - M->base) < M->n;
+        if (!use) continue;
+        const int64_t gi = (int64_t)slot * kSlots + tid;
+        const bool node = tid < kNodes;
+        float v[6];
+        const float x = p.X[gi], y = p.Y[gi], z = p.Z[gi];
+        v[0] = node ? x : INFINITY; v[1] = node ? x : -INFINITY;
+        v[2] = node ? y : INFINITY; v[3] = node ? y : -INFINITY;
+        v[4] = node ? z : INFINITY; v[5] = node ? z : -INFINITY;
 #pragma unroll
-        for (int a = 0; a < 6; ++a) b[a] = pb[a];
-        for (int q = 1; q < kNPatch; ++q) {
+        for (int o = 32; o > 0; o >>= 1) {
 #pragma unroll
             for (int a = 0; a < 6; a += 2) {
-                b[a] = fminf(b[a], pb[q * 6 + a]);
-                b[a + 1] = fmaxf(b[a + 1], pb[q * 6 + a + 1]);
+                v[a] = fminf(v[a], __shfl_xor(v[a], o));
+                v[a + 1] = fmaxf(v[a + 1], __shfl_xor(v[a + 1], o));
             }
         }
-        float* o = p.BBOX + (int64_t)slot * 6;
-#pragma unroll
-        for (int a = 0; a < 6; ++a) o[a] = b[a];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            atomicMin(&M->ext[2 * a], f2o(b[2 * a]));
-            atomicMax(&M->ext[2 * a + 1], f2o(b[2 * a + 1]));
+        if (lane == 0)
+            for (int a = 0; a < 6; ++a) part[wv][a] = v[a];
+        __syncthreads();
+        if (tid < 6) {
+            float b = part[0][tid];
+            b = (tid & 1) ? fmaxf(b, part[1][tid]) : fminf(b, part[1][tid]);
+            b = (tid & 1) ? fmaxf(b, part[2][tid]) : fminf(b, part[2][tid]);
+            p.BBOX[(int64_t)slot * 6 + tid] = b;
+            if (accsys != s) {  // flush the previous system's partial extent
+                if (accsys >= 0) {
+                    SysMeta* A = p.meta + accsys;
+                    if (tid & 1) atomicMax(&A->ext[tid], acc[tid]);
+                    else atomicMin(&A->ext[tid], acc[tid]);
+                }
+                acc[tid] = f2o(b);
+            } else {
+                acc[tid] = (tid & 1) ? max(acc[tid], f2o(b)) : min(acc[tid], f2o(b));
+            }
         }
+        __syncthreads();
+        if (tid == 0) accsys = s;
+        __syncthreads();
+    }
+    if (tid < 6 && accsys >= 0) {
+        SysMeta* A = p.meta + accsys;
+        if (tid & 1) atomicMax(&A->ext[tid], acc[tid]);
+        else atomicMin(&A->ext[tid], acc[tid]);
     }
 }
 
@@ -1026,6 +1076,27 @@ __global__ __launch_bounds__(256) void k_grid_dims(Pool p, Range r) {
         M->gmin[0] = e[0]; M->gmin[1] = e[2]; M->gmin[2] = e[4];
         M->dims[0] = ok ? xd : 1; M->dims[1] = ok ? yd : 1; M->dims[2] = ok ? zd : 1;
         if (!ok) atomicOr(&M->status, (uint32_t)CS3D_SYS_GRID_TOO_LARGE);
+        // search bins: edge a little above attraction_range (pairs within range are in
+        // adjacent bins), two bins of margin; coarser when the extent outgrows the budget
+        float h = (P->attraction_range + P->pair_skin) * 1.001f;
+        if (!(h > 0.f)) h = 1.0f;
+        const float sz[3] = {e[1] - e[0], e[3] - e[2], e[5] - e[4]};
+        int bd[3] = {5, 5, 5};
+        for (int it = 0; it < 64; ++it) {
+            int64_t nb = 1;
+            for (int a = 0; a < 3; ++a) {
+                const float q = sz[a] / h;
+                bd[a] = (q >= 0.f && q < 1.0e7f) ? (int)q + 5 : 5;
+                nb *= bd[a];
+            }
+            if (nb <= M->bbudget) break;
+            h *= 1.25f;
+        }
+        for (int a = 0; a < 3; ++a) {
+            M->bmin[a] = e[2 * a] - 2.0f * h;
+            M->bdims[a] = bd[a];
+        }
+        M->bh = h;
     }
 }
 
@@ -1113,10 +1184,20 @@ __global__ __launch_bounds__(192) void k_velocity_ab(Pool p, Range r) {
     }
 }
 
-// marks systems for a grid rebuild (per-kernel entry point and start-up)
-__global__ void k_set_rebuild(Pool p, Range r, int32_t value) {
+// plain streaming copy used to calibrate HBM byte counters
+__global__ __launch_bounds__(256) void k_copy_calibrate(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                        int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// sets the grid-rebuild and candidate-search flags of systems (-1 = unchanged)
+__global__ void k_set_flags(Pool p, Range r, int32_t rebuild, int32_t search) {
     const int s = r.sys_lo + blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < r.sys_hi && p.meta[s].active) p.meta[s].rebuild = value;
+    if (s < r.sys_hi && p.meta[s].active) {
+        if (rebuild >= 0) p.meta[s].rebuild = rebuild;
+        if (search >= 0) p.meta[s].need_search = search;
+    }
 }
 
 // ------------------------------------------------------------------------------
@@ -1131,19 +1212,23 @@ static int grid_cells(const Range& r, int per_cu) {
 void launch_integrate(const Pool& p, const Range& r, int flags, int64_t step_override, hipStream_t st) {
     hipLaunchKernelGGL(k_integrate, dim3(grid_cells(r, 16)), dim3(192), 0, st, p, r, flags, step_override);
 }
+void launch_search(const Pool& p, const Range& r, int always, hipStream_t st) {
+    hipLaunchKernelGGL(k_search, dim3(grid_cells(r, 16)), dim3(192), 0, st, p, r, always);
+}
 void launch_con(const Pool& p, const Range& r, int mode, bool use_va, hipStream_t st) {
     const float* vx = use_va ? p.VAX : p.VX;
     const float* vy = use_va ? p.VAY : p.VY;
     const float* vz = use_va ? p.VAZ : p.VZ;
-    hipLaunchKernelGGL(k_con_dis, dim3(grid_cells(r, 8)), dim3(192), 0, st, p, r, mode, p.X, p.Y, p.Z, vx, vy, vz,
-                       p.PBOX, p.BBOX);
+    hipLaunchKernelGGL(k_con_dis, dim3(grid_cells(r, 16)), dim3(192), 0, st, p, r, mode, p.X, p.Y, p.Z, vx, vy, vz,
+                       p.BBOX);
 }
-void launch_dis(const Pool& p, const Range& r, int mode, int64_t step_override, bool use_va, hipStream_t st) {
+void launch_dis(const Pool& p, const Range& r, int mode, int64_t step_override, int clear_bins, bool use_va,
+                hipStream_t st) {
     const float* vx = use_va ? p.VAX : p.VX;
     const float* vy = use_va ? p.VAY : p.VY;
     const float* vz = use_va ? p.VAZ : p.VZ;
-    hipLaunchKernelGGL(k_dis_post, dim3(grid_cells(r, 8)), dim3(192), 0, st, p, r, mode, step_override, p.X, p.Y,
-                       p.Z, vx, vy, vz, p.PBOX, p.BBOX);
+    hipLaunchKernelGGL(k_dis_post, dim3(grid_cells(r, 16)), dim3(192), 0, st, p, r, mode, step_override, clear_bins,
+                       p.X, p.Y, p.Z, vx, vy, vz);
 }
 void launch_divide(const Pool& p, const Range& r, int mode, hipStream_t st) {
     const int ns = r.sys_hi - r.sys_lo;
@@ -1152,8 +1237,9 @@ void launch_divide(const Pool& p, const Range& r, int mode, hipStream_t st) {
 void launch_grid(const Pool& p, const Range& r, int vox_blocks, hipStream_t st) {
     const int n = r.slot_hi - r.slot_lo;
     const int cb = (n + 255) / 256;
+    const int bb = (n + kBoxCells - 1) / kBoxCells;
     const int ns = r.sys_hi - r.sys_lo;
-    hipLaunchKernelGGL(k_cellbox, dim3(cb > 0 ? cb : 1), dim3(256), 0, st, p, r);
+    hipLaunchKernelGGL(k_cellbox, dim3(bb > 0 ? bb : 1), dim3(192), 0, st, p, r);
     hipLaunchKernelGGL(k_grid_dims, dim3(vox_blocks, ns > 0 ? ns : 1), dim3(256), 0, st, p, r);
     hipLaunchKernelGGL(k_insert, dim3(cb > 0 ? cb : 1), dim3(256), 0, st, p, r);
     hipLaunchKernelGGL(k_sort, dim3(vox_blocks, ns > 0 ? ns : 1), dim3(256), 0, st, p, r);
@@ -1161,10 +1247,13 @@ void launch_grid(const Pool& p, const Range& r, int vox_blocks, hipStream_t st) 
 void launch_velocity_ab(const Pool& p, const Range& r, hipStream_t st) {
     hipLaunchKernelGGL(k_velocity_ab, dim3(grid_cells(r, 8)), dim3(192), 0, st, p, r);
 }
-void launch_set_rebuild(const Pool& p, const Range& r, int value, hipStream_t st) {
+void launch_set_flags(const Pool& p, const Range& r, int rebuild, int search, hipStream_t st) {
     const int ns = r.sys_hi - r.sys_lo;
-    hipLaunchKernelGGL(k_set_rebuild, dim3((ns + 255) / 256 > 0 ? (ns + 255) / 256 : 1), dim3(256), 0, st, p, r,
-                       value);
+    hipLaunchKernelGGL(k_set_flags, dim3((ns + 255) / 256 > 0 ? (ns + 255) / 256 : 1), dim3(256), 0, st, p, r, rebuild,
+                       search);
+}
+void launch_copy_calibrate(const void* src, void* dst, int64_t n4, hipStream_t st) {
+    hipLaunchKernelGGL(k_copy_calibrate, dim3(2048), dim3(256), 0, st, (const float4*)src, (float4*)dst, n4);
 }
 
 }  // namespace cs3d

@@ -22,6 +22,8 @@ PP = C.POINTER(abi.Params)
 MP = C.POINTER(abi.Mesh)
 RP = C.POINTER(abi.RanmarState)
 st_t = C.c_int
+PHASES = 5
+PHASE_NAMES = ("integrate", "con_dis", "dis_post", "divide", "grid")
 
 # every symbol declared in include/cs3d.h with its (argtypes, restype)
 SIGNATURES = {
@@ -33,7 +35,7 @@ SIGNATURES = {
     "cs3d_mesh_build": ([vp, vp, vp, vp, vp, vp, MP], st_t),
     "cs3d_ranmar_init": ([RP, C.c_int32, C.c_int32], st_t),
     "cs3d_ranmar": ([RP, vp, C.c_int32], st_t),
-    "cs3d_ctx_create": ([C.c_int, MP, C.c_int64, C.c_int64, C.POINTER(vp)], st_t),
+    "cs3d_ctx_create": ([C.c_int, MP, C.c_int64, C.c_int64, C.c_int64, C.POINTER(vp)], st_t),
     "cs3d_ctx_destroy": ([vp], st_t),
     "cs3d_sync": ([vp], st_t),
     "cs3d_system_add": ([vp, PP, C.c_int32, C.c_int32, C.c_int64, vp, vp, vp, vp, vp, vp, C.POINTER(C.c_int32)],
@@ -48,6 +50,8 @@ SIGNATURES = {
     "cs3d_field_bytes": ([vp, C.c_int32, C.c_int], C.c_size_t),
     "cs3d_step": ([vp, C.c_int32, C.c_int32], st_t),
     "cs3d_set_graph": ([vp, C.c_int32], st_t),
+    "cs3d_set_graph_steps": ([vp, C.c_int32], st_t),
+    "cs3d_step_timed": ([vp, C.c_int32, C.c_int32, vp], st_t),
     "cs3d_k_integrate": ([vp, C.c_int32], st_t),
     "cs3d_k_pressure_update": ([vp, C.c_int32, C.c_int64], st_t),
     "cs3d_k_con_force": ([vp, C.c_int32], st_t),
@@ -58,6 +62,7 @@ SIGNATURES = {
     "cs3d_k_volumes": ([vp, C.c_int32, C.c_int64], st_t),
     "cs3d_k_grid": ([vp, C.c_int32], st_t),
     "cs3d_k_divide": ([vp, C.c_int32, C.POINTER(C.c_int32)], st_t),
+    "cs3d_copy_calibrate": ([vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)], st_t),
     "cs3d_traj_open": ([C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(vp)], st_t),
     "cs3d_traj_append": ([vp, C.c_int32, C.c_int32, C.c_int32, vp, vp, vp, vp], st_t),
     "cs3d_traj_close": ([vp, C.c_int32], st_t),
@@ -115,12 +120,14 @@ def build_mesh(tables, rest):
 
 
 class Context:
-    def __init__(self, mesh, capacity_cells, voxel_budget=None, device=0):
+    def __init__(self, mesh, capacity_cells, voxel_budget=None, device=0, bin_budget=None):
         if voxel_budget is None:
             voxel_budget = max(1 << 20, capacity_cells * 32 * 40)
+        if bin_budget is None:
+            bin_budget = max(1 << 22, capacity_cells * 640 * 65)
         h = vp()
-        check(lib().cs3d_ctx_create(device, C.byref(mesh), int(capacity_cells), int(voxel_budget), C.byref(h)),
-              "cs3d_ctx_create")
+        check(lib().cs3d_ctx_create(device, C.byref(mesh), int(capacity_cells), int(voxel_budget), int(bin_budget),
+                                    C.byref(h)), "cs3d_ctx_create")
         self.h = h
         self.mesh = mesh
         self.systems = {}
@@ -169,6 +176,23 @@ class Context:
     def set_graph(self, enable):
         check(lib().cs3d_set_graph(self.h, 1 if enable else 0), "cs3d_set_graph")
 
+    def set_graph_steps(self, steps):
+        check(lib().cs3d_set_graph_steps(self.h, int(steps)), "cs3d_set_graph_steps")
+
+    def copy_calibrate(self, nbytes, reps=10):
+        """Average ms of a plain copy kernel over nbytes (reads nbytes, writes nbytes)."""
+        out = C.c_double()
+        check(lib().cs3d_copy_calibrate(self.h, int(nbytes), int(reps), C.byref(out)), "cs3d_copy_calibrate")
+        return out.value
+
+    def step_timed(self, n_steps, target=abi.ALL):
+        """Eager steps with HIP events around each phase; returns total ms per phase
+        (integrate, con_dis, dis_post, divide, grid)."""
+        sid = target.id if isinstance(target, System) else target
+        out = np.zeros(PHASES, dtype=np.float64)
+        check(lib().cs3d_step_timed(self.h, sid, int(n_steps), out.ctypes.data_as(vp)), "cs3d_step_timed")
+        return out
+
 
 class System:
     def __init__(self, ctx, sid, params, max_cells):
@@ -200,7 +224,7 @@ class System:
     def _dtype(self, f):
         if f == abi.F_DIVFLAG:
             return np.int8
-        if f in (abi.F_GRID, abi.F_VOXCOUNT, abi.F_VOXLIST):
+        if f in (abi.F_GRID, abi.F_VOXCOUNT, abi.F_VOXLIST, abi.F_PAIRCOUNT, abi.F_PAIRS):
             return np.int32
         return np.float32
 

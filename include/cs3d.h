@@ -112,7 +112,10 @@ typedef struct cs3d_params {
     int32_t grid_interval;         /* rebuild bbox + voxel grid every k steps (1 = reference) */
     int32_t daughter_com_fix;      /* 0 = reference: a new daughter's centre of mass stays 0 until the next
                                       CenterOfMass; 1 = set it from the daughter's nodes at division */
-    int32_t reserved[7];
+    float pair_skin;               /* engine: contact candidates are searched within attraction_range +
+                                      pair_skin and re-searched once a node has moved pair_skin/2; every step
+                                      applies the exact reference tests, so results do not depend on it */
+    int32_t reserved[6];
 } cs3d_params;
 
 /* Fields readable/writable with cs3d_get / cs3d_set.  Sizes are per system
@@ -134,7 +137,9 @@ typedef enum cs3d_field {
     CS3D_F_VOXCOUNT,                           /* int32 per voxel (budget entries), d_NoofNNlist */
     CS3D_F_VOXLIST,                            /* int32 budget*capacity, d_NNlist (each list ascending) */
     CS3D_F_RANMAR,                             /* cs3d_ranmar_state */
-    CS3D_F_PATCHBOX,                           /* 12 patches x 6 floats per cell (engine-specific) */
+    CS3D_F_PAIRCOUNT,                          /* int32 per node: stored contact partners (engine-specific) */
+    CS3D_F_PAIRS,                              /* int32 32 x 192 per cell: partner keys cell*192 + node,
+                                                  ascending, k-major (engine-specific) */
     CS3D_F_GROWTH_SCALE,                       /* per-cell multiplier of the pressure increment (default 1) */
     CS3D_F__COUNT
 } cs3d_field;
@@ -171,10 +176,12 @@ cs3d_status cs3d_ranmar_init(cs3d_ranmar_state* st, int32_t ij, int32_t kl);
 cs3d_status cs3d_ranmar(cs3d_ranmar_state* st, float* out, int32_t n);
 
 /* ---- context ------------------------------------------------------------ */
-/* Allocates a pool of capacity_cells cell slots and voxel_budget voxels on
- * `device`.  Replaces the cudaMalloc block of main (GPUbounce.cu:358-615). */
+/* Allocates a pool of capacity_cells cell slots, a voxel pool of voxel_budget
+ * int32 words (voxel counts and lists of all systems) and a search-bin pool of
+ * bin_budget int32 words on `device`.  Replaces the cudaMalloc block of main
+ * (GPUbounce.cu:358-615). */
 cs3d_status cs3d_ctx_create(int device, const cs3d_mesh* mesh, int64_t capacity_cells,
-                            int64_t voxel_budget, cs3d_ctx** out);
+                            int64_t voxel_budget, int64_t bin_budget, cs3d_ctx** out);
 cs3d_status cs3d_ctx_destroy(cs3d_ctx* ctx);
 cs3d_status cs3d_sync(cs3d_ctx* ctx);
 
@@ -210,6 +217,22 @@ size_t cs3d_field_bytes(cs3d_ctx* ctx, cs3d_sys sys, cs3d_field f);
 cs3d_status cs3d_step(cs3d_ctx* ctx, cs3d_sys sys, int32_t n_steps);
 /* Enables (1) or disables (0) HIP graph capture for cs3d_step. */
 cs3d_status cs3d_set_graph(cs3d_ctx* ctx, int32_t enable);
+/* Number of steps captured per HIP graph (default 10). */
+cs3d_status cs3d_set_graph_steps(cs3d_ctx* ctx, int32_t steps);
+
+/* Phases of one step, in launch order. */
+#define CS3D_PHASES 5
+#define CS3D_PH_INTEGRATE 0  /* Integrate + ForwardTime + PressureUpdate; contact-candidate search
+                                (search bins + candidate lists) on the steps that need it */
+#define CS3D_PH_CON_DIS 1    /* CalculateConForce + first CalculateDisForce + VelocityUpdateA/B */
+#define CS3D_PH_DIS_POST 2   /* second CalculateDisForce + CenterOfMass + volumes */
+#define CS3D_PH_DIVIDE 3     /* division phase and end of step */
+#define CS3D_PH_GRID 4       /* bounding boxes + extent + voxel lists */
+/* Runs n_steps eagerly with HIP events recorded on the context's stream
+ * around each phase and adds the elapsed milliseconds of every phase to
+ * phase_ms[CS3D_PHASES] (no counterpart in the reference, which has no timers;
+ * it replaces profiling by hand). Synchronous. */
+cs3d_status cs3d_step_timed(cs3d_ctx* ctx, cs3d_sys sys, int32_t n_steps, double* phase_ms);
 
 /* ---- per-kernel entry points (state injection / parity) ------------------
  * Each runs one reference kernel (or fixed kernel group) on the device state
@@ -239,6 +262,13 @@ cs3d_status cs3d_k_grid(cs3d_ctx* ctx, cs3d_sys sys);
  * postscriptinit.cu:148-256, PressureKernels.cu:45-65). Returns events via
  * n_events (nullable). */
 cs3d_status cs3d_k_divide(cs3d_ctx* ctx, cs3d_sys sys, int32_t* n_events);
+
+/* ---- measurement helpers (no counterpart in the reference) ---------------
+ * Copies `bytes` bytes between two scratch buffers of that size with a plain
+ * float4 copy kernel (k_copy_calibrate), `reps` times, and returns the average
+ * launch time in ms: the known-size kernel used to scale HBM byte counters.
+ * Allocates and frees its buffers. */
+cs3d_status cs3d_copy_calibrate(cs3d_ctx* ctx, int64_t bytes, int32_t reps, double* avg_ms);
 
 /* ---- trajectory (binary format of GPUbounce.cu:1007-1018, 2446-2477) ----- */
 typedef struct cs3d_traj cs3d_traj;
