@@ -220,15 +220,3 @@ def test_graph_equals_eager(oracle_mesh):
             assert np.array_equal(s1.get(f), s2.get(f))
     c1.close()
     c2.close()
-
-
-def test_device_ranmar_and_kat(oracle_mesh):
-    """Host RANMAR of libcs3d reproduces the known-answer test of marsaglia.h:68-74."""
-    A, E = _abi(), _engine()
-    st = A.RanmarState()
-    E.check(E.lib().cs3d_ranmar_init(C.byref(st), 1802, 9373), "init")
-    buf = np.zeros(20000, np.float32)
-    E.check(E.lib().cs3d_ranmar(C.byref(st), buf.ctypes.data_as(C.c_void_p), 20000), "draw")
-    six = np.zeros(6, np.float32)
-    E.check(E.lib().cs3d_ranmar(C.byref(st), six.ctypes.data_as(C.c_void_p), 6), "draw")
-    assert list(six * 4096 * 4096) == [6533892.0, 14220222.0, 7275067.0, 6172232.0, 8354498.0, 10633180.0]

@@ -1,0 +1,148 @@
+"""The C ABI of libcs3d on the CPU: the library loads, exports every function
+declared in include/cs3d.h, and its host-side helpers (defaults, validation,
+mesh rest state, RANMAR, trajectory writer) behave as the reference's host code.
+No call here touches a GPU."""
+import ctypes as C
+import os
+import re
+import struct
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+from cellsim3d_amd import abi, configs, engine
+from cellsim3d_amd import mesh as M
+
+HEADER = os.path.join(ROOT, "include", "cs3d.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(cs3d_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_function():
+    names = declared_functions()
+    assert len(names) >= 30
+    L = C.CDLL(engine.LIB_PATH)
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # the Python binding types exactly the declared set
+    assert sorted(engine.SIGNATURES) == names
+
+
+def test_status_strings_and_version():
+    L = engine.lib()
+    for s in range(abi.UNSUPPORTED + 1):
+        assert L.cs3d_status_str(s)
+    assert L.cs3d_version().decode().strip()
+
+
+def test_ranmar_known_answer_through_the_abi():
+    """marsaglia.h:68-74, via cs3d_ranmar_init / cs3d_ranmar."""
+    L = engine.lib()
+    st = abi.RanmarState()
+    engine.check(L.cs3d_ranmar_init(C.byref(st), 1802, 9373), "init")
+    buf = np.zeros(20000, np.float32)
+    engine.check(L.cs3d_ranmar(C.byref(st), buf.ctypes.data_as(C.c_void_p), 20000), "draw")
+    six = np.zeros(6, np.float32)
+    engine.check(L.cs3d_ranmar(C.byref(st), six.ctypes.data_as(C.c_void_p), 6), "draw")
+    assert list(six.astype(np.float64) * 4096 * 4096) == [6533892.0, 14220222.0, 7275067.0, 6172232.0,
+                                                          8354498.0, 10633180.0]
+    assert L.cs3d_ranmar_init(C.byref(st), 1, 30082) == abi.INVALID_ARG
+
+
+def test_params_default_matches_inp_json_defaults():
+    got = engine.default_params().as_dict()
+    want = abi.default_params().as_dict()
+    for k, v in want.items():
+        if k == "reserved":
+            continue
+        assert np.allclose(got[k], v, rtol=0, atol=0), k
+
+
+@pytest.mark.parametrize("field,value", [
+    ("mass", 0.0), ("dt", -1e-4), ("grid_dl", 0.0), ("attraction_range", -0.1), ("voxel_capacity", 0),
+    ("grid_interval", 0), ("pair_skin", -0.01), ("ranmar_kl", 30082), ("ranmar_ij", -1),
+])
+def test_params_check_rejects(field, value):
+    p = engine.default_params()
+    assert engine.lib().cs3d_params_check(C.byref(p)) == abi.OK
+    setattr(p, field, value)
+    assert engine.lib().cs3d_params_check(C.byref(p)) == abi.INVALID_ARG
+    assert engine.lib().cs3d_last_error()
+
+
+def test_params_check_rejects_random_forcing():
+    p = engine.default_params()
+    p.add_rands = 1
+    assert engine.lib().cs3d_params_check(C.byref(p)) == abi.UNSUPPORTED
+
+
+def test_mesh_build_matches_oracle_rest_state(gp30):
+    from oracle import oracle as O
+    tables = M.tables(gp30)
+    rest = configs.rest_positions(gp30)
+    a = engine.build_mesh(tables, rest)
+    b = O.build_mesh(tables, rest)
+    for f in ("nn", "sign", "faces"):
+        assert np.array_equal(np.ctypeslib.as_array(getattr(a, f)), np.ctypeslib.as_array(getattr(b, f)))
+    for f in ("R0", "theta0"):
+        x, y = np.ctypeslib.as_array(getattr(a, f)), np.ctypeslib.as_array(getattr(b, f))
+        assert np.max(np.abs(x - y)) <= 2 * np.spacing(np.float32(np.max(np.abs(y)))), f
+    # rest lengths are the bond lengths; rest angles lie between 100 and 130 degrees
+    R0 = np.ctypeslib.as_array(a.R0).reshape(3, abi.SLOTS)[:, :180]
+    th = np.degrees(np.ctypeslib.as_array(a.theta0)[:540])
+    x = gp30["x"]
+    nn = gp30["nn"]
+    for k in range(3):
+        assert np.allclose(R0[k], np.linalg.norm(x - x[nn[:, k]], axis=1), rtol=1e-6)
+    assert 100 < th.min() and th.max() < 130
+
+
+def test_mesh_build_rejects_bad_indices(gp30):
+    nn, sign, faces = (t.copy() for t in M.tables(gp30))
+    nn[5] = 180
+    rest = configs.rest_positions(gp30)
+    with pytest.raises(engine.EngineError):
+        engine.build_mesh((nn, sign, faces), rest)
+
+
+def _traj_reference_bytes(max_cells, var, max_frames, frames, final_cells):
+    """The byte layout of GPUbounce.cu:1007-1018, 2446-2477 and 1508-1511,
+    assembled field by field (little-endian int32 / float32)."""
+    out = bytearray(struct.pack("<3i", final_cells, var, max_frames))
+    for step, frame, X, Y, Z, types in frames:
+        n = len(X) // abi.SLOTS
+        out += struct.pack("<3i", step, frame, n)
+        for c in range(n):
+            out += struct.pack("<i", c)
+            for A in (X, Y, Z):
+                out += struct.pack("<192f", *A[c * abi.SLOTS:(c + 1) * abi.SLOTS])
+            if var:
+                out += struct.pack("<i", int(types[c]))
+    return bytes(out)
+
+
+@pytest.mark.parametrize("var", [0, 1])
+def test_trajectory_byte_layout(tmp_path, var):
+    L = engine.lib()
+    rng = np.random.default_rng(var)
+    frames = []
+    for step, frame, n in ((0, 1, 2), (100, 2, 3)):
+        X, Y, Z = (rng.normal(size=n * abi.SLOTS).astype(np.float32) for _ in range(3))
+        types = np.arange(n, dtype=np.int32) % 2
+        frames.append((step, frame, X, Y, Z, types))
+    path = str(tmp_path / "traj.bin")
+    h = C.c_void_p()
+    engine.check(L.cs3d_traj_open(path.encode(), 10000, var, 7, C.byref(h)), "open")
+    for step, frame, X, Y, Z, types in frames:
+        engine.check(L.cs3d_traj_append(h, step, frame, len(X) // abi.SLOTS, X.ctypes.data_as(C.c_void_p),
+                                        Y.ctypes.data_as(C.c_void_p), Z.ctypes.data_as(C.c_void_p),
+                                        types.ctypes.data_as(C.c_void_p)), "append")
+    engine.check(L.cs3d_traj_close(h, 3), "close")
+    got = open(path, "rb").read()
+    assert got == _traj_reference_bytes(10000, var, 7, frames, 3)
