@@ -21,6 +21,7 @@ void launch_dis(const Pool& p, const Range& r, int mode, int64_t step_override, 
                 hipStream_t st);
 void launch_divide(const Pool& p, const Range& r, int mode, hipStream_t st);
 void launch_grid(const Pool& p, const Range& r, int vox_blocks, hipStream_t st);
+void launch_vox_lists(const Pool& p, const Range& r, int vox_blocks, hipStream_t st);
 void launch_velocity_ab(const Pool& p, const Range& r, hipStream_t st);
 void launch_set_flags(const Pool& p, const Range& r, int rebuild, int search, hipStream_t st);
 void launch_copy_calibrate(const void* src, void* dst, int64_t n4, hipStream_t st);
@@ -158,15 +159,31 @@ cs3d_status write_meta(cs3d_ctx* c, cs3d_sys s, const SysMeta& m) {
     return CS3D_OK;
 }
 
-void enqueue_step(cs3d_ctx* c, const Range& r, int vb) {
+// One step of the loop in GPUbounce.cu:1055-1506 as five launches:
+//   integrate  Integrate + ForwardTime + PressureUpdate; per-cell part of a grid
+//              rebuilt at the end of the last step; search bins (search steps)
+//   search     contact candidates (steps flagged by the previous step)
+//   con_dis    CalculateConForce + CalculateDisForce (post-Integrate velocities) +
+//              VelocityUpdateA/B
+//   dis_post   CalculateDisForce (end velocities) + CenterOfMass [+ volumes] +
+//              bounding_boxes + next-step search decision
+//   divide     division phase, step bookkeeping, extent and grid geometry
+void enqueue_step_phase(cs3d_ctx* c, const Range& r, int phase) {
     const Pool& p = c->pool;
-    launch_integrate(p, r, I_KICK | I_PRESSURE | I_DISP, -1, c->stream);
-    launch_integrate(p, r, I_BINS | I_IFSEARCH, -1, c->stream);
-    launch_search(p, r, 0, c->stream);
-    launch_con(p, r, M_STEP, true, c->stream);
-    launch_dis(p, r, M_STEP, -1, 1, false, c->stream);
-    launch_divide(p, r, M_STEP, c->stream);
-    launch_grid(p, r, vb, c->stream);
+    switch (phase) {
+        case CS3D_PH_INTEGRATE:
+            launch_integrate(p, r, I_KICK | I_PRESSURE | I_BINS | I_IFSEARCH | I_GRIDCOUNT, -1, c->stream);
+            break;
+        case CS3D_PH_SEARCH: launch_search(p, r, 0, c->stream); break;
+        case CS3D_PH_CON_DIS: launch_con(p, r, M_STEP, true, c->stream); break;
+        case CS3D_PH_DIS_POST: launch_dis(p, r, M_STEP, -1, 1, false, c->stream); break;
+        case CS3D_PH_DIVIDE: launch_divide(p, r, M_STEP, c->stream); break;
+        default: break;
+    }
+}
+
+void enqueue_step(cs3d_ctx* c, const Range& r, int /*vb*/) {
+    for (int q = 0; q < CS3D_PHASES; ++q) enqueue_step_phase(c, r, q);
 }
 
 cs3d_status run_startup(cs3d_ctx* c, cs3d_sys s) {
@@ -291,6 +308,7 @@ extern "C" cs3d_status cs3d_ctx_create(int device, const cs3d_mesh* mesh, int64_
         if ((st = dalloc(c, *a, capacity_cells)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.FLAG, capacity_cells)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.BBOX, capacity_cells * 6)) != CS3D_OK) return cleanup(st);
+    if ((st = dalloc(c, p.BBOXN, capacity_cells * 6)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.PAIRS, capacity_cells * kPMax * kSlots)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.NPAIR, capacity_cells * kSlots)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.CAND, capacity_cells * kPMax * kSlots)) != CS3D_OK) return cleanup(st);
@@ -386,6 +404,7 @@ extern "C" cs3d_status cs3d_system_add_raw(cs3d_ctx* c, const cs3d_params* prm, 
     for (float* a : cellz) HIPCHK(hipMemsetAsync(a + base, 0, (size_t)max_cells * 4, c->stream));
     HIPCHK(hipMemsetAsync(p.FLAG + base, 0, max_cells, c->stream));
     HIPCHK(hipMemsetAsync(p.BBOX + base * 6, 0, (size_t)max_cells * 24, c->stream));
+    HIPCHK(hipMemsetAsync(p.BBOXN + base * 6, 0, (size_t)max_cells * 24, c->stream));
     HIPCHK(hipMemsetAsync(p.NPAIR + nb, 0, (size_t)max_cells * kSlots * 4, c->stream));
     HIPCHK(hipMemsetAsync(p.vox + vreg, 0, (size_t)vb * (1 + vcap) * 4, c->stream));
     HIPCHK(hipMemsetAsync(p.bins + breg, 0, (size_t)bb * 4, c->stream));
@@ -535,6 +554,15 @@ extern "C" cs3d_status cs3d_get(cs3d_ctx* c, cs3d_sys s, cs3d_field f, void* hos
         }
         return CS3D_OK;
     }
+    if (f == CS3D_F_BBOX || f == CS3D_F_VOXCOUNT || f == CS3D_F_VOXLIST) {
+        // the fused step keeps voxel counts only and leaves the per-cell part of a
+        // grid rebuilt in the last step to the next step: complete it and fill the
+        // sorted voxel lists of the current grid
+        const Range r = range_of(c, s);
+        launch_vox_lists(c->pool, r, vox_blocks_of(c, s), c->stream);
+        launch_set_flags(c->pool, r, 0, -1, c->stream);
+        HIPCHK(hipGetLastError());
+    }
     HIPCHK(hipMemcpyAsync(host, L.ptr, bytes, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return CS3D_OK;
@@ -593,25 +621,15 @@ extern "C" cs3d_status cs3d_step_timed(cs3d_ctx* c, cs3d_sys s, int32_t n_steps,
     if (s != CS3D_ALL && !valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
     const Range r = range_of(c, s);
     if (r.slot_hi <= r.slot_lo || r.sys_hi <= r.sys_lo) return CS3D_OK;
-    const int vb = vox_blocks_of(c, s);
-    const Pool& p = c->pool;
     hipEvent_t ev[CS3D_PHASES + 1];
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     cs3d_status st = CS3D_OK;
     for (int k = 0; k < n_steps && st == CS3D_OK; ++k) {
         (void)hipEventRecord(ev[0], c->stream);
-        launch_integrate(p, r, I_KICK | I_PRESSURE | I_DISP, -1, c->stream);
-        launch_integrate(p, r, I_BINS | I_IFSEARCH, -1, c->stream);
-        launch_search(p, r, 0, c->stream);
-        (void)hipEventRecord(ev[1], c->stream);
-        launch_con(p, r, M_STEP, true, c->stream);
-        (void)hipEventRecord(ev[2], c->stream);
-        launch_dis(p, r, M_STEP, -1, 1, false, c->stream);
-        (void)hipEventRecord(ev[3], c->stream);
-        launch_divide(p, r, M_STEP, c->stream);
-        (void)hipEventRecord(ev[4], c->stream);
-        launch_grid(p, r, vb, c->stream);
-        (void)hipEventRecord(ev[5], c->stream);
+        for (int q = 0; q < CS3D_PHASES; ++q) {
+            enqueue_step_phase(c, r, q);
+            (void)hipEventRecord(ev[q + 1], c->stream);
+        }
         hipError_t e = hipEventSynchronize(ev[5]);
         if (e != hipSuccess) { st = fail(CS3D_HIP_ERROR, hipGetErrorString(e)); break; }
         for (int q = 0; q < CS3D_PHASES; ++q) {

@@ -5,6 +5,7 @@
 //                (SoA, 192-slot stride per cell as the reference's d_X, GPUbounce.cu:365-373)
 //   cell arrays  float[cap_cells]        CM x/y/z, pressure, stiffness, volume, area, growth scale
 //                float[cap_cells * 6]    bbox used by the contact candidate test (rebuilt with the grid)
+//                float[cap_cells * 6]    bbox of the positions at the end of the latest step
 //   pair lists   int32[cap_cells * kPMax * 192]  contact partners of each node, (cell, node)
 //                ascending, k-major per cell so lanes access consecutive words
 //                int32[cap_cells * 192]  partner count per node
@@ -40,7 +41,9 @@ struct SysMeta {
     int64_t step;       // completed steps
     int64_t horizon;    // stop after this many steps (-1 = none)
     uint32_t status;    // CS3D_SYS_* bits (sticky)
-    int32_t rebuild;    // 1 = rebuild bbox + grid at the end of the current step
+    int32_t rebuild;    // fused step: 1 = the grid was rebuilt at the end of the last step and its
+                        // per-cell part (bbox adoption, voxel counts) is still to be done by the
+                        // next step; explicit grid kernels: 1 = rebuild now
     int64_t vbase;      // offset of counts[] in the voxel pool
     int64_t lbase;      // offset of lists[] in the voxel pool
     int32_t vbudget;    // voxels available
@@ -56,6 +59,7 @@ struct SysMeta {
     float bh;           // bin edge
     int32_t bdims[3];   // bins per axis
     int32_t need_search;  // 1 = search contact candidates this step (a node moved pair_skin/2)
+    int32_t search_next;  // set during a step when the next step's drift takes a node pair_skin/2 away
     cs3d_ranmar_state rng;
 };
 
@@ -81,6 +85,7 @@ struct Pool {
     float *P, *E, *VOL, *AREA, *GS;
     int8_t* FLAG;
     float* BBOX;
+    float* BBOXN;
     int32_t* PAIRS;
     int32_t* NPAIR;
     int32_t* CAND;
@@ -122,6 +127,8 @@ enum : int32_t {
     I_CLEAR = 8,      // clear the search-bin counters of the nodes
     I_DISP = 16,      // flag a candidate search when a node moved pair_skin/2 since the last one
     I_IFSEARCH = 32,  // I_BINS only on systems that search this step
+    I_GRIDCOUNT = 64, // systems whose grid was rebuilt at the end of the last step: adopt the new
+                      // bboxes and count the voxel occupancy (overflow check)
 };
 
 }  // namespace cs3d

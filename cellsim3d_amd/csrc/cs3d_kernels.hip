@@ -77,6 +77,69 @@ __device__ __forceinline__ int f2o(float f) {
 }
 __device__ __forceinline__ float o2f(int i) { return __int_as_float(i >= 0 ? i : i ^ 0x7FFFFFFF); }
 
+// Integrate's velocity half-kick of one component (propagate.cu:698-705): the
+// sum in float, the rest in double (0.5 is a double literal), rounded to float.
+__device__ __forceinline__ float half_kick(float v, float fc, float fd, float dt, float rdt, float m) {
+    const float fr = 0.0f;  // random force disabled (add_rands = 0)
+    return (float)((double)v + 0.5 * (double)(dt * fc + dt * fd + rdt * fr) / (double)m);
+}
+
+// makeNNlist's voxel range of a cell (propagatebound.cu:224-245): the voxels of
+// its bbox widened by attraction_range, clamped to the grid.
+struct VoxRange {
+    int sx, ex, sy, ey, sz, ez;
+};
+__device__ __forceinline__ VoxRange vox_range(const float* b, const SysMeta* M, const cs3d_params* P) {
+    const float DL = P->grid_dl, at = P->attraction_range;
+    const float mx = M->gmin[0], my = M->gmin[1], mz = M->gmin[2];
+    VoxRange v;
+    v.sx = (int)((b[0] - at - mx) / DL); if (v.sx < 0) v.sx = 0;
+    v.ex = (int)((b[1] + at - mx) / DL); if (v.ex >= M->dims[0]) v.ex = M->dims[0] - 1;
+    v.sy = (int)((b[2] - at - my) / DL); if (v.sy < 0) v.sy = 0;
+    v.ey = (int)((b[3] + at - my) / DL); if (v.ey >= M->dims[1]) v.ey = M->dims[1] - 1;
+    v.sz = (int)((b[4] - at - mz) / DL); if (v.sz < 0) v.sz = 0;
+    v.ez = (int)((b[5] + at - mz) / DL); if (v.ez >= M->dims[2]) v.ez = M->dims[2] - 1;
+    return v;
+}
+
+// Grid geometry from the global extent e = {minx, maxx, miny, maxy, minz, maxz}:
+// host Xdiv/Ydiv/Zdiv (GPUbounce.cu:1331-1334) with the voxel origin at the
+// extent's minimum, and the search bins (edge a little above attraction_range +
+// pair_skin so that every pair within that range lies in adjacent bins, two bins
+// of margin, coarser when the extent outgrows the bin budget).  Returns the
+// number of voxel counters to clear.  Called by one thread.
+__device__ int64_t set_grid_geometry(SysMeta* M, const cs3d_params* P, const float* e) {
+    const float DL = P->grid_dl;
+    const int xd = (int)((e[1] - e[0]) / DL + 1.0f);
+    const int yd = (int)((e[3] - e[2]) / DL + 1.0f);
+    const int zd = (int)((e[5] - e[4]) / DL + 1.0f);
+    const int64_t nv = (int64_t)xd * yd * zd;
+    const bool ok = xd > 0 && yd > 0 && zd > 0 && nv <= M->vbudget;
+    M->gmin[0] = e[0]; M->gmin[1] = e[2]; M->gmin[2] = e[4];
+    M->dims[0] = ok ? xd : 1; M->dims[1] = ok ? yd : 1; M->dims[2] = ok ? zd : 1;
+    if (!ok) atomicOr(&M->status, (uint32_t)CS3D_SYS_GRID_TOO_LARGE);
+    float h = (P->attraction_range + P->pair_skin) * 1.001f;
+    if (!(h > 0.f)) h = 1.0f;
+    const float sz[3] = {e[1] - e[0], e[3] - e[2], e[5] - e[4]};
+    int bd[3] = {5, 5, 5};
+    for (int it = 0; it < 64; ++it) {
+        int64_t nb = 1;
+        for (int a = 0; a < 3; ++a) {
+            const float q = sz[a] / h;
+            bd[a] = (q >= 0.f && q < 1.0e7f) ? (int)q + 5 : 5;
+            nb *= bd[a];
+        }
+        if (nb <= M->bbudget) break;
+        h *= 1.25f;
+    }
+    for (int a = 0; a < 3; ++a) {
+        M->bmin[a] = e[2 * a] - 2.0f * h;
+        M->bdims[a] = bd[a];
+    }
+    M->bh = h;
+    return ok ? nv : (int64_t)M->vbudget;
+}
+
 // ------------------------------------------------------------------------------
 // Integrate + ForwardTime (propagate.cu:681-712, 738-763), PressureUpdate
 // (PressureKernels.cu:4-41) and insertion of the new positions into the
@@ -92,18 +155,38 @@ __global__ __launch_bounds__(192) void k_integrate(Pool p, Range r, int32_t flag
         if (!sys_live(M)) continue;
         const int c = slot - M->base;
         if (c >= M->n) continue;
-        if ((flags & I_IFSEARCH) && !M->need_search) continue;
         const cs3d_params* P = p.prm + s;
         const int64_t gi = (int64_t)slot * kSlots + tid;
+        if ((flags & I_GRIDCOUNT) && M->rebuild && !(M->status & CS3D_SYS_GRID_TOO_LARGE)) {
+            // the grid was rebuilt at the end of the last step: this cell's new bbox
+            // becomes the one the contact rule uses, and its voxels are counted
+            // (makeNNlist's atomicAdd, propagatebound.cu:246-250) to catch overflow
+            const float* bn = p.BBOXN + (int64_t)slot * 6;
+            float b[6];
+#pragma unroll
+            for (int a = 0; a < 6; ++a) b[a] = bn[a];
+            if (tid < 6) p.BBOX[(int64_t)slot * 6 + tid] = b[tid];
+            const VoxRange v = vox_range(b, M, P);
+            const int nx = v.ex - v.sx + 1, ny = v.ey - v.sy + 1, nz = v.ez - v.sz + 1;
+            const int total = (nx > 0 && ny > 0 && nz > 0) ? nx * ny * nz : 0;
+            int32_t* counts = p.vox + M->vbase;
+            const int xd = M->dims[0], yd = M->dims[1], cap = M->vcap;
+            bool over = false;
+            for (int t = tid; t < total; t += blockDim.x) {
+                const int j3 = v.sz + t % nz, j2 = v.sy + (t / nz) % ny, j1 = v.sx + t / (nz * ny);
+                if (atomicAdd(&counts[j3 * xd * yd + j2 * xd + j1], 1) >= cap) over = true;
+            }
+            if (__any(over) && (tid & 63) == __ffsll((unsigned long long)__ballot(over)) - 1)
+                atomicOr(&M->status, (uint32_t)CS3D_SYS_VOXEL_OVERFLOW);
+        }
         float x = p.X[gi], y = p.Y[gi], z = p.Z[gi];
         if (node && (flags & I_KICK)) {
             const float dt = P->dt, m = P->mass;
             const float rdt = sqrtf(dt);
-            const float fr = 0.0f;  // random force disabled (add_rands = 0)
             float vx = p.VX[gi], vy = p.VY[gi], vz = p.VZ[gi];
-            vx = (float)((double)vx + 0.5 * (double)(dt * p.FCX[gi] + dt * p.FDX[gi] + rdt * fr) / (double)m);
-            vy = (float)((double)vy + 0.5 * (double)(dt * p.FCY[gi] + dt * p.FDY[gi] + rdt * fr) / (double)m);
-            vz = (float)((double)vz + 0.5 * (double)(dt * p.FCZ[gi] + dt * p.FDZ[gi] + rdt * fr) / (double)m);
+            vx = half_kick(vx, p.FCX[gi], p.FDX[gi], dt, rdt, m);
+            vy = half_kick(vy, p.FCY[gi], p.FDY[gi], dt, rdt, m);
+            vz = half_kick(vz, p.FCZ[gi], p.FDZ[gi], dt, rdt, m);
             x = x + vx * dt;
             y = y + vy * dt;
             z = z + vz * dt;
@@ -121,7 +204,7 @@ __global__ __launch_bounds__(192) void k_integrate(Pool p, Range r, int32_t flag
                 atomicOr(&M->need_search, 1);
         }
         if (node && (flags & I_CLEAR)) p.bins[M->bbase + bin_of(M, x, y, z)] = 0;
-        if (node && (flags & I_BINS)) {
+        if (node && (flags & I_BINS) && (!(flags & I_IFSEARCH) || M->need_search)) {
             const int b = bin_of(M, x, y, z);
             int32_t* counts = p.bins + M->bbase;
             const int k = atomicAdd(&counts[b], 1);
@@ -466,6 +549,22 @@ __global__ __launch_bounds__(192) void k_con_dis(Pool p, Range r, int32_t mode, 
         if (fused) { svx[tid] = rVX[gi]; svy[tid] = rVY[gi]; svz[tid] = rVZ[gi]; }
         __syncthreads();
         uint32_t st = 0;
+        const bool angles = forces && P->constrain_angles;
+        if (angles) {  // each angle once (propagate.cu:26-60), before anything else is live
+            if (node) {
+                const float k = P->angle_k;
+                const float* th = mesh->theta0 + 3 * tid;
+                const int n0 = mesh->nn[tid], n1 = mesh->nn[kSlots + tid], n2 = mesh->nn[2 * kSlots + tid];
+#pragma unroll 1
+                for (int q = 0; q < 3; ++q) {  // angles (n0,n1), (n1,n2), (n0,n2) of this node
+                    const int a = q == 1 ? n1 : n0;
+                    const int b = q == 0 ? n1 : n2;
+                    st |= angle_pair(sx[a] - X, sy[a] - Y, sz[a] - Z, sx[b] - X, sy[b] - Y, sz[b] - Z, th[q], k,
+                                     u.angF + (tid * 3 + q) * 6);
+                }
+            }
+            __syncthreads();
+        }
         float FX = 0.f, FY = 0.f, FZ = 0.f;
         int nn3[3] = {0, 0, 0};
         if (forces) {
@@ -492,39 +591,25 @@ __global__ __launch_bounds__(192) void k_con_dis(Pool p, Range r, int32_t mode, 
                 FY += g * (inv * ry);
                 FZ += g * (inv * rz);
             }
-            // ---- angles, propagate.cu:81-181 (each angle once, gathered in reference order)
-            if (P->constrain_angles) {
-                const float k = P->angle_k;
-                if (node) {
-                    const float* th = mesh->theta0 + 3 * tid;
-                    const int A[3] = {nn3[0], nn3[1], nn3[0]};
-                    const int B[3] = {nn3[1], nn3[2], nn3[2]};
+            // ---- angle forces gathered in the summation order of CalculateAngleForce (propagate.cu:81-181)
+            if (angles && node) {
+                const float* a = u.angF + tid * 18;
+                float ax = 0.f, ay = 0.f, az = 0.f;
 #pragma unroll
-                    for (int q = 0; q < 3; ++q) {
-                        st |= angle_pair(sx[A[q]] - X, sy[A[q]] - Y, sz[A[q]] - Z, sx[B[q]] - X, sy[B[q]] - Y,
-                                         sz[B[q]] - Z, th[q], k, u.angF + (tid * 3 + q) * 6);
-                    }
+                for (int q = 0; q < 3; ++q) {
+                    ax = ax - (a[q * 6 + 0] + a[q * 6 + 3]);
+                    ay = ay - (a[q * 6 + 1] + a[q * 6 + 4]);
+                    az = az - (a[q * 6 + 2] + a[q * 6 + 5]);
                 }
-                __syncthreads();
-                if (node) {
-                    const float* a = u.angF + tid * 18;
-                    float ax = 0.f, ay = 0.f, az = 0.f;
+                const int32_t* gt = mesh->gather + tid * 6;
 #pragma unroll
-                    for (int q = 0; q < 3; ++q) {
-                        ax = ax - (a[q * 6 + 0] + a[q * 6 + 3]);
-                        ay = ay - (a[q * 6 + 1] + a[q * 6 + 4]);
-                        az = az - (a[q * 6 + 2] + a[q * 6 + 5]);
-                    }
-                    const int32_t* gt = mesh->gather + tid * 6;
-#pragma unroll
-                    for (int q = 0; q < 6; ++q) {
-                        const float* f = u.angF + gt[q];
-                        ax = ax + f[0];
-                        ay = ay + f[1];
-                        az = az + f[2];
-                    }
-                    FX += ax; FY += ay; FZ += az;
+                for (int q = 0; q < 6; ++q) {
+                    const float* f = u.angF + gt[q];
+                    ax = ax + f[0];
+                    ay = ay + f[1];
+                    az = az + f[2];
                 }
+                FX += ax; FY += ay; FZ += az;
             }
         }
         if (search && node) {
@@ -618,10 +703,12 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
     __shared__ float svx[kSlots], svy[kSlots], svz[kSlots];
     __shared__ float red[3][128];
     __shared__ float cm[3];
+    __shared__ float bpart[3][6];
     const int tid = threadIdx.x;
     const bool node = tid < kNodes;
     const MeshDev* mesh = p.mesh;
     const bool do_dis = mode == M_STEP || mode == M_DIS_V || mode == M_DIS_VA;
+    const bool fused = mode == M_STEP;
     for (int slot = r.slot_lo + blockIdx.x; slot < r.slot_hi; slot += gridDim.x) {
         const int s = p.slot_sys[slot];
         if (s < 0) continue;
@@ -637,6 +724,7 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
         const int np = (do_dis && node) ? p.NPAIR[gi] : 0;
         __syncthreads();
         uint32_t st = 0;
+        bool far = false;
         if (do_dis && node) {
             float fx = 0.f, fy = 0.f, fz = 0.f;
             const float vix = svx[tid], viy = svy[tid], viz = svz[tid];
@@ -668,7 +756,36 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
             fz = fz - P->gamma_visc * viz;
             p.FDX[gi] = fx; p.FDY[gi] = fy; p.FDZ[gi] = fz;
             if (clear_bins && (mode != M_STEP || M->need_search)) clear_own_bin(p, M, X, Y, Z);
+            if (fused) {
+                // the next step's Integrate moves this node to X + v' dt with v' from the
+                // values now in V, F_con and F_dis (bit-identical to k_integrate): search
+                // again next step when that is pair_skin/2 or more from the last search
+                const float dt = P->dt, m = P->mass, rdt = sqrtf(dt);
+                const float qx = X + half_kick(vix, p.FCX[gi], fx, dt, rdt, m) * dt - p.XBX[gi];
+                const float qy = Y + half_kick(viy, p.FCY[gi], fy, dt, rdt, m) * dt - p.XBY[gi];
+                const float qz = Z + half_kick(viz, p.FCZ[gi], fz, dt, rdt, m) * dt - p.XBZ[gi];
+                const float h = 0.5f * P->pair_skin;
+                far = !(qx * qx + qy * qy + qz * qz <= h * h);
+            }
         }
+        if (fused && __any(far) && (tid & 63) == __ffsll((unsigned long long)__ballot(far)) - 1)
+            atomicOr(&M->search_next, 1);
+        if (fused) {  // this cell's bbox at the end of the step (bounding_boxes, propagatebound.cu:10-141)
+            float b[6] = {node ? X : INFINITY, node ? X : -INFINITY, node ? Y : INFINITY,
+                          node ? Y : -INFINITY, node ? Z : INFINITY, node ? Z : -INFINITY};
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+                for (int a = 0; a < 6; a += 2) {
+                    b[a] = fminf(b[a], __shfl_xor(b[a], o));
+                    b[a + 1] = fmaxf(b[a + 1], __shfl_xor(b[a + 1], o));
+                }
+            }
+            if ((tid & 63) == 0)
+#pragma unroll
+                for (int a = 0; a < 6; ++a) bpart[tid >> 6][a] = b[a];
+        }  // combined after the next barrier
+
         const bool do_com = mode == M_STEP || mode == M_COM;
         const int64_t stepno = step_override >= 0 ? step_override : M->step + 1;
         const bool do_vol =
@@ -696,6 +813,12 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
                     cm[0] = a / 180.0f; cm[1] = b / 180.0f; cm[2] = d / 180.0f;
                     p.CMX[slot] = cm[0]; p.CMY[slot] = cm[1]; p.CMZ[slot] = cm[2];
                 }
+            }
+            if (fused && tid < 6) {
+                float b = bpart[0][tid];
+                b = (tid & 1) ? fmaxf(b, bpart[1][tid]) : fminf(b, bpart[1][tid]);
+                b = (tid & 1) ? fmaxf(b, bpart[2][tid]) : fminf(b, bpart[2][tid]);
+                p.BBOXN[(int64_t)slot * 6 + tid] = b;
             }
             __syncthreads();
         } else if (do_vol) {
@@ -832,12 +955,39 @@ __device__ void random_vector_basis(cs3d_ranmar_state* st, float* n, const float
 // the daughter stiffness rule (GPUbounce.cu:1254-1260).  In M_STEP it also ends
 // the step: step counter, and whether the grid is rebuilt this step.
 // ------------------------------------------------------------------------------
+// min (even a) / max (odd a) of 12 values over a 256-lane workgroup; the result
+// lands in out[0..12) (LDS), visible after the trailing barrier
+__device__ __forceinline__ void block_minmax12(float* v, float (*part)[12], float* out) {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+        for (int a = 0; a < 12; a += 2) {
+            v[a] = fminf(v[a], __shfl_xor(v[a], o));
+            v[a + 1] = fmaxf(v[a + 1], __shfl_xor(v[a + 1], o));
+        }
+    }
+    if ((tid & 63) == 0)
+#pragma unroll
+        for (int a = 0; a < 12; ++a) part[tid >> 6][a] = v[a];
+    __syncthreads();
+    if (tid < 12) {
+        float b = part[0][tid];
+        for (int w = 1; w < 4; ++w) b = (tid & 1) ? fmaxf(b, part[w][tid]) : fminf(b, part[w][tid]);
+        out[tid] = b;
+    }
+    __syncthreads();
+}
+
 __global__ __launch_bounds__(256) void k_sys_divide(Pool p, Range r, int32_t mode) {
     __shared__ int32_t scan[256];
     __shared__ int32_t nev, ncur, stop;
     __shared__ float nrm[3];
     __shared__ int32_t bad;
     __shared__ float redc[3][128];
+    __shared__ float mpart[4][12];
+    __shared__ float mres[12];
+    __shared__ int64_t nclear;
     const int tid = threadIdx.x;
     const int s = r.sys_lo + blockIdx.x;
     if (s >= r.sys_hi) return;
@@ -897,6 +1047,11 @@ __global__ __launch_bounds__(256) void k_sys_divide(Pool p, Range r, int32_t mod
             const int64_t pi = (int64_t)(base + rank) * kSlots + tid;
             const int64_t di = (int64_t)(base + d) * kSlots + tid;
             const float cmx = p.CMX[base + rank], cmy = p.CMY[base + rank], cmz = p.CMZ[base + rank];
+            // bboxes of parent and daughter after the split (min/max pairs: parent x, y,
+            // z, then daughter x, y, z)
+            float bx[12];
+#pragma unroll
+            for (int a = 0; a < 12; a += 2) { bx[a] = INFINITY; bx[a + 1] = -INFINITY; }
             if (tid < kNodes) {
                 const float x0 = p.X[pi], y0 = p.Y[pi], z0 = p.Z[pi];
 #pragma unroll
@@ -910,6 +1065,9 @@ __global__ __launch_bounds__(256) void k_sys_divide(Pool p, Range r, int32_t mod
                     x = x + (cmx + half * Nx); y = y + (cmy + half * Ny); z = z + (cmz + half * Nz);
                     const int64_t o = side ? di : pi;
                     p.X[o] = x; p.Y[o] = y; p.Z[o] = z;
+                    bx[6 * side + 0] = x; bx[6 * side + 1] = x;
+                    bx[6 * side + 2] = y; bx[6 * side + 3] = y;
+                    bx[6 * side + 4] = z; bx[6 * side + 5] = z;
                 }
                 p.VX[di] = p.VX[pi]; p.VY[di] = p.VY[pi]; p.VZ[di] = p.VZ[pi];
             } else if (tid < kSlots) {
@@ -919,6 +1077,10 @@ __global__ __launch_bounds__(256) void k_sys_divide(Pool p, Range r, int32_t mod
             if (tid < kSlots) {  // a new cell's forces start at zero (never-written slots in the reference)
                 p.FCX[di] = 0.f; p.FCY[di] = 0.f; p.FCZ[di] = 0.f;
                 p.FDX[di] = 0.f; p.FDY[di] = 0.f; p.FDZ[di] = 0.f;
+            }
+            if (step_mode) {
+                block_minmax12(bx, mpart, mres);
+                if (tid < 12) p.BBOXN[(int64_t)(base + (tid < 6 ? rank : d)) * 6 + tid % 6] = mres[tid];
             }
             if (tid == 0) {
                 const int gd = base + d, gp = base + rank;
@@ -966,16 +1128,41 @@ __global__ __launch_bounds__(256) void k_sys_divide(Pool p, Range r, int32_t mod
         }
     }
     __syncthreads();
+    const int events = ncur - n0;
+    const int gint = P->grid_interval > 0 ? P->grid_interval : 1;
+    const bool rebuild = step_mode && (events > 0 || stepno % gint == 0);
+    if (rebuild) {
+        // grid rebuild from the end-of-step bboxes: minmaxpre/minmaxpost
+        // (propagatebound.cu:145-204, 266-323, sentinels +-1e8) and the grid
+        // geometry; the per-cell part (bbox adoption, voxel counts) is done by the
+        // next step's Integrate
+        float e[12];
+#pragma unroll
+        for (int a = 0; a < 12; a += 2) { e[a] = 1.0E8f; e[a + 1] = -1.0E8f; }
+        for (int c = tid; c < ncur; c += blockDim.x) {
+            const float* b = p.BBOXN + (int64_t)(base + c) * 6;
+#pragma unroll
+            for (int a = 0; a < 6; a += 2) {
+                e[a] = fminf(e[a], b[a]);
+                e[a + 1] = fmaxf(e[a + 1], b[a + 1]);
+            }
+        }
+        block_minmax12(e, mpart, mres);
+        if (tid == 0) nclear = set_grid_geometry(M, P, mres);
+        __syncthreads();
+        int32_t* counts = p.vox + M->vbase;
+        for (int64_t v = tid; v < nclear; v += blockDim.x) counts[v] = 0;
+    }
     if (tid == 0) {
         if (bad) atomicOr(&M->status, (uint32_t)bad);
-        const int events = ncur - n0;
         M->n_events = events;
         M->n = ncur;
         if (step_mode) {
-            const int gi = P->grid_interval > 0 ? P->grid_interval : 1;
-            M->rebuild = (events > 0 || stepno % gi == 0) ? 1 : 0;
+            M->rebuild = rebuild ? 1 : 0;
             M->step = stepno;
-            M->need_search = events > 0 ? 1 : 0;  // divided cells moved: search again next step
+            // divided cells moved, or the next drift leaves the skin: search next step
+            M->need_search = (events > 0 || M->search_next) ? 1 : 0;
+            M->search_next = 0;
         } else if (events > 0) {
             M->need_search = 1;
         }
@@ -1072,41 +1259,17 @@ __global__ __launch_bounds__(256) void k_grid_dims(Pool p, Range r) {
     int32_t* counts = p.vox + M->vbase;
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nclear; v += (int64_t)gridDim.x * blockDim.x)
         counts[v] = 0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        M->gmin[0] = e[0]; M->gmin[1] = e[2]; M->gmin[2] = e[4];
-        M->dims[0] = ok ? xd : 1; M->dims[1] = ok ? yd : 1; M->dims[2] = ok ? zd : 1;
-        if (!ok) atomicOr(&M->status, (uint32_t)CS3D_SYS_GRID_TOO_LARGE);
-        // search bins: edge a little above attraction_range (pairs within range are in
-        // adjacent bins), two bins of margin; coarser when the extent outgrows the budget
-        float h = (P->attraction_range + P->pair_skin) * 1.001f;
-        if (!(h > 0.f)) h = 1.0f;
-        const float sz[3] = {e[1] - e[0], e[3] - e[2], e[5] - e[4]};
-        int bd[3] = {5, 5, 5};
-        for (int it = 0; it < 64; ++it) {
-            int64_t nb = 1;
-            for (int a = 0; a < 3; ++a) {
-                const float q = sz[a] / h;
-                bd[a] = (q >= 0.f && q < 1.0e7f) ? (int)q + 5 : 5;
-                nb *= bd[a];
-            }
-            if (nb <= M->bbudget) break;
-            h *= 1.25f;
-        }
-        for (int a = 0; a < 3; ++a) {
-            M->bmin[a] = e[2 * a] - 2.0f * h;
-            M->bdims[a] = bd[a];
-        }
-        M->bh = h;
-    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) (void)set_grid_geometry(M, P, e);
 }
 
-__global__ void k_insert(Pool p, Range r) {
+// force = 1: fill the lists of the current grid whatever the rebuild flag says
+__global__ void k_insert(Pool p, Range r, int32_t force) {
     for (int slot = r.slot_lo + blockIdx.x * blockDim.x + threadIdx.x; slot < r.slot_hi;
          slot += gridDim.x * blockDim.x) {
         const int s = p.slot_sys[slot];
         if (s < 0) continue;
         SysMeta* M = p.meta + s;
-        if (!M->rebuild) continue;
+        if (!force && !M->rebuild) continue;
         const int c = slot - M->base;
         if (c == 0) {  // extent accumulators back to the sentinels of minmaxpre (+-1e8)
 #pragma unroll
@@ -1114,23 +1277,15 @@ __global__ void k_insert(Pool p, Range r) {
         }
         if (c >= M->n || (M->status & CS3D_SYS_GRID_TOO_LARGE)) continue;
         const cs3d_params* P = p.prm + s;
-        const float* b = p.BBOX + (int64_t)slot * 6;
-        const float DL = P->grid_dl, at = P->attraction_range;
-        const float mx = M->gmin[0], my = M->gmin[1], mz = M->gmin[2];
-        const int xd = M->dims[0], yd = M->dims[1], zd = M->dims[2];
-        int sx = (int)((b[0] - at - mx) / DL); if (sx < 0) sx = 0;
-        int ex = (int)((b[1] + at - mx) / DL); if (ex >= xd) ex = xd - 1;
-        int sy = (int)((b[2] - at - my) / DL); if (sy < 0) sy = 0;
-        int ey = (int)((b[3] + at - my) / DL); if (ey >= yd) ey = yd - 1;
-        int sz = (int)((b[4] - at - mz) / DL); if (sz < 0) sz = 0;
-        int ez = (int)((b[5] + at - mz) / DL); if (ez >= zd) ez = zd - 1;
+        const VoxRange vr = vox_range(p.BBOX + (int64_t)slot * 6, M, P);
+        const int xd = M->dims[0], yd = M->dims[1];
         int32_t* counts = p.vox + M->vbase;
         int32_t* lists = p.vox + M->lbase;
         const int cap = M->vcap;
         bool over = false;
-        for (int j1 = sx; j1 <= ex; ++j1)
-            for (int j2 = sy; j2 <= ey; ++j2)
-                for (int j3 = sz; j3 <= ez; ++j3) {
+        for (int j1 = vr.sx; j1 <= vr.ex; ++j1)
+            for (int j2 = vr.sy; j2 <= vr.ey; ++j2)
+                for (int j3 = vr.sz; j3 <= vr.ez; ++j3) {
                     const int v = j3 * xd * yd + j2 * xd + j1;
                     const int k = atomicAdd(&counts[v], 1);
                     if (k < cap) lists[(int64_t)v * cap + k] = c;
@@ -1140,11 +1295,11 @@ __global__ void k_insert(Pool p, Range r) {
     }
 }
 
-__global__ __launch_bounds__(256) void k_sort(Pool p, Range r) {
+__global__ __launch_bounds__(256) void k_sort(Pool p, Range r, int32_t force) {
     const int s = r.sys_lo + blockIdx.y;
     if (s >= r.sys_hi) return;
     SysMeta* M = p.meta + s;
-    if (!M->rebuild) return;
+    if (!force && !M->rebuild) return;
     if (M->status & CS3D_SYS_GRID_TOO_LARGE) return;
     const int64_t nv = (int64_t)M->dims[0] * M->dims[1] * M->dims[2];
     const int cap = M->vcap;
@@ -1160,6 +1315,30 @@ __global__ __launch_bounds__(256) void k_sort(Pool p, Range r) {
             L[j + 1] = key;
         }
     }
+}
+
+// Completes a grid whose per-cell part is pending (the fused step leaves it to
+// the next step's Integrate): the new bboxes become current.  Then the voxel
+// counters of the current grid are cleared so that k_insert can fill the lists.
+__global__ __launch_bounds__(192) void k_grid_adopt(Pool p, Range r) {
+    for (int slot = r.slot_lo + blockIdx.x; slot < r.slot_hi; slot += gridDim.x) {
+        const int s = p.slot_sys[slot];
+        if (s < 0) continue;
+        const SysMeta* M = p.meta + s;
+        if (!M->rebuild || slot - M->base >= M->n) continue;
+        if (threadIdx.x < 6) p.BBOX[(int64_t)slot * 6 + threadIdx.x] = p.BBOXN[(int64_t)slot * 6 + threadIdx.x];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_vox_clear(Pool p, Range r) {
+    const int s = r.sys_lo + blockIdx.y;
+    if (s >= r.sys_hi) return;
+    const SysMeta* M = p.meta + s;
+    if (M->status & CS3D_SYS_GRID_TOO_LARGE) return;
+    const int64_t nv = (int64_t)M->dims[0] * M->dims[1] * M->dims[2];
+    int32_t* counts = p.vox + M->vbase;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (int64_t)gridDim.x * blockDim.x)
+        counts[v] = 0;
 }
 
 // VelocityUpdateA then VelocityUpdateB (propagate.cu:714-736) on V (per-kernel entry point)
@@ -1241,8 +1420,18 @@ void launch_grid(const Pool& p, const Range& r, int vox_blocks, hipStream_t st) 
     const int ns = r.sys_hi - r.sys_lo;
     hipLaunchKernelGGL(k_cellbox, dim3(bb > 0 ? bb : 1), dim3(192), 0, st, p, r);
     hipLaunchKernelGGL(k_grid_dims, dim3(vox_blocks, ns > 0 ? ns : 1), dim3(256), 0, st, p, r);
-    hipLaunchKernelGGL(k_insert, dim3(cb > 0 ? cb : 1), dim3(256), 0, st, p, r);
-    hipLaunchKernelGGL(k_sort, dim3(vox_blocks, ns > 0 ? ns : 1), dim3(256), 0, st, p, r);
+    hipLaunchKernelGGL(k_insert, dim3(cb > 0 ? cb : 1), dim3(256), 0, st, p, r, 0);
+    hipLaunchKernelGGL(k_sort, dim3(vox_blocks, ns > 0 ? ns : 1), dim3(256), 0, st, p, r, 0);
+}
+// voxel lists of the current grid (the fused step keeps only the counts)
+void launch_vox_lists(const Pool& p, const Range& r, int vox_blocks, hipStream_t st) {
+    const int n = r.slot_hi - r.slot_lo;
+    const int cb = (n + 255) / 256;
+    const int ns = r.sys_hi - r.sys_lo;
+    hipLaunchKernelGGL(k_grid_adopt, dim3(grid_cells(r, 16)), dim3(192), 0, st, p, r);
+    hipLaunchKernelGGL(k_vox_clear, dim3(vox_blocks, ns > 0 ? ns : 1), dim3(256), 0, st, p, r);
+    hipLaunchKernelGGL(k_insert, dim3(cb > 0 ? cb : 1), dim3(256), 0, st, p, r, 1);
+    hipLaunchKernelGGL(k_sort, dim3(vox_blocks, ns > 0 ? ns : 1), dim3(256), 0, st, p, r, 1);
 }
 void launch_velocity_ab(const Pool& p, const Range& r, hipStream_t st) {
     hipLaunchKernelGGL(k_velocity_ab, dim3(grid_cells(r, 8)), dim3(192), 0, st, p, r);

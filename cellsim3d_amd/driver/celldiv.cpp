@@ -1,0 +1,669 @@
+// celldiv - host driver of the C180 cell-mechanics engine: a drop-in for the
+// reference's CellDiv executable (src/GPUbounce.cu main, :289-1512).
+//
+//   celldiv <no_of_cells> <inp.json> <gpuid> [--grid-interval K] [--pair-skin S]
+//           [--mesh-dir DIR] [--no-graph] [--daughter-com-fix]
+//
+// Same command line, parameter file, input mesh files (C180, C180NN, C180C,
+// C180_pentahexa in the working directory or --mesh-dir) and outputs: the
+// trajectory (binary or ASCII, core.trajFileName), the forces CSV
+// (core.forces_file when core.write_cont_force), and the empty psfil and
+// velocity2.xyz the reference creates.  All simulation work runs on the GPU
+// through the C ABI of libcs3d (include/cs3d.h); this file only parses,
+// places, and writes.
+//
+// Exit codes follow the reference: -1 for I/O, parse or allocation errors
+// (GPUbounce.cu:301, 1616-1886), 27 when the rigid box is too small (:1666-1670),
+// -69 when the population outgrows MaxNoofC180s (:1265-1270), -60 for bad
+// time-step parameters (:2079-2083).  Device-side failures (the reference's
+// traps) print the system status and exit with 0, as CudaErrorCheck does
+// (:39-45).
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/cs3d.h"
+#include "json_lite.h"
+
+namespace {
+
+constexpr int S = CS3D_SLOTS;
+constexpr int NN = CS3D_NODES;
+
+// Everything read_json_params (GPUbounce.cu:1914-2247) reads that this driver uses.
+struct Settings {
+    cs3d_params p;
+    int max_cells = 0;
+    int time_steps = 0;  // core.div_time_steps (read with asFloat into an int, :1959)
+    int equi_steps = 0;  // core.non_div_time_steps
+    int traj_write_int = 0;
+    std::string traj_file;
+    bool binary = true;
+    bool write_cont_force = false;
+    std::string forces_file;
+    int ran_z_offset = 0;
+    float z_offset = 0.f;
+    int restart = 0;
+    bool do_adaptive_dt = false;
+    float dt_max = 0.f, dt_tol = 0.f;
+    bool correct_com = false;
+    int new_cell_count_int = 0;
+    bool count_only_internal = false;
+    float rad_frac = 0.f;
+    int use_walls = 0;
+    float closeness_to_center = 0.f;
+    float fraction_of_softer_cells = 0.f;
+    int number_of_softer_cells = 0;
+    bool flatbox = false;
+    bool rand_pos = false;
+    bool use_pbcs = false;
+    int rand_seed = -1;
+};
+
+// read_json_params: parameter mapping (:1933-2076) and validation (:2077-2244).
+int read_params(const char* path, Settings& s) {
+    std::ifstream in(path);
+    if (!in) {
+        std::printf("Failed to parse %s\n", path);
+        return -1;
+    }
+    std::stringstream ss;
+    ss << in.rdbuf();
+    jsonlite::Value root;
+    try {
+        root = jsonlite::parse(ss.str());
+    } catch (const std::exception& e) {
+        std::printf("Failed to parse %s\n%s\n", path, e.what());
+        return -1;
+    }
+    std::printf("%s parsed successfully\n", path);
+    cs3d_params_default(&s.p);
+    cs3d_params& p = s.p;
+    const jsonlite::Value& core = root["core"];
+    if (core.is_null()) {
+        std::printf("ERROR: Cannot load core simulation parameters\nExiting");
+        return -1;
+    }
+    s.max_cells = core["MaxNoofC180s"].as_int();
+    p.mass = core["particle_mass"].as_float();
+    p.repulsion_range = core["repulsion_range"].as_float();
+    p.attraction_range = core["attraction_range"].as_float();
+    p.repulsion_strength = core["repulsion_strength"].as_float();
+    p.attraction_strength = core["attraction_strength"].as_float();
+    p.youngs_mod = core["Youngs_mod"].as_float();
+    p.stiffness1 = core["stiffFactor1"].as_float() * p.youngs_mod;
+    p.viscotic_damping = core["viscotic_damping"].as_float();
+    p.internal_damping = core["internal_damping"].as_float();
+    p.division_vol = core["division_Vol"].as_float();
+    s.ran_z_offset = core["random_z_offset?"].as_int();
+    s.z_offset = core["z_offset"].as_float();
+    s.time_steps = (int)core["div_time_steps"].as_float();
+    p.dt = core["time_interval"].as_float();
+    s.restart = core["Restart"].as_int();
+    s.traj_write_int = core["trajWriteInt"].as_int();
+    s.equi_steps = core["non_div_time_steps"].as_int();
+    s.traj_file = core["trajFileName"].as_string();
+    s.binary = core["binaryOutput"].as_bool();
+    p.max_pressure = core["maxPressure"].as_float();
+    p.min_pressure = core["minPressure"].as_float();
+    p.gamma_visc = core["gamma_visc"].as_float();
+    p.growth_rate = core["growth_rate"].as_float();
+    p.check_sphericity = core["checkSphericity"].as_bool();
+    p.constrain_angles = core["constrainAngles"].as_bool();
+    s.dt_max = core["dt_max"].as_float();
+    s.dt_tol = core["dt_tol"].as_float();
+    s.do_adaptive_dt = core["doAdaptive_dt"].as_bool();
+    p.phase_count = core["phase_count"].as_int();
+    s.write_cont_force = core["write_cont_force"].as_bool();
+    s.forces_file = core["forces_file"].as_string();
+    s.correct_com = core["correct_com"].as_bool();
+
+    const jsonlite::Value& cnt = root["counting"];
+    if (cnt.is_null()) {
+        std::printf("ERROR: Cannot load counting parameters\nExiting");
+        return -1;
+    }
+    s.count_only_internal = cnt["count_only_internal_cells?"].as_bool();
+    s.rad_frac = cnt["radius_cutoff"].as_float();
+    s.new_cell_count_int = cnt["cell_count_int"].as_int();
+
+    const jsonlite::Value& pop = root["population"];
+    if (pop.is_null()) {
+        std::printf("ERROR: Cannot load population parameters\nExiting");
+        return -1;
+    }
+    p.do_pop_model = pop["doPopModel"].as_int();
+    p.max_pop = pop["max_pop"].as_float();
+
+    const jsonlite::Value& walls = root["walls"];
+    if (walls.is_null()) {
+        std::printf("ERROR: Cannot load wall parameters\nExiting");
+        return -1;
+    }
+    s.use_walls = walls["useWalls"].as_int();
+    p.thresh_dist = walls["threshDist"].as_float();
+
+    const jsonlite::Value& div = root["divParams"];
+    if (div.is_null()) {
+        std::printf("ERROR: Cannot load division parameters\n");
+        return -1;
+    }
+    p.use_div_plane_basis = div["useDivPlaneBasis"].as_int();
+    p.div_plane_basis[0] = div["divPlaneBasisX"].as_float();
+    p.div_plane_basis[1] = div["divPlaneBasisY"].as_float();
+    p.div_plane_basis[2] = div["divPlaneBasisZ"].as_float();
+
+    const jsonlite::Value& stf = root["stiffnessParams"];
+    if (stf.is_null()) {
+        std::printf("ERROR: Cannot load stiffness parameters\n");
+        return -1;
+    }
+    p.use_different_stiffnesses = stf["useDifferentStiffnesses"].as_bool();
+    p.stiffness2 = stf["softStiffFactor"].as_float() * p.youngs_mod;
+    s.number_of_softer_cells = stf["numberOfSofterCells"].as_int();
+    s.closeness_to_center = stf["closenessToCenter"].as_float();
+    s.fraction_of_softer_cells = stf["fractionOfSofterCells"].as_float();
+    p.daught_same_stiffness = stf["daughtSameStiffness"].as_bool();
+    p.recalc_r0 = stf["recalc_r0"].as_bool();
+
+    const jsonlite::Value& box = root["boxParams"];
+    if (box.is_null()) {
+        std::printf("ERROR: Cannot load box parameters\n");
+        return -1;
+    }
+    p.use_rigid_box = box["useRigidSimulationBox"].as_bool();
+    s.use_pbcs = box["usePBCs"].as_bool();
+    p.box_max[0] = box["box_len_x"].as_float();
+    p.box_max[1] = box["box_len_y"].as_float();
+    p.box_max[2] = box["box_len_z"].as_float();
+    s.flatbox = box["flatbox"].as_bool();
+    s.rand_pos = box["rand_pos"].as_bool();
+
+    if (s.ran_z_offset == 0) s.z_offset = 0.0f;
+    if (s.dt_tol > s.dt_max || s.dt_max <= 0 || s.dt_tol < 0) {
+        std::printf("ERROR: Invalid time step parameters\nFATAL ERROR\n");
+        std::exit(-60);
+    }
+    const jsonlite::Value& rnd = root["rand_params"];
+    if (rnd.is_null()) {
+        std::printf("ERROR: Cannot load randomness parameters\n");
+        return -1;
+    }
+    p.add_rands = rnd["add_rands"].as_bool();
+    s.rand_seed = rnd["rand_seed"].as_int();
+
+    if (s.rad_frac < 0.4f || s.rad_frac > 0.8f) {
+        std::printf("radFrac not in [0.4, 0.8] setting to 1.\n");
+        s.count_only_internal = false;
+    }
+    if (s.traj_write_int == 0) s.traj_write_int = 1;
+    if (s.new_cell_count_int == 0) s.new_cell_count_int = 1;
+    if (s.traj_write_int > s.time_steps + s.equi_steps) {
+        std::printf("Trajectory write interval is too large\n");
+        return -1;
+    }
+    if (s.time_steps % s.traj_write_int != 0) {
+        std::printf("Invalid trajectory write interval. Time steps must be divisible by it. \n");
+        return -1;
+    }
+    if (s.new_cell_count_int > s.time_steps) {
+        std::printf("New cell counting interval is too large. \n");
+        return -1;
+    }
+    if (s.equi_steps <= 0) s.equi_steps = 0;
+    if (p.do_pop_model != 1) p.do_pop_model = 0;
+    if (p.max_pressure < 0) {
+        std::printf("Invalid maximum pressure value of %f\nDisabling population modelling...", p.max_pressure);
+        p.do_pop_model = 0;
+    }
+    if (!(s.closeness_to_center >= 0 && s.closeness_to_center <= 1)) {
+        std::printf("ERROR: closenessToCenter is not in [0, 1]\nERROR: invalid input parameter\n");
+        return -1;
+    }
+    if (s.use_walls && p.use_rigid_box) {
+        std::printf("ERROR: Cannot use infinite XY walls and rigid simulation box simultaneously.\n"
+                    "ERROR: Only use on or the other.\n");
+        return -1;
+    }
+    if (s.fraction_of_softer_cells > 1.0f) {
+        std::printf("ERROR: Softer cell fraction is > 1\n");
+        return -1;
+    }
+    p.div_time_steps = s.time_steps;
+    p.non_div_time_steps = s.equi_steps;
+    return 0;
+}
+
+// Features of the reference's main that this driver does not provide; each is
+// refused with a message rather than silently ignored.
+int refuse_unsupported(const Settings& s) {
+    const char* why = nullptr;
+    if (s.p.add_rands) why = "rand_params.add_rands = 1 (the reference's cuRAND stream is not reproduced)";
+    else if (s.do_adaptive_dt) why = "core.doAdaptive_dt = 1";
+    else if (s.correct_com) why = "core.correct_com = 1";
+    else if (s.p.use_different_stiffnesses) why = "stiffnessParams.useDifferentStiffnesses = 1 (softer-cell selection)";
+    else if (s.use_pbcs) why = "boxParams.usePBCs = 1 (unavailable in the reference as well)";
+    if (!why) return 0;
+    std::printf("ERROR: not supported by this driver: %s\n", why);
+    return -1;
+}
+
+bool read_mesh(const std::string& dir, std::vector<float>& x, std::vector<float>& y, std::vector<float>& z,
+               std::vector<int32_t>& nn, std::vector<int32_t>& sign, std::vector<int32_t>& faces) {
+    auto path = [&](const char* f) { return dir.empty() || dir == "." ? std::string(f) : dir + "/" + f; };
+    FILE* f = std::fopen(path("C180").c_str(), "r");  // GPUbounce.cu:1616-1626
+    if (!f) {
+        std::printf("Unable to open file C180\n");
+        return false;
+    }
+    x.assign(NN, 0.f); y.assign(NN, 0.f); z.assign(NN, 0.f);
+    for (int a = 0; a < NN; ++a)
+        if (std::fscanf(f, "%f %f %f", &x[a], &y[a], &z[a]) != 3) {
+            std::printf("   Unable to read file C180 on line %d\n", a + 1);
+            std::fclose(f);
+            return false;
+        }
+    std::fclose(f);
+    f = std::fopen(path("C180NN").c_str(), "r");  // read_fullerene_nn, :1822-1838 (1-based)
+    if (!f) {
+        std::printf("Unable to open file C180NN\n");
+        return false;
+    }
+    nn.assign(3 * S, 0);
+    sign.assign(S, 0);
+    for (int i = 0; i < NN; ++i) {
+        int a, b, c, sg;
+        if (std::fscanf(f, "%d, %d, %d, %d", &a, &b, &c, &sg) != 4) {
+            std::printf("Unable to read line %d in file C180NN\n", i);
+            std::fclose(f);
+            return false;
+        }
+        nn[i] = a - 1; nn[S + i] = b - 1; nn[2 * S + i] = c - 1;
+        sign[i] = sg;
+    }
+    std::fclose(f);
+    f = std::fopen(path("C180C").c_str(), "r");  // :1840-1856, 270 bonds (read for validation)
+    if (!f) {
+        std::printf("Unable to open file C180C\n");
+        return false;
+    }
+    for (int i = 0; i < 270; ++i) {
+        int a, b;
+        if (std::fscanf(f, "%d, %d", &a, &b) != 2) {
+            std::printf("Unable to read line %d in file C180C\n", i);
+            std::fclose(f);
+            return false;
+        }
+    }
+    std::fclose(f);
+    f = std::fopen(path("C180_pentahexa").c_str(), "r");  // :1858-1886
+    if (!f) {
+        std::printf("Unable to open file C180_pentahexa\n");
+        return false;
+    }
+    faces.assign(CS3D_FACES * 7, 0);
+    for (int q = 0; q < 12; ++q) {
+        int r[5];
+        if (std::fscanf(f, "%d %d %d %d %d", &r[0], &r[1], &r[2], &r[3], &r[4]) != 5) {
+            std::printf("Error: Unable to read line %d in file C180_pentahexa\n", q);
+            std::fclose(f);
+            return false;
+        }
+        const int row[7] = {r[0], r[1], r[2], r[3], r[4], r[0], r[0]};
+        for (int k = 0; k < 7; ++k) faces[7 * q + k] = row[k];
+    }
+    for (int q = 0; q < 80; ++q) {
+        int r[6];
+        if (std::fscanf(f, "%d %d %d %d %d %d", &r[0], &r[1], &r[2], &r[3], &r[4], &r[5]) != 6) {
+            std::printf("Error: Unable to read line %d in file C180_pentahexa\n", q);
+            std::fclose(f);
+            return false;
+        }
+        const int row[7] = {r[0], r[1], r[2], r[3], r[4], r[5], r[0]};
+        for (int k = 0; k < 7; ++k) faces[84 + 7 * q + k] = row[k];
+    }
+    std::fclose(f);
+    return true;
+}
+
+// initialize_C180s (GPUbounce.cu:1598-1774): the mesh centred on its node mean
+// (float sums, divided by the double 180.0) and copied to the start positions.
+int place_cells(const Settings& s, int n, int side, std::vector<float> ix, std::vector<float> iy,
+                std::vector<float> iz, cs3d_ranmar_state* rng, std::vector<float>& X, std::vector<float>& Y,
+                std::vector<float>& Z) {
+    float sx = 0, sy = 0, sz = 0;
+    for (int i = 0; i < NN; ++i) { sx += ix[i]; sy += iy[i]; sz += iz[i]; }
+    sx = (float)((double)sx / 180.0);
+    sy = (float)((double)sy / 180.0);
+    sz = (float)((double)sz / 180.0);
+    for (int i = 0; i < NN; ++i) { ix[i] -= sx; iy[i] -= sy; iz[i] -= sz; }
+    X.assign((size_t)s.max_cells * S, 0.f);
+    Y.assign((size_t)s.max_cells * S, 0.f);
+    Z.assign((size_t)s.max_cells * S, 0.f);
+    const float L1 = 3.0f;  // lattice spacing (GPUbounce.cu:107)
+    const float* bm = s.p.box_max;
+    if (s.p.use_rigid_box) {
+        float rcheck = powf(0.75 * (1.f / 3.14159) * 0.786, 1.f / 3.f);
+        const float vol = bm[0] * bm[1] * bm[2];
+        const int k = (int)std::floor(vol / 0.786);
+        if (k < n) {
+            std::fprintf(stderr, "ERROR: Simulation box is too small\n       Big enough for %d\n", k);
+            return 27;
+        }
+        std::printf("Can fit upto %d cells\n", k);
+        if (s.rand_pos) {
+            std::vector<float> cx, cy, cz;
+            while ((int)cx.size() < n) {
+                float r[3];
+                cs3d_ranmar(rng, r, 3);
+                const float CMx = r[0] * (bm[0] - 1.f) + 1.f;
+                const float CMy = r[1] * (bm[1] - 1.f) + 1.f;
+                const float CMz = s.flatbox ? bm[2] / 2 : r[2] * (bm[2] - 1.f) + 1.f;
+                bool far = !(CMx + rcheck > bm[0] || CMy + rcheck > bm[1] || CMz + rcheck > bm[2]);
+                for (size_t q = 0; q < cx.size() && far; ++q) {
+                    const float dx = cx[q] - CMx, dy = cy[q] - CMy, dz = cz[q] - CMz;
+                    if (sqrtf(dx * dx + dy * dy + dz * dz) < 2 * rcheck) far = false;
+                }
+                if (far) { cx.push_back(CMx); cy.push_back(CMy); cz.push_back(CMz); }
+            }
+            for (int c = 0; c < n; ++c)
+                for (int a = 0; a < NN; ++a) {
+                    X[(size_t)c * S + a] = ix[a] + cx[c];
+                    Y[(size_t)c * S + a] = iy[a] + cy[c];
+                    Z[(size_t)c * S + a] = iz[a] + cz[c];
+                }
+        } else {
+            const float ctr[3] = {0.5f * bm[0], 0.5f * bm[1], 0.5f * bm[2]};
+            for (int c = 0; c < n; ++c) {
+                const int ey = c % side, ex = c / side;
+                for (int a = 0; a < NN; ++a) {
+                    X[(size_t)c * S + a] = (float)(ix[a] + L1 * ex + 0.5 * L1 + ctr[0]);
+                    Y[(size_t)c * S + a] = (float)(iy[a] + L1 * ey + 0.5 * L1 + ctr[1]);
+                    Z[(size_t)c * S + a] = iz[a] + ctr[2];
+                }
+            }
+        }
+        for (int i = 0; i < n * S; ++i)
+            if (X[i] > bm[0] || Y[i] > bm[1] || Z[i] > bm[2]) {
+                std::printf("node outside the box\n%f %f %f\n", X[i], Y[i], Z[i]);
+                break;
+            }
+    } else {
+        for (int c = 0; c < n; ++c) {
+            const int ey = c % side, ex = c / side;
+            for (int a = 0; a < NN; ++a) {
+                X[(size_t)c * S + a] = (float)(ix[a] + L1 * ex + 0.5 * L1);
+                Y[(size_t)c * S + a] = (float)(iy[a] + L1 * ey + 0.5 * L1);
+                Z[(size_t)c * S + a] = iz[a] + s.z_offset;
+            }
+        }
+    }
+    return 0;
+}
+
+struct Output {
+    FILE* ascii = nullptr;
+    cs3d_traj* bin = nullptr;
+    FILE* forces = nullptr;
+};
+
+bool get_field(cs3d_ctx* ctx, cs3d_sys sys, cs3d_field f, void* host, size_t bytes) {
+    const cs3d_status st = cs3d_get(ctx, sys, f, host, bytes);
+    if (st != CS3D_OK) {
+        std::printf("ERROR: cs3d_get(%d): %s (%s)\n", (int)f, cs3d_status_str(st), cs3d_last_error());
+        return false;
+    }
+    return true;
+}
+
+// write_traj (GPUbounce.cu:2412-2444) / WriteBinaryTraj (:2446-2477) of one frame
+bool write_positions(const Settings& s, Output& out, int step, int frame, int n, const std::vector<float>& X,
+                     const std::vector<float>& Y, const std::vector<float>& Z) {
+    if (out.bin) {
+        if (cs3d_traj_append(out.bin, step, frame, n, X.data(), Y.data(), Z.data(), nullptr) != CS3D_OK) return false;
+    } else if (out.ascii) {
+        std::fprintf(out.ascii, "%d\n", n * 192);
+        std::fprintf(out.ascii, "Step: %d frame: %d\n", step, step / s.traj_write_int);
+        for (int c = 0; c < n; ++c) {
+            std::fprintf(out.ascii, "cell: %d\n", c);
+            for (int q = 0; q < S; ++q)
+                std::fprintf(out.ascii, "%.7f,  %.7f,  %.7f\n", X[(size_t)c * S + q], Y[(size_t)c * S + q],
+                             Z[(size_t)c * S + q]);
+        }
+    }
+    return true;
+}
+
+// writeForces (GPUbounce.cu:2567-2598): total conservative force, velocity,
+// position, pressure and volume of every node
+bool write_forces(cs3d_ctx* ctx, cs3d_sys sys, Output& out, int step, int n, const std::vector<float>& X,
+                  const std::vector<float>& Y, const std::vector<float>& Z) {
+    if (out.forces) {
+        std::vector<float> FX((size_t)n * S), FY((size_t)n * S), FZ((size_t)n * S);
+        std::vector<float> VX((size_t)n * S), VY((size_t)n * S), VZ((size_t)n * S), P(n), V(n);
+        if (n > 0 && (!get_field(ctx, sys, CS3D_F_FCX, FX.data(), FX.size() * 4) ||
+                      !get_field(ctx, sys, CS3D_F_FCY, FY.data(), FY.size() * 4) ||
+                      !get_field(ctx, sys, CS3D_F_FCZ, FZ.data(), FZ.size() * 4) ||
+                      !get_field(ctx, sys, CS3D_F_VX, VX.data(), VX.size() * 4) ||
+                      !get_field(ctx, sys, CS3D_F_VY, VY.data(), VY.size() * 4) ||
+                      !get_field(ctx, sys, CS3D_F_VZ, VZ.data(), VZ.size() * 4) ||
+                      !get_field(ctx, sys, CS3D_F_PRESSURE, P.data(), P.size() * 4) ||
+                      !get_field(ctx, sys, CS3D_F_VOLUME, V.data(), V.size() * 4)))
+            return false;
+        for (int c = 0; c < n; ++c)
+            for (int q = 0; q < NN; ++q) {
+                const size_t g = (size_t)c * S + q;
+                const float fm = sqrtf(FX[g] * FX[g] + FY[g] * FY[g] + FZ[g] * FZ[g]);
+                const float vm = sqrtf(VX[g] * VX[g] + VY[g] * VY[g] + VZ[g] * VZ[g]);
+                std::fprintf(out.forces,
+                             "%d,%d,%d,%d,%d,%.8f,%.8f,%.8f,%.8f,%.8f,%.8f,%.8f,%.8f,%.8f,%.8f,%.8f,%.8f,%.8f\n", step,
+                             n, c, q, c * 192 + q, FX[g], FY[g], FZ[g], fm, VX[g], VY[g], VZ[g], vm, X[g], Y[g], Z[g],
+                             P[c], V[c]);
+            }
+    }
+    return true;
+}
+
+bool write_frame(cs3d_ctx* ctx, cs3d_sys sys, const Settings& s, Output& out, int step, int frame, int n) {
+    std::vector<float> X((size_t)n * S), Y((size_t)n * S), Z((size_t)n * S);
+    if (n > 0 && (!get_field(ctx, sys, CS3D_F_X, X.data(), X.size() * 4) ||
+                  !get_field(ctx, sys, CS3D_F_Y, Y.data(), Y.size() * 4) ||
+                  !get_field(ctx, sys, CS3D_F_Z, Z.data(), Z.size() * 4)))
+        return false;
+    return write_positions(s, out, step, frame, n, X, Y, Z) && write_forces(ctx, sys, out, step, n, X, Y, Z);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    std::printf("celldiv (C180 cell mechanics on AMD Instinct, libcs3d %s)\n", cs3d_version());
+    if (argc < 4) {
+        std::printf("Usage: celldiv no_of_cells inpFile.json gpuid [--grid-interval K] [--pair-skin S] "
+                    "[--mesh-dir DIR] [--no-graph] [--daughter-com-fix] [--setup-only]\n");
+        return 0;
+    }
+    const int n_cells = std::atoi(argv[1]);
+    const int gpuid = std::atoi(argv[3]);
+    int grid_interval = 1;
+    float pair_skin = -1.f;
+    std::string mesh_dir = ".";
+    bool graph = true, com_fix = false, setup_only = false;
+    for (int a = 4; a < argc; ++a) {
+        const std::string o = argv[a];
+        if (o == "--grid-interval" && a + 1 < argc) grid_interval = std::atoi(argv[++a]);
+        else if (o == "--pair-skin" && a + 1 < argc) pair_skin = (float)std::atof(argv[++a]);
+        else if (o == "--mesh-dir" && a + 1 < argc) mesh_dir = argv[++a];
+        else if (o == "--no-graph") graph = false;
+        else if (o == "--daughter-com-fix") com_fix = true;
+        else if (o == "--setup-only") setup_only = true;
+        else {
+            std::printf("unknown option %s\n", o.c_str());
+            return -1;
+        }
+    }
+    Settings s;
+    if (read_params(argv[2], s) != 0) return -1;
+    std::printf("%d\n", s.max_cells);
+    const int side = (int)(std::sqrt((double)n_cells) + 0.5);
+    if (n_cells > s.max_cells || n_cells < 1) {
+        std::printf("Usage: Celldiv no_of_threads\n       no_of_threads should be a square, n^2, < %d\n",
+                    s.max_cells);
+        return 0;
+    }
+    if (refuse_unsupported(s) != 0) return -1;
+    s.p.grid_interval = grid_interval;
+    if (pair_skin >= 0) s.p.pair_skin = pair_skin;
+    s.p.daughter_com_fix = com_fix ? 1 : 0;
+
+    // generate_random (GPUbounce.cu:1777-1810): RANMAR seeds
+    int ij, kl;
+    if (s.rand_seed < 0) {
+        const time_t now = std::time(nullptr);
+        ij = (int)(now % 31328);
+        kl = (int)(now % 30081);
+    } else {
+        ij = s.rand_seed;
+        kl = s.rand_seed * 100;
+    }
+    s.p.ranmar_ij = ij;
+    s.p.ranmar_kl = kl;
+    cs3d_ranmar_state rng;
+    if (cs3d_ranmar_init(&rng, ij, kl) != CS3D_OK) {
+        std::printf("RANMAR seeds out of range: %d, %d\n", ij, kl);
+        std::exit(1);  // rmarin's exit (marsaglia.h:79-83)
+    }
+    std::printf("RNG seeds: %d, %d\n", ij, kl);
+    if (cs3d_params_check(&s.p) != CS3D_OK) {
+        std::printf("ERROR: %s\n", cs3d_last_error());
+        return -1;
+    }
+
+    std::vector<float> ix, iy, iz;
+    std::vector<int32_t> nn, sign, faces;
+    if (!read_mesh(mesh_dir, ix, iy, iz, nn, sign, faces)) return -1;
+    std::vector<float> X, Y, Z;
+    const int rc = place_cells(s, n_cells, side, ix, iy, iz, &rng, X, Y, Z);
+    if (rc != 0) return rc;
+    cs3d_mesh mesh;
+    if (cs3d_mesh_build(nn.data(), sign.data(), faces.data(), X.data(), Y.data(), Z.data(), &mesh) != CS3D_OK) {
+        std::printf("ERROR: mesh: %s\n", cs3d_last_error());
+        return -1;
+    }
+
+    // outputs, GPUbounce.cu:332-335, 768-778, 1007-1048
+    if (FILE* f = std::fopen("psfil", "w")) std::fclose(f);
+    Output out;
+    if (s.binary) {
+        const int max_frames = (s.time_steps + s.equi_steps + 1) / s.traj_write_int;
+        if (cs3d_traj_open(s.traj_file.c_str(), s.max_cells, s.p.use_different_stiffnesses, max_frames, &out.bin) !=
+            CS3D_OK) {
+            std::printf("Failed to open %s \n", s.traj_file.c_str());
+            return -1;
+        }
+    } else {
+        out.ascii = std::fopen(s.traj_file.c_str(), "w");
+        if (!out.ascii) {
+            std::printf("Failed to open %s \n", s.traj_file.c_str());
+            return -1;
+        }
+        std::fprintf(out.ascii, "Header Start:\nMaximum number of cells:\n%d\n", s.max_cells);
+        std::fprintf(out.ascii, "Using variable stiffness:\n%s\n", s.p.use_different_stiffnesses ? "True" : "False");
+        std::fprintf(out.ascii, "Maximum number of frames:\n%d\n", (s.time_steps + s.equi_steps + 1) / s.traj_write_int);
+        std::fprintf(out.ascii, "Header End\n");
+    }
+    if (s.write_cont_force) {
+        out.forces = std::fopen(s.forces_file.c_str(), "w");
+        if (out.forces)
+            std::fprintf(out.forces, "step,num_cells,cell_ind,node_ind,glob_node_ind,FX,FY,FZ,F,VX,VY,VZ,V,X,Y,Z,P,Vol\n");
+    }
+    if (FILE* f = std::fopen("velocity2.xyz", "w")) std::fclose(f);
+    // frame 0: the start positions (the device start-up does not move nodes)
+    int frame = 1;
+    if (!write_positions(s, out, 0, frame, n_cells, X, Y, Z)) return -1;
+    if (setup_only) {  // parameters, mesh, placement and frame 0 only; no device
+        if (out.bin) cs3d_traj_close(out.bin, n_cells);
+        if (out.ascii) std::fclose(out.ascii);
+        if (out.forces) std::fclose(out.forces);
+        std::printf("setup done: %d cells placed\n", n_cells);
+        return 0;
+    }
+
+    // device pools: voxel lists for at least the 1024*1024/4 counters the reference
+    // clears (GPUbounce.cu:1329), search bins sized like the engine's default
+    const int64_t vcells = std::max<int64_t>(1 << 18, 32 * (int64_t)s.max_cells);
+    const int64_t vwords = vcells * (1 + s.p.voxel_capacity);
+    const int64_t bwords = std::max<int64_t>(1 << 22, (int64_t)s.max_cells * 640 * 65);
+    cs3d_ctx* ctx = nullptr;
+    cs3d_status st = cs3d_ctx_create(gpuid, &mesh, s.max_cells, vwords, bwords, &ctx);
+    if (st != CS3D_OK) {
+        std::printf("Could not set to divice %d: %s\n", gpuid, cs3d_last_error());
+        return -1;
+    }
+    cs3d_set_graph(ctx, graph ? 1 : 0);
+    cs3d_sys sys = -1;
+    st = cs3d_system_add(ctx, &s.p, n_cells, s.max_cells, vcells, X.data(), Y.data(), Z.data(), nullptr, nullptr,
+                         nullptr, &sys);
+    if (st != CS3D_OK) {
+        std::printf("ERROR: system: %s (%s)\n", cs3d_status_str(st), cs3d_last_error());
+        cs3d_ctx_destroy(ctx);
+        return -1;
+    }
+    // the RANMAR stream continues where placement left it (one stream, as in the reference)
+    cs3d_set(ctx, sys, CS3D_F_RANMAR, &rng, sizeof(rng));
+
+    const int total = s.time_steps + s.equi_steps;
+    int n = n_cells;
+    int rcode = 0;
+    if (!write_forces(ctx, sys, out, 0, n, X, Y, Z)) rcode = -1;
+    int step = 0;
+    while (rcode == 0 && step < total) {
+        const int next = std::min(total, (step / s.traj_write_int + 1) * s.traj_write_int);
+        st = cs3d_step(ctx, sys, next - step);
+        if (st == CS3D_OK) st = cs3d_sync(ctx);
+        if (st != CS3D_OK) {
+            std::printf("Error: %s (%s)\n", cs3d_status_str(st), cs3d_last_error());
+            rcode = -1;
+            break;
+        }
+        int32_t ncur = 0;
+        int64_t done = 0;
+        uint32_t status = 0;
+        cs3d_system_info(ctx, sys, &ncur, &done, &status);
+        step = (int)done;
+        n = ncur;
+        if (status & CS3D_SYS_CAPACITY) {
+            std::printf("ERROR: Population is %d, only allocated enough memory for %d\n"
+                        "ERROR: Fatal error, crashing...\n", n + 1, s.max_cells);
+            rcode = -69;
+            break;
+        }
+        if (status & CS3D_SYS_FATAL_MASK) {
+            std::printf("Device failure at step %d: system status 0x%x\n", step, status);
+            rcode = 0;
+            break;
+        }
+        if (step < next) {
+            std::printf("Error: the system stopped at step %d\n", step);
+            rcode = -1;
+            break;
+        }
+        if (step % s.traj_write_int == 0) {
+            ++frame;
+            if (!write_frame(ctx, sys, s, out, step, frame, n)) rcode = -1;
+        }
+    }
+    if (out.bin) cs3d_traj_close(out.bin, n);
+    if (out.ascii) std::fclose(out.ascii);
+    if (out.forces) std::fclose(out.forces);
+    cs3d_ctx_destroy(ctx);
+    std::printf("%d cells after %d steps\n", n, step);
+    return rcode;
+}

@@ -23,7 +23,7 @@ MP = C.POINTER(abi.Mesh)
 RP = C.POINTER(abi.RanmarState)
 st_t = C.c_int
 PHASES = 5
-PHASE_NAMES = ("integrate", "con_dis", "dis_post", "divide", "grid")
+PHASE_NAMES = ("integrate", "search", "con_dis", "dis_post", "divide")
 
 # every symbol declared in include/cs3d.h with its (argtypes, restype)
 SIGNATURES = {

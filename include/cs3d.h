@@ -220,14 +220,15 @@ cs3d_status cs3d_set_graph(cs3d_ctx* ctx, int32_t enable);
 /* Number of steps captured per HIP graph (default 10). */
 cs3d_status cs3d_set_graph_steps(cs3d_ctx* ctx, int32_t steps);
 
-/* Phases of one step, in launch order. */
+/* Phases of one step, in launch order (one kernel launch each). */
 #define CS3D_PHASES 5
-#define CS3D_PH_INTEGRATE 0  /* Integrate + ForwardTime + PressureUpdate; contact-candidate search
-                                (search bins + candidate lists) on the steps that need it */
-#define CS3D_PH_CON_DIS 1    /* CalculateConForce + first CalculateDisForce + VelocityUpdateA/B */
-#define CS3D_PH_DIS_POST 2   /* second CalculateDisForce + CenterOfMass + volumes */
-#define CS3D_PH_DIVIDE 3     /* division phase and end of step */
-#define CS3D_PH_GRID 4       /* bounding boxes + extent + voxel lists */
+#define CS3D_PH_INTEGRATE 0  /* Integrate + ForwardTime + PressureUpdate; adoption of the bboxes and
+                                voxel counts of a grid rebuilt at the end of the previous step;
+                                search-bin insertion on steps that search */
+#define CS3D_PH_SEARCH 1     /* contact-candidate lists (only on steps that search) */
+#define CS3D_PH_CON_DIS 2    /* CalculateConForce + first CalculateDisForce + VelocityUpdateA/B */
+#define CS3D_PH_DIS_POST 3   /* second CalculateDisForce + CenterOfMass + volumes + bounding_boxes */
+#define CS3D_PH_DIVIDE 4     /* division phase, end of step, extent + grid geometry on rebuild steps */
 /* Runs n_steps eagerly with HIP events recorded on the context's stream
  * around each phase and adds the elapsed milliseconds of every phase to
  * phase_ms[CS3D_PHASES] (no counterpart in the reference, which has no timers;
