@@ -26,6 +26,17 @@ def unit_mesh():
     return _MESH_CACHE["gp30"]
 
 
+# volume of the reference's own C180 mesh (radius ~0.58); inp.json's division_Vol
+# of 2.9 is 3.69 times it (inp.json:14, GPUbounce.cu:1656 uses 0.786)
+REF_C180_VOLUME = 0.78665
+REF_DIVISION_VOL = 2.9
+
+
+def division_vol_unit(v0):
+    """inp.json's division volume carried over to a mesh of volume v0."""
+    return REF_DIVISION_VOL / REF_C180_VOLUME * v0
+
+
 def mesh_volume(m):
     """Polyhedral volume of a centred mesh in float64 (signed fan decomposition
     about each face centroid; independent of the kernel formula's abs value)."""
@@ -133,7 +144,8 @@ def colony(seed=1, max_cells=20000):
 
 def ensemble(n_tissues=256, seed=1, nmin=1, nmax=512, phi=0.30, steps=5000):
     """Independent tissues: sizes log-uniform in [nmin, nmax]; stiffness, growth
-    rate and adhesion strength each scaled by U[0.7, 1.3]."""
+    rate and adhesion strength each scaled by U[0.7, 1.3]; division at inp.json's
+    division volume carried over to the unit-radius mesh."""
     rng = np.random.default_rng(seed)
     sizes = np.rint(np.exp(rng.uniform(np.log(nmin), np.log(nmax), size=n_tissues))).astype(int)
     sizes = np.clip(sizes, nmin, nmax)
@@ -144,12 +156,12 @@ def ensemble(n_tissues=256, seed=1, nmin=1, nmax=512, phi=0.30, steps=5000):
         _, X, Y, Z, info = lattice_tissue(int(n), phi, seed=seed * 1000 + t)
         p = base_params(1 + (seed * 7 + t) % 300)
         f = rng.uniform(0.7, 1.3, size=3)
-        p.youngs_mod = np.float32(p.youngs_mod * f[0])
-        p.stiffness1 = p.youngs_mod
-        p.stiffness2 = np.float32(0.8 * p.youngs_mod)
+        # bond stiffness = stiffFactor1 * Youngs_mod (GPUbounce.cu:1953); the contact
+        # forces keep the global Youngs_mod
+        p.stiffness1 = np.float32(p.stiffness1 * f[0])
         p.growth_rate = np.float32(p.growth_rate * f[1])
         p.attraction_strength = np.float32(p.attraction_strength * f[2])
-        p.division_vol = 2.0 * v0
+        p.division_vol = division_vol_unit(v0)
         p.div_time_steps = steps
         p.non_div_time_steps = 0
         out.append({"params": p, "X": X, "Y": Y, "Z": Z, "n": int(n), "steps": steps})

@@ -297,6 +297,7 @@ extern "C" cs3d_status cs3d_ctx_create(int device, const cs3d_mesh* mesh, int64_
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(CS3D_HIP_ERROR, "hipStreamCreate failed"));
     Pool& p = c->pool;
+    p.wg_per_cu = 16;
     const int64_t ns = capacity_cells * kSlots;
     cs3d_status st = CS3D_OK;
     float** nodes[] = {&p.X, &p.Y, &p.Z, &p.VX, &p.VY, &p.VZ, &p.VAX, &p.VAY, &p.VAZ,
@@ -606,6 +607,29 @@ extern "C" cs3d_status cs3d_set_graph(cs3d_ctx* c, int32_t enable) {
     if (!c) return fail(CS3D_INVALID_ARG, "NULL context");
     c->use_graph = enable != 0;
     if (!c->use_graph) drop_graphs(c);
+    return CS3D_OK;
+}
+
+extern "C" cs3d_status cs3d_set_launch(cs3d_ctx* c, int32_t wg_per_cu) {
+    if (!c || wg_per_cu < 1 || wg_per_cu > 1024) return fail(CS3D_INVALID_ARG, "wg_per_cu must be in [1, 1024]");
+    c->pool.wg_per_cu = wg_per_cu;
+    drop_graphs(c);
+    return CS3D_OK;
+}
+
+extern "C" cs3d_status cs3d_systems_poll(cs3d_ctx* c, int32_t first, int32_t count, int32_t* n_cells, int64_t* steps,
+                                         uint32_t* status) {
+    if (!c || first < 0 || count < 0 || first + count > kMaxSys) return fail(CS3D_INVALID_ARG, "bad poll range");
+    if (count == 0) return CS3D_OK;
+    std::vector<SysMeta> m(count);
+    HIPCHK(hipMemcpyAsync(m.data(), c->pool.meta + first, sizeof(SysMeta) * count, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int32_t i = 0; i < count; ++i) {
+        const bool used = c->sys[first + i].used;
+        if (n_cells) n_cells[i] = used ? m[i].n : -1;
+        if (steps) steps[i] = used ? m[i].step : -1;
+        if (status) status[i] = used ? m[i].status : 0u;
+    }
     return CS3D_OK;
 }
 

@@ -98,6 +98,7 @@ struct Pool {
     int32_t* vox;
     int32_t* bins;
     const MeshDev* mesh;
+    int32_t wg_per_cu;  // host side: cap of the grid of the cell kernels, workgroups per CU
 };
 
 // What a launch works on: cell slots [slot_lo, slot_hi) and systems [sys_lo, sys_hi).

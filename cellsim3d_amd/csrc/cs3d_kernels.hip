@@ -1382,23 +1382,25 @@ __global__ void k_set_flags(Pool p, Range r, int32_t rebuild, int32_t search) {
 // ------------------------------------------------------------------------------
 // launchers (called by cs3d_api.hip)
 // ------------------------------------------------------------------------------
+// Grid of a cell kernel: one workgroup per cell slot up to 256 CUs x wg_per_cu
+// workgroups; beyond that each workgroup strides over several cells.
 static int grid_cells(const Range& r, int per_cu) {
     const int n = r.slot_hi - r.slot_lo;
-    const int cap = 256 * per_cu;
+    const int cap = 256 * (per_cu > 0 ? per_cu : 16);
     return n < 1 ? 1 : (n < cap ? n : cap);
 }
 
 void launch_integrate(const Pool& p, const Range& r, int flags, int64_t step_override, hipStream_t st) {
-    hipLaunchKernelGGL(k_integrate, dim3(grid_cells(r, 16)), dim3(192), 0, st, p, r, flags, step_override);
+    hipLaunchKernelGGL(k_integrate, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), 0, st, p, r, flags, step_override);
 }
 void launch_search(const Pool& p, const Range& r, int always, hipStream_t st) {
-    hipLaunchKernelGGL(k_search, dim3(grid_cells(r, 16)), dim3(192), 0, st, p, r, always);
+    hipLaunchKernelGGL(k_search, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), 0, st, p, r, always);
 }
 void launch_con(const Pool& p, const Range& r, int mode, bool use_va, hipStream_t st) {
     const float* vx = use_va ? p.VAX : p.VX;
     const float* vy = use_va ? p.VAY : p.VY;
     const float* vz = use_va ? p.VAZ : p.VZ;
-    hipLaunchKernelGGL(k_con_dis, dim3(grid_cells(r, 16)), dim3(192), 0, st, p, r, mode, p.X, p.Y, p.Z, vx, vy, vz,
+    hipLaunchKernelGGL(k_con_dis, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), 0, st, p, r, mode, p.X, p.Y, p.Z, vx, vy, vz,
                        p.BBOX);
 }
 void launch_dis(const Pool& p, const Range& r, int mode, int64_t step_override, int clear_bins, bool use_va,
@@ -1406,7 +1408,7 @@ void launch_dis(const Pool& p, const Range& r, int mode, int64_t step_override, 
     const float* vx = use_va ? p.VAX : p.VX;
     const float* vy = use_va ? p.VAY : p.VY;
     const float* vz = use_va ? p.VAZ : p.VZ;
-    hipLaunchKernelGGL(k_dis_post, dim3(grid_cells(r, 16)), dim3(192), 0, st, p, r, mode, step_override, clear_bins,
+    hipLaunchKernelGGL(k_dis_post, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), 0, st, p, r, mode, step_override, clear_bins,
                        p.X, p.Y, p.Z, vx, vy, vz);
 }
 void launch_divide(const Pool& p, const Range& r, int mode, hipStream_t st) {
@@ -1428,13 +1430,13 @@ void launch_vox_lists(const Pool& p, const Range& r, int vox_blocks, hipStream_t
     const int n = r.slot_hi - r.slot_lo;
     const int cb = (n + 255) / 256;
     const int ns = r.sys_hi - r.sys_lo;
-    hipLaunchKernelGGL(k_grid_adopt, dim3(grid_cells(r, 16)), dim3(192), 0, st, p, r);
+    hipLaunchKernelGGL(k_grid_adopt, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), 0, st, p, r);
     hipLaunchKernelGGL(k_vox_clear, dim3(vox_blocks, ns > 0 ? ns : 1), dim3(256), 0, st, p, r);
     hipLaunchKernelGGL(k_insert, dim3(cb > 0 ? cb : 1), dim3(256), 0, st, p, r, 1);
     hipLaunchKernelGGL(k_sort, dim3(vox_blocks, ns > 0 ? ns : 1), dim3(256), 0, st, p, r, 1);
 }
 void launch_velocity_ab(const Pool& p, const Range& r, hipStream_t st) {
-    hipLaunchKernelGGL(k_velocity_ab, dim3(grid_cells(r, 8)), dim3(192), 0, st, p, r);
+    hipLaunchKernelGGL(k_velocity_ab, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), 0, st, p, r);
 }
 void launch_set_flags(const Pool& p, const Range& r, int rebuild, int search, hipStream_t st) {
     const int ns = r.sys_hi - r.sys_lo;

@@ -51,6 +51,8 @@ SIGNATURES = {
     "cs3d_step": ([vp, C.c_int32, C.c_int32], st_t),
     "cs3d_set_graph": ([vp, C.c_int32], st_t),
     "cs3d_set_graph_steps": ([vp, C.c_int32], st_t),
+    "cs3d_set_launch": ([vp, C.c_int32], st_t),
+    "cs3d_systems_poll": ([vp, C.c_int32, C.c_int32, vp, vp, vp], st_t),
     "cs3d_step_timed": ([vp, C.c_int32, C.c_int32, vp], st_t),
     "cs3d_k_integrate": ([vp, C.c_int32], st_t),
     "cs3d_k_pressure_update": ([vp, C.c_int32, C.c_int64], st_t),
@@ -179,6 +181,18 @@ class Context:
     def set_graph_steps(self, steps):
         check(lib().cs3d_set_graph_steps(self.h, int(steps)), "cs3d_set_graph_steps")
 
+    def set_launch(self, wg_per_cu):
+        check(lib().cs3d_set_launch(self.h, int(wg_per_cu)), "cs3d_set_launch")
+
+    def poll(self, first, count):
+        """(n_cells, steps, status) arrays of systems first .. first+count-1."""
+        n = np.zeros(count, dtype=np.int32)
+        s = np.zeros(count, dtype=np.int64)
+        st = np.zeros(count, dtype=np.uint32)
+        check(lib().cs3d_systems_poll(self.h, int(first), int(count), n.ctypes.data_as(vp), s.ctypes.data_as(vp),
+                                      st.ctypes.data_as(vp)), "cs3d_systems_poll")
+        return n, s, st
+
     def copy_calibrate(self, nbytes, reps=10):
         """Average ms of a plain copy kernel over nbytes (reads nbytes, writes nbytes)."""
         out = C.c_double()
@@ -187,7 +201,7 @@ class Context:
 
     def step_timed(self, n_steps, target=abi.ALL):
         """Eager steps with HIP events around each phase; returns total ms per phase
-        (integrate, con_dis, dis_post, divide, grid)."""
+        (integrate, search, con_dis, dis_post, divide)."""
         sid = target.id if isinstance(target, System) else target
         out = np.zeros(PHASES, dtype=np.float64)
         check(lib().cs3d_step_timed(self.h, sid, int(n_steps), out.ctypes.data_as(vp)), "cs3d_step_timed")

@@ -1,0 +1,200 @@
+"""Serving: a request loop with continuous batching on one GPU.
+
+Simulation requests (a tissue, its parameters and a horizon in steps) arrive
+over time.  The server admits a request as soon as its state fits in the
+device pool (cell slots, voxel and search-bin regions, which are sized from an
+HBM budget), uploads it once, and from then on steps it together with every
+other resident tissue: each chunk is one cs3d_step over all systems, and a
+system that reaches its horizon stops on the device while the others go on.
+Between chunks the server polls all systems with one device read, returns the
+final state of the finished ones and frees their slots for the queue.
+
+The reference has no serving mode; a tissue's result is the same whether it
+runs alone or in the mix (tests/test_gpu_serving.py checks this bit for bit).
+
+    python -m cellsim3d_amd.serving [--requests 64] [--rate 20] ...
+prints one JSON line: p50/p99 completion latency, aggregate node-updates/s and
+the utilisation of the offered load.
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import abi, configs, engine
+from . import mesh as meshmod
+
+# device bytes per cell slot (node arrays, contact lists, per-cell scalars),
+# and the voxel and search-bin words cs3d_system_add reserves per system:
+# max(4096, 32 * cells) voxels of (1 + capacity) words, max(16384, 640 * cells)
+# bins of 65 words
+SLOT_BYTES = (15 + 3) * 192 * 4 + 2 * 40 * 192 * 4 + 2 * 192 * 4 + 16 * 4 + 48 + 1
+VOX_WORDS_PER_CELL = 32 * 65
+BIN_WORDS_PER_CELL = 640 * 65
+VOX_WORDS_MIN = 4096 * 65
+BIN_WORDS_MIN = 16384 * 65
+SMALL_SYSTEMS = 64  # room for this many systems at the minimum reservations
+
+
+class Job:
+    def __init__(self, jid, arrival, params, X, Y, Z, n, horizon, max_cells=None):
+        self.id = jid
+        self.arrival = arrival
+        self.params = params
+        self.X, self.Y, self.Z = X, Y, Z
+        self.n = n
+        self.horizon = horizon
+        self.max_cells = max_cells or (n + max(2, n // 8))
+        self.admitted = None
+        self.finished = None
+        self.result = None
+
+    def bytes(self):
+        return self.max_cells * SLOT_BYTES + 4 * (max(VOX_WORDS_MIN, self.max_cells * VOX_WORDS_PER_CELL) +
+                                                  max(BIN_WORDS_MIN, self.max_cells * BIN_WORDS_PER_CELL))
+
+
+def make_jobs(requests, grid_interval=10, phi=0.30):
+    """Jobs from configs.serving_requests entries (seeded tissues)."""
+    jobs = []
+    for r in requests:
+        _, X, Y, Z, info = configs.lattice_tissue(r["n"], phi, seed=r["seed"])
+        p = configs.base_params(1 + r["seed"] % 300)
+        # fixed cell counts: horizons reach 1e5 steps, over which growing cells would
+        # divide many times; growth_rate = 0 turns volumes and division off
+        # (GPUbounce.cu:1168)
+        p.growth_rate = 0.0
+        p.division_vol = configs.division_vol_unit(info["v0"])
+        p.div_time_steps = r["horizon"]
+        p.non_div_time_steps = 0
+        p.grid_interval = grid_interval
+        jobs.append(Job(len(jobs), r["arrival"], p, X, Y, Z, r["n"], r["horizon"]))
+    return jobs
+
+
+class Server:
+    def __init__(self, gm, hbm_budget_bytes, max_cells_total=None, chunk_steps=50, graph_steps=10,
+                 wg_per_cu=16, fields=(abi.F_X, abi.F_Y, abi.F_Z)):
+        per_cell = SLOT_BYTES + 4 * (VOX_WORDS_PER_CELL + BIN_WORDS_PER_CELL)
+        cap = int(hbm_budget_bytes // per_cell)
+        if max_cells_total:
+            cap = min(cap, int(max_cells_total))
+        self.capacity = cap
+        self.ctx = engine.Context(gm, capacity_cells=cap,
+                                  voxel_budget=cap * VOX_WORDS_PER_CELL + SMALL_SYSTEMS * VOX_WORDS_MIN,
+                                  bin_budget=cap * BIN_WORDS_PER_CELL + SMALL_SYSTEMS * BIN_WORDS_MIN)
+        self.ctx.set_graph_steps(graph_steps)
+        self.ctx.set_launch(wg_per_cu)
+        self.chunk = chunk_steps
+        self.fields = fields
+        self.node_updates = 0.0
+
+    def close(self):
+        self.ctx.close()
+
+    def _admit(self, job, now):
+        try:
+            s = self.ctx.add_system(job.params, job.X, job.Y, job.Z, job.n, max_cells=job.max_cells,
+                                    voxels=max(4096, 32 * job.max_cells))
+        except engine.EngineError as e:
+            if e.status == abi.CAPACITY:
+                return None
+            raise
+        s.set_horizon(job.horizon)
+        job.admitted = now
+        return s
+
+    def _finish(self, job, s, now):
+        n = s.n
+        job.result = {f: s.get(f)[:n * 192].copy() for f in self.fields}
+        job.result["n"] = n
+        job.result["status"] = s.info()[2]
+        job.finished = now
+        s.remove()
+
+    def run(self, jobs, time_scale=1.0, log=None):
+        """Serves jobs (arrival times in seconds, scaled by time_scale) and returns
+        the wall time.  Each job gets admitted/finished times and a result."""
+        queue = sorted(jobs, key=lambda j: j.arrival)
+        active = {}
+        t0 = time.perf_counter()
+        qi = 0
+        while qi < len(queue) or active:
+            now = time.perf_counter() - t0
+            while qi < len(queue) and queue[qi].arrival * time_scale <= now:
+                s = self._admit(queue[qi], now)
+                if s is None:
+                    if not active:
+                        raise RuntimeError("job %d (%d cells) does not fit an empty pool of %d cells" %
+                                           (queue[qi].id, queue[qi].n, self.capacity))
+                    break  # first in line waits for room (arrival order)
+                active[s.id] = (queue[qi], s)
+                qi += 1
+            if not active:
+                if qi < len(queue):
+                    time.sleep(max(0.0, queue[qi].arrival * time_scale - (time.perf_counter() - t0)))
+                continue
+            self.ctx.step(self.chunk)
+            hi = max(active) + 1
+            ncell, steps, status = self.ctx.poll(0, hi)
+            now = time.perf_counter() - t0
+            for sid in list(active):
+                job, s = active[sid]
+                done = steps[sid] >= job.horizon or (status[sid] & abi.SYS_FATAL_MASK)
+                if done:
+                    self.node_updates += float(job.n) * 180.0 * float(steps[sid])
+                    self._finish(job, s, now)
+                    del active[sid]
+                    if log:
+                        log("job %d done: %d cells, %d steps, latency %.3f s" %
+                            (job.id, job.n, job.horizon, now - job.arrival * time_scale))
+        return time.perf_counter() - t0
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--requests", type=int, default=64)
+    ap.add_argument("--rate", type=float, default=20.0)
+    ap.add_argument("--nmax", type=int, default=4096)
+    ap.add_argument("--hmin", type=float, default=1e3)
+    ap.add_argument("--hmax", type=float, default=1e5)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--max-cells", type=int, default=200000, help="cap of the device pool in cells")
+    ap.add_argument("--chunk", type=int, default=50)
+    ap.add_argument("--seed", type=int, default=1)
+    args = ap.parse_args(argv)
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    m = configs.unit_mesh()
+    gm = engine.build_mesh(meshmod.tables(m), configs.rest_positions(m))
+    reqs = configs.serving_requests(args.requests, args.seed, args.rate, 8, args.nmax, args.hmin, args.hmax)
+    jobs = make_jobs(reqs)
+    offered = sum(j.n * 180.0 * j.horizon for j in jobs)
+    srv = Server(gm, args.budget_gb * 1e9, max_cells_total=args.max_cells, chunk_steps=args.chunk)
+    log("serving: %d requests, %d cells in total, pool of %d cells" % (len(jobs), sum(j.n for j in jobs),
+                                                                      srv.capacity))
+    wall = srv.run(jobs, log=log)
+    srv.close()
+    lat = np.array([j.finished - j.arrival for j in jobs])
+    span = jobs[-1].arrival if jobs else 0.0
+    out = {
+        "workload": "%d requests, Poisson %.0f/s, N ~ log-U[8, %d], horizon ~ log-U[%.0e, %.0e] steps, "
+                    "FP32, growth off" % (
+            len(jobs), args.rate, args.nmax, args.hmin, args.hmax),
+        "p50_latency_s": float(np.percentile(lat, 50)),
+        "p99_latency_s": float(np.percentile(lat, 99)),
+        "wall_s": wall,
+        "node_updates_per_s": srv.node_updates / wall,
+        "offered_node_updates_per_s": offered / span if span > 0 else None,
+        "utilisation": (offered / span) / (srv.node_updates / wall) if span > 0 else None,
+    }
+    print(json.dumps(out), flush=True)
+    return out
+
+
+if __name__ == "__main__":
+    main()

@@ -138,7 +138,7 @@ typedef enum cs3d_field {
     CS3D_F_VOXLIST,                            /* int32 budget*capacity, d_NNlist (each list ascending) */
     CS3D_F_RANMAR,                             /* cs3d_ranmar_state */
     CS3D_F_PAIRCOUNT,                          /* int32 per node: stored contact partners (engine-specific) */
-    CS3D_F_PAIRS,                              /* int32 32 x 192 per cell: partner keys cell*192 + node,
+    CS3D_F_PAIRS,                              /* int32 40 x 192 per cell: partner keys cell*192 + node,
                                                   ascending, k-major (engine-specific) */
     CS3D_F_GROWTH_SCALE,                       /* per-cell multiplier of the pressure increment (default 1) */
     CS3D_F__COUNT
@@ -219,6 +219,16 @@ cs3d_status cs3d_step(cs3d_ctx* ctx, cs3d_sys sys, int32_t n_steps);
 cs3d_status cs3d_set_graph(cs3d_ctx* ctx, int32_t enable);
 /* Number of steps captured per HIP graph (default 10). */
 cs3d_status cs3d_set_graph_steps(cs3d_ctx* ctx, int32_t steps);
+/* Launch shape of the cell kernels: at most 256 x wg_per_cu workgroups (one
+ * cell each; beyond that a workgroup strides over several cells).  Default 16.
+ * No counterpart in the reference (one block per cell, GPUbounce.cu:1059-1164);
+ * set by the ensemble autotuner. */
+cs3d_status cs3d_set_launch(cs3d_ctx* ctx, int32_t wg_per_cu);
+/* Cell counts, completed steps and status bits of systems first .. first+count-1
+ * in one device read (-1 / 0 for unused ids); for the serving loop, which has
+ * no counterpart in the reference. */
+cs3d_status cs3d_systems_poll(cs3d_ctx* ctx, int32_t first, int32_t count, int32_t* n_cells, int64_t* steps,
+                              uint32_t* status);
 
 /* Phases of one step, in launch order (one kernel launch each). */
 #define CS3D_PHASES 5
