@@ -13,6 +13,13 @@ Prints ONE JSON line.  `roofline` covers the dominant kernel (the fused
 contact/friction kernel), timed live with HIP events on the engine's stream;
 `cpu_baseline` is the CPU oracle (a C++/OpenMP restatement of the reference
 kernels) timed on this host on a bounded sample of the same workload.
+Further objects in the same line:
+  bonded_roofline  the standalone bonded-force kernel on 20,000 cells (3.6 M
+                   nodes), springs + pressure (HBM-bound, 24 B per node) and
+                   with bending (VALU-bound), HIP-event timed
+  ensemble         256 independent tissues batched in one context against the
+                   same tissues run one after another (plain path, sampled)
+  serving          a Poisson request stream through the continuous-batching loop
 """
 import argparse
 import json
@@ -29,6 +36,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 #   con_dis:   reads x, v_after_integrate (24 B), writes F_con, v (24 B)
 #   dis_post:  reads x, v (24 B), writes F_dis (12 B)
 ALG_BYTES = {"integrate": 72.0, "con_dis": 48.0, "dis_post": 36.0}
+BONDED_BYTES = 24.0  # x in, F_con out, per node
 
 
 def log(msg):
@@ -68,6 +76,72 @@ def cpu_baseline(cfg, mesh_tables, rest, budget_s=20.0):
     }
 
 
+def bonded_roofline(gm, n_cells=20000, log=None):
+    """Standalone bonded kernel (cs3d_k_con_force_bonded) on n_cells cells."""
+    from cellsim3d_amd import configs, engine
+    cfg = configs.single_tissue(n_cells, 0.40, seed=2, grid_interval=10, steps=10)
+    res = {}
+    with engine.Context(gm, capacity_cells=2 * n_cells + 8, voxel_budget=1 << 26) as ctx:
+        for angles in (0, 1):
+            p = cfg["params"].copy()
+            p.constrain_angles = angles
+            s = ctx.add_system(p, cfg["X"], cfg["Y"], cfg["Z"], n_cells, max_cells=n_cells, voxels=1 << 20)
+            ms = s.time_kernel(0, reps=50)
+            gbs = BONDED_BYTES * n_cells * 180 / (ms * 1e-3) / 1e9
+            key = "with_bending" if angles else "springs_pressure"
+            res[key] = {"avg_launch_ms": round(ms, 5), "achieved_GBs": round(gbs, 1),
+                        "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)}
+            if log:
+                log("bench: bonded kernel (%s) %d cells: %.4f ms, %.0f GB/s" % (key, n_cells, ms, gbs))
+            s.remove()
+    res["workload"] = "%d cells (%d nodes), %d B per node (x in, F_con out)" % (n_cells, n_cells * 180,
+                                                                               BONDED_BYTES)
+    return res
+
+
+def measure_extras(args, gm, log):
+    from cellsim3d_amd import configs, ensemble, serving
+    out = {"bonded_roofline": bonded_roofline(gm, log=log)}
+    g, tissues = ensemble.build(256, 1, 1, 512, args.ensemble_steps, 10)
+    wg, gs, _ = ensemble.autotune(g, tissues, probe_steps=50)
+    dt_b, ctx, _ = ensemble.run_batched(g, tissues, args.ensemble_steps, wg_per_cu=wg, graph_steps=gs, warmup=gs)
+    ctx.close()
+    sample = min(100, args.ensemble_steps)
+    dt_p = ensemble.run_plain(g, tissues, args.ensemble_steps, sample_steps=sample)
+    nu = ensemble.node_updates(tissues, args.ensemble_steps)
+    cells = sum(t["n"] for t in tissues)
+    out["ensemble"] = {
+        "workload": "256 tissues, N ~ log-U[1, 512] (%d cells), stiffness, growth rate and adhesion x U[0.7, 1.3], "
+                    "%d steps, FP32" % (cells, args.ensemble_steps),
+        "batched_sims_per_hour": round(256 / dt_b * 3600.0, 1),
+        "batched_node_updates_per_s": nu / dt_b,
+        "plain_sims_per_hour": round(256 / dt_p * 3600.0, 1),
+        "plain_node_updates_per_s": nu / dt_p,
+        "plain_sample": "%d timed steps per tissue, scaled to %d" % (sample, args.ensemble_steps),
+        "batched_over_plain": round(dt_p / dt_b, 2),
+        "autotuned": {"wg_per_cu": wg, "graph_steps": gs},
+    }
+    log("bench: ensemble batched %.3f s, plain %.3f s (x%.1f)" % (dt_b, dt_p, dt_p / dt_b))
+    reqs = configs.serving_requests(args.serving_requests, 1, 20.0, 8, 4096, 1e3, 1e4)
+    jobs = serving.make_jobs(reqs)
+    srv = serving.Server(g, 200e9, max_cells_total=100000, chunk_steps=50)
+    wall = srv.run(jobs)
+    srv.close()
+    import numpy as np
+    lat = np.array([j.finished - j.arrival for j in jobs])
+    offered = sum(j.n * 180.0 * j.horizon for j in jobs) / jobs[-1].arrival
+    out["serving"] = {
+        "workload": "%d requests, Poisson 20/s, N ~ log-U[8, 4096], horizon ~ log-U[1e3, 1e4] steps, FP32, "
+                    "growth off, 200 GB budget" % len(jobs),
+        "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+        "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+        "node_updates_per_s": srv.node_updates / wall,
+        "utilisation": round(offered / (srv.node_updates / wall), 3),
+    }
+    log("bench: serving p50 %.3f s p99 %.3f s" % (out["serving"]["p50_latency_s"], out["serving"]["p99_latency_s"]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +155,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run the known-size copy kernel (for HBM counter scaling)")
+    ap.add_argument("--no-extras", action="store_true", help="skip the bonded, ensemble and serving measurements")
+    ap.add_argument("--ensemble-steps", type=int, default=1000)
+    ap.add_argument("--serving-requests", type=int, default=32)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -143,6 +220,9 @@ def main():
         calib = {"bytes_read": nbytes, "bytes_written": nbytes, "avg_ms": cms,
                  "GBs": round(2 * nbytes / (cms * 1e-3) / 1e9, 1)}
         log("bench: copy kernel %d MiB: %.4f ms (%.0f GB/s read+write)" % (args.calib_copy_mb, cms, calib["GBs"]))
+    extras = {}
+    if not args.no_extras:
+        extras = measure_extras(args, gm, log)
     cpu = None
     if not args.no_cpu:
         log("bench: CPU oracle baseline (~%.0f s)" % args.cpu_seconds)
@@ -172,6 +252,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    out.update(extras)
     if calib:
         out["copy_calibration"] = calib
     if cpu:

@@ -162,6 +162,9 @@ def ensemble(n_tissues=256, seed=1, nmin=1, nmax=512, phi=0.30, steps=5000):
         p.growth_rate = np.float32(p.growth_rate * f[1])
         p.attraction_strength = np.float32(p.attraction_strength * f[2])
         p.division_vol = division_vol_unit(v0)
+        # daughters double the local density; lists of 64 keep dense voxels within
+        # capacity (the reference's 32-entry lists would silently overflow)
+        p.voxel_capacity = 64
         p.div_time_steps = steps
         p.non_div_time_steps = 0
         out.append({"params": p, "X": X, "Y": Y, "Z": Z, "n": int(n), "steps": steps})

@@ -19,6 +19,7 @@ void launch_search(const Pool& p, const Range& r, int always, hipStream_t st);
 void launch_con(const Pool& p, const Range& r, int mode, bool use_va, hipStream_t st);
 void launch_dis(const Pool& p, const Range& r, int mode, int64_t step_override, int clear_bins, bool use_va,
                 hipStream_t st);
+void launch_bonded(const Pool& p, const Range& r, int angles, hipStream_t st);
 void launch_divide(const Pool& p, const Range& r, int mode, hipStream_t st);
 void launch_grid(const Pool& p, const Range& r, int vox_blocks, hipStream_t st);
 void launch_vox_lists(const Pool& p, const Range& r, int vox_blocks, hipStream_t st);
@@ -707,7 +708,7 @@ cs3d_status finish(cs3d_ctx* c) {
 extern "C" cs3d_status cs3d_k_integrate(cs3d_ctx* c, cs3d_sys s) {
     if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
     const Range r = range_of(c, s);
-    launch_integrate(c->pool, r, I_KICK, 0, c->stream);
+    launch_integrate(c->pool, r, I_KICK | I_WRITEV, 0, c->stream);
     launch_set_flags(c->pool, r, -1, 1, c->stream);
     return finish(c);
 }
@@ -731,7 +732,7 @@ extern "C" cs3d_status cs3d_k_con_force(cs3d_ctx* c, cs3d_sys s) {
 
 extern "C" cs3d_status cs3d_k_con_force_bonded(cs3d_ctx* c, cs3d_sys s) {
     if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
-    launch_con(c->pool, range_of(c, s), M_BONDED, false, c->stream);
+    launch_bonded(c->pool, range_of(c, s), c->sys[s].prm.constrain_angles ? 1 : 0, c->stream);
     return finish(c);
 }
 
@@ -784,6 +785,31 @@ extern "C" cs3d_status cs3d_k_divide(cs3d_ctx* c, cs3d_sys s, int32_t* n_events)
         if (st != CS3D_OK) return st;
         *n_events = m.n_events;
     }
+    return CS3D_OK;
+}
+
+extern "C" cs3d_status cs3d_time_kernel(cs3d_ctx* c, cs3d_sys s, int32_t which, int32_t reps, double* avg_ms) {
+    if (!valid_sys(c, s) || reps < 1 || !avg_ms || which < 0 || which > 1)
+        return fail(CS3D_INVALID_ARG, "bad kernel timing arguments");
+    const Range r = range_of(c, s);
+    auto launch = [&]() {
+        if (which == 0) launch_bonded(c->pool, r, c->sys[s].prm.constrain_angles ? 1 : 0, c->stream);
+        else launch_dis(c->pool, r, M_COM, 0, 0, false, c->stream);
+    };
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    launch();  // warm-up
+    (void)hipEventRecord(e0, c->stream);
+    for (int q = 0; q < reps; ++q) launch();
+    (void)hipEventRecord(e1, c->stream);
+    hipError_t e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return fail(CS3D_HIP_ERROR, hipGetErrorString(e));
+    *avg_ms = ms / reps;
     return CS3D_OK;
 }
 

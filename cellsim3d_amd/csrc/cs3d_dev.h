@@ -130,6 +130,7 @@ enum : int32_t {
     I_IFSEARCH = 32,  // I_BINS only on systems that search this step
     I_GRIDCOUNT = 64, // systems whose grid was rebuilt at the end of the last step: adopt the new
                       // bboxes and count the voxel occupancy (overflow check)
+    I_WRITEV = 128,   // with I_KICK: also store the new velocity in V (standalone Integrate)
 };
 
 }  // namespace cs3d

@@ -191,6 +191,7 @@ __global__ __launch_bounds__(192) void k_integrate(Pool p, Range r, int32_t flag
             y = y + vy * dt;
             z = z + vz * dt;
             p.VAX[gi] = vx; p.VAY[gi] = vy; p.VAZ[gi] = vz;
+            if (flags & I_WRITEV) { p.VX[gi] = vx; p.VY[gi] = vy; p.VZ[gi] = vz; }
             p.X[gi] = x; p.Y[gi] = y; p.Z[gi] = z;
         }
         if (flags & I_DISP) {  // moved pair_skin/2 since the last candidate search?
@@ -275,11 +276,12 @@ __device__ int find_candidates(const Pool& p, const SysMeta* M, const cs3d_param
             bi[q] = in ? (z3 * bdy + y3) * bdx + x3 : 0;
             cn[q] = in ? min(counts[bi[q]], kBCap) : 0;
         }
-        float4 head[3][kHead];
+        float4 head[3][kHead];  // loads of empty slots are masked off
 #pragma unroll
         for (int q = 0; q < 3; ++q)
 #pragma unroll
-            for (int e = 0; e < kHead; ++e) head[q][e] = ents[(int64_t)bi[q] * kBCap + e];
+            for (int e = 0; e < kHead; ++e)
+                head[q][e] = e < cn[q] ? ents[(int64_t)bi[q] * kBCap + e] : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
 #pragma unroll
@@ -683,6 +685,93 @@ __global__ __launch_bounds__(192) void k_con_dis(Pool p, Range r, int32_t mode, 
             }
         }
         if (node && forces) { p.FCX[gi] = FX; p.FCY[gi] = FY; p.FCZ[gi] = FZ; }
+        if (st) flag(M, st);
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------
+// Bonded part of CalculateConForce alone (springs, pressure, and angles when
+// constrainAngles): propagate.cu:253-337, the same arithmetic and summation
+// order as k_con_dis, in a lean kernel for the standalone bonded entry point.
+// Without angles it streams x in and F_con out (24 B per node).
+// ------------------------------------------------------------------------------
+template <bool ANGLES>
+__global__ __launch_bounds__(192) void k_bonded(Pool p, Range r) {
+    __shared__ float sx[kSlots], sy[kSlots], sz[kSlots];
+    __shared__ float angF[ANGLES ? kNodes * 18 : 1];
+    const int tid = threadIdx.x;
+    const bool node = tid < kNodes;
+    const MeshDev* mesh = p.mesh;
+    for (int slot = r.slot_lo + blockIdx.x; slot < r.slot_hi; slot += gridDim.x) {
+        const int s = p.slot_sys[slot];
+        if (s < 0) continue;
+        SysMeta* M = p.meta + s;
+        if (!sys_live(M)) continue;
+        if (slot - M->base >= M->n) continue;
+        const cs3d_params* P = p.prm + s;
+        if ((P->constrain_angles != 0) != ANGLES) continue;
+        const int64_t gi = (int64_t)slot * kSlots + tid;
+        const float X = p.X[gi], Y = p.Y[gi], Z = p.Z[gi];
+        sx[tid] = X; sy[tid] = Y; sz[tid] = Z;
+        __syncthreads();
+        uint32_t st = 0;
+        if (ANGLES) {
+            if (node) {
+                const float k = P->angle_k;
+                const float* th = mesh->theta0 + 3 * tid;
+                const int n0 = mesh->nn[tid], n1 = mesh->nn[kSlots + tid], n2 = mesh->nn[2 * kSlots + tid];
+#pragma unroll 1
+                for (int q = 0; q < 3; ++q) {
+                    const int a = q == 1 ? n1 : n0;
+                    const int b = q == 0 ? n1 : n2;
+                    st |= angle_pair(sx[a] - X, sy[a] - Y, sz[a] - Z, sx[b] - X, sy[b] - Y, sz[b] - Z, th[q], k,
+                                     angF + (tid * 3 + q) * 6);
+                }
+            }
+            __syncthreads();
+        }
+        if (node) {
+            if (isnan(X) || isnan(Y) || isnan(Z)) st |= CS3D_SYS_NAN;
+            float FX = 0.f, FY = 0.f, FZ = 0.f;
+            const float stiffness = p.E[slot];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const int N1 = mesh->nn[i * kSlots + tid];
+                const float dX = sx[N1] - X, dY = sy[N1] - Y, dZ = sz[N1] - Z;
+                const float R = sqrtf(dX * dX + dY * dY + dZ * dZ);
+                const float R0 = mesh->R0[i * kSlots + tid];
+                FX += stiffness * (R - R0) / R0 * dX / R;
+                FY += stiffness * (R - R0) / R0 * dY / R;
+                FZ += stiffness * (R - R0) / R0 * dZ / R;
+            }
+            const float rx = X - p.CMX[slot], ry = Y - p.CMY[slot], rz = Z - p.CMZ[slot];
+            const float inv = 1.0f / sqrtf(rx * rx + ry * ry + rz * rz);
+            const float g = 3.0f * p.P[slot];
+            FX += g * (inv * rx);
+            FY += g * (inv * ry);
+            FZ += g * (inv * rz);
+            if (ANGLES) {
+                const float* a = angF + tid * 18;
+                float ax = 0.f, ay = 0.f, az = 0.f;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    ax = ax - (a[q * 6 + 0] + a[q * 6 + 3]);
+                    ay = ay - (a[q * 6 + 1] + a[q * 6 + 4]);
+                    az = az - (a[q * 6 + 2] + a[q * 6 + 5]);
+                }
+                const int32_t* gt = mesh->gather + tid * 6;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    const float* f = angF + gt[q];
+                    ax = ax + f[0];
+                    ay = ay + f[1];
+                    az = az + f[2];
+                }
+                FX += ax; FY += ay; FZ += az;
+            }
+            p.FCX[gi] = FX; p.FCY[gi] = FY; p.FCZ[gi] = FZ;
+        }
         if (st) flag(M, st);
         __syncthreads();
     }
@@ -1402,6 +1491,11 @@ void launch_con(const Pool& p, const Range& r, int mode, bool use_va, hipStream_
     const float* vz = use_va ? p.VAZ : p.VZ;
     hipLaunchKernelGGL(k_con_dis, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), 0, st, p, r, mode, p.X, p.Y, p.Z, vx, vy, vz,
                        p.BBOX);
+}
+// angles: 0 = systems without bending, 1 = with, -1 = both launches
+void launch_bonded(const Pool& p, const Range& r, int angles, hipStream_t st) {
+    if (angles <= 0) hipLaunchKernelGGL(k_bonded<false>, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), 0, st, p, r);
+    if (angles != 0) hipLaunchKernelGGL(k_bonded<true>, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), 0, st, p, r);
 }
 void launch_dis(const Pool& p, const Range& r, int mode, int64_t step_override, int clear_bins, bool use_va,
                 hipStream_t st) {

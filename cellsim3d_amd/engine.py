@@ -54,6 +54,7 @@ SIGNATURES = {
     "cs3d_set_launch": ([vp, C.c_int32], st_t),
     "cs3d_systems_poll": ([vp, C.c_int32, C.c_int32, vp, vp, vp], st_t),
     "cs3d_step_timed": ([vp, C.c_int32, C.c_int32, vp], st_t),
+    "cs3d_time_kernel": ([vp, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double)], st_t),
     "cs3d_k_integrate": ([vp, C.c_int32], st_t),
     "cs3d_k_pressure_update": ([vp, C.c_int32, C.c_int64], st_t),
     "cs3d_k_con_force": ([vp, C.c_int32], st_t),
@@ -262,6 +263,12 @@ class System:
 
     def step(self, n_steps):
         self.ctx.step(n_steps, self.id)
+
+    def time_kernel(self, which, reps=20):
+        """Average ms of one launch: which 0 = bonded force, 1 = centre of mass."""
+        out = C.c_double()
+        check(lib().cs3d_time_kernel(self.ctx.h, self.id, int(which), int(reps), C.byref(out)), "cs3d_time_kernel")
+        return out.value
 
     # per-kernel entry points
     def k_integrate(self):

@@ -70,6 +70,7 @@ def make_jobs(requests, grid_interval=10, phi=0.30):
         p.div_time_steps = r["horizon"]
         p.non_div_time_steps = 0
         p.grid_interval = grid_interval
+        p.voxel_capacity = 64
         jobs.append(Job(len(jobs), r["arrival"], p, X, Y, Z, r["n"], r["horizon"]))
     return jobs
 

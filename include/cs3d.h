@@ -244,6 +244,11 @@ cs3d_status cs3d_systems_poll(cs3d_ctx* ctx, int32_t first, int32_t count, int32
  * phase_ms[CS3D_PHASES] (no counterpart in the reference, which has no timers;
  * it replaces profiling by hand). Synchronous. */
 cs3d_status cs3d_step_timed(cs3d_ctx* ctx, cs3d_sys sys, int32_t n_steps, double* phase_ms);
+/* Average milliseconds of one launch of a kernel on the system's state, timed
+ * with HIP events over reps launches on the context's stream: which = 0 the
+ * bonded force (cs3d_k_con_force_bonded), 1 CenterOfMass (cs3d_k_com).  For
+ * roofline measurements; no counterpart in the reference. */
+cs3d_status cs3d_time_kernel(cs3d_ctx* ctx, cs3d_sys sys, int32_t which, int32_t reps, double* avg_ms);
 
 /* ---- per-kernel entry points (state injection / parity) ------------------
  * Each runs one reference kernel (or fixed kernel group) on the device state

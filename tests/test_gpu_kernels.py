@@ -1,0 +1,99 @@
+"""Per-kernel parity on injected state: each cs3d_k_* entry point (through the
+C ABI) against the oracle's restatement of the same reference kernel on the
+same inputs.  Bar: 1e-5 scale-relative (bit-exact in practice; the fraction of
+identical values is printed)."""
+import numpy as np
+import pytest
+
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+S = 192
+TOL = 1e-5
+
+
+def _state(n=27, seed=4, angles=1):
+    from cellsim3d_amd import configs
+    m, X, Y, Z, info = configs.lattice_tissue(n, 0.40, seed=seed)
+    p = configs.base_params(seed)
+    p.constrain_angles = angles
+    p.division_vol = 1.5 * info["v0"]
+    p.voxel_capacity = 64
+    rng = np.random.default_rng(seed)
+    V = [np.zeros(n * S, np.float32) for _ in range(3)]
+    FC = [np.zeros(n * S, np.float32) for _ in range(3)]
+    FD = [np.zeros(n * S, np.float32) for _ in range(3)]
+    for A in V:
+        A.reshape(n, S)[:, :180] = rng.normal(0, 0.3, (n, 180)).astype(np.float32)
+    for A in FC + FD:
+        A.reshape(n, S)[:, :180] = rng.normal(0, 40, (n, 180)).astype(np.float32)
+    P = rng.uniform(50, 60, n).astype(np.float32)
+    return p, X, Y, Z, V, FC, FD, P, n
+
+
+def _system(oracle_mesh, p, X, Y, Z, n, V=None):
+    from cellsim3d_amd import engine
+    ctx = engine.Context(oracle_mesh, capacity_cells=n + 8, voxel_budget=1 << 22)
+    s = ctx.add_system(p, X, Y, Z, n, max_cells=n, voxels=1 << 14, V=V)
+    return ctx, s
+
+
+def _check(name, a, b):
+    err = relerr(a, b)
+    print("%s: rel err %.3g, bit-exact fraction %.4f" % (name, err, float(np.mean(a == b))))
+    assert np.all(np.isfinite(a))
+    assert err <= TOL, "%s rel err %g" % (name, err)
+
+
+@pytest.mark.parametrize("angles", [0, 1])
+def test_bonded_force(oracle_mesh, angles):
+    import oracle_kernels as K
+    from cellsim3d_amd import abi
+    p, X, Y, Z, V, FC, FD, P, n = _state(angles=angles)
+    print("bonded kernel, angles=%d, %d cells" % (angles, n), flush=True)
+    ctx, s = _system(oracle_mesh, p, X, Y, Z, n)
+    s.set(abi.F_PRESSURE, P)
+    CM = [s.get(f)[:n] for f in (abi.F_CMX, abi.F_CMY, abi.F_CMZ)]
+    s.k_con_force_bonded()
+    _, F = K.con_force(n, p, oracle_mesh, X, Y, Z, CM=CM, P=P, bonded_only=True)
+    for a, f in enumerate((abi.F_FCX, abi.F_FCY, abi.F_FCZ)):
+        got = s.get(f)[:n * S].reshape(n, S)[:, :180]
+        _check("FC[%d]" % a, got, F[a].reshape(n, S)[:, :180])
+    ctx.close()
+
+
+def test_integrate_velocity_com_volumes_pressure(oracle_mesh):
+    import oracle_kernels as K
+    from cellsim3d_amd import abi
+    p, X, Y, Z, V, FC, FD, P, n = _state()
+    print("integrate/velocity/com/volumes/pressure, %d cells" % n, flush=True)
+    ctx, s = _system(oracle_mesh, p, X, Y, Z, n, V=V)
+    for f, A in zip((abi.F_FCX, abi.F_FCY, abi.F_FCZ, abi.F_FDX, abi.F_FDY, abi.F_FDZ), FC + FD):
+        s.set(f, A)
+    s.set(abi.F_PRESSURE, P)
+    s.k_integrate()
+    (X1, Y1, Z1), V1 = K.integrate(n, p, X, Y, Z, V, FC, FD)
+    for f, ref in zip((abi.F_X, abi.F_Y, abi.F_Z, abi.F_VX, abi.F_VY, abi.F_VZ), (X1, Y1, Z1) + tuple(V1)):
+        _check("integrate field %d" % f, s.get(f)[:n * S], ref)
+    s.k_velocity_ab()
+    V2 = K.velocity_ab(n, p, V1, FC, FD)
+    for f, ref in zip((abi.F_VX, abi.F_VY, abi.F_VZ), V2):
+        _check("velocity_ab field %d" % f, s.get(f)[:n * S], ref)
+    s.k_com()
+    cm = K.com(n, X1, Y1, Z1)
+    for f, ref in zip((abi.F_CMX, abi.F_CMY, abi.F_CMZ), cm):
+        _check("com field %d" % f, s.get(f)[:n], ref)
+    s.k_volumes(1)
+    faces = np.ctypeslib.as_array(oracle_mesh.faces)
+    st, vol, area, flag = K.volumes(n, p, faces, X1, Y1, Z1, CM=cm)
+    _check("volume", s.get(abi.F_VOLUME)[:n], vol)
+    _check("area", s.get(abi.F_AREA)[:n], area)
+    assert np.array_equal(s.get(abi.F_DIVFLAG)[:n], flag)
+    s.k_pressure_update(5)
+    E = np.full(n, p.stiffness1, np.float32)
+    from oracle import oracle as O
+    import ctypes as C
+    Pn = K.pressure_update(n, p, P, E, 5, np.float32(O.lib().or_rgrowth(C.byref(p), n)))
+    _check("pressure", s.get(abi.F_PRESSURE)[:n], Pn)
+    ctx.close()
