@@ -389,42 +389,43 @@ struct PairConsts {
     float ar, ar2, rr, ka, kr, gext;
 };
 
-// Contact (CON: propagate.cu:411-440) and friction (DIS: propagate.cu:633-653)
-// over the node's stored partners, in reference order.
-template <bool CON, bool DIS>
-__device__ __forceinline__ void pair_forces(const int32_t* lst, int base, int np, float X, float Y, float Z,
-                                            float vix, float viy, float viz, float nx, float ny, float nz,
-                                            const PairConsts& k, const float* __restrict__ rX,
-                                            const float* __restrict__ rY, const float* __restrict__ rZ,
-                                            const float* __restrict__ rVX, const float* __restrict__ rVY,
-                                            const float* __restrict__ rVZ, float& cfx, float& cfy, float& cfz,
-                                            float& fdx, float& fdy, float& fdz) {
-    for (int i = 0; i < np; ++i) {
-        const int key = lst[i * kSlots];
-        const int b = key / kSlots;
-        const int64_t gj = (int64_t)(base + b) * kSlots + (key - b * kSlots);
-        const float dX = X - rX[gj], dY = Y - rY[gj], dZ = Z - rZ[gj];
-        const float R2 = dX * dX + dY * dY + dZ * dZ;
-        if (CON && R2 < k.ar2) {
-            const float R = sqrtf(R2);
-            if (R < k.ar) {
-                cfx += k.ka * (k.ar - R) / R * dX;
-                cfy += k.ka * (k.ar - R) / R * dY;
-                cfz += k.ka * (k.ar - R) / R * dZ;
-            }
-            if (R <= k.rr) {
-                cfx += k.kr * (k.rr - R) / R * dX;
-                cfy += k.kr * (k.rr - R) / R * dY;
-                cfz += k.kr * (k.rr - R) / R * dZ;
-            }
+// Partner key -> global node index (keys are local cell * 192 + node).
+__device__ __forceinline__ int64_t key_node(int base, int key) {
+    const int b = key / kSlots;
+    return (int64_t)(base + b) * kSlots + (key - b * kSlots);
+}
+
+constexpr int kChunk = 4;  // partners whose loads are issued together
+
+// Friction (propagate.cu:633-653) over the node's stored partners, in stored
+// (reference) order.  The keys, then the partner velocities, of kChunk partners
+// are loaded before any of them is used.
+__device__ __forceinline__ void pair_friction(const int32_t* lst, int base, int np, float vix, float viy, float viz,
+                                              float nx, float ny, float nz, float gext,
+                                              const float* __restrict__ rVX, const float* __restrict__ rVY,
+                                              const float* __restrict__ rVZ, float& fdx, float& fdy, float& fdz) {
+    for (int i0 = 0; i0 < np; i0 += kChunk) {
+        int64_t g[kChunk];
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) g[j] = i0 + j < np ? key_node(base, lst[(i0 + j) * kSlots]) : 0;
+        float wx[kChunk], wy[kChunk], wz[kChunk];
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+            const bool in = i0 + j < np;
+            wx[j] = in ? rVX[g[j]] : 0.f;
+            wy[j] = in ? rVY[g[j]] : 0.f;
+            wz[j] = in ? rVZ[g[j]] : 0.f;
         }
-        if (DIS) {
-            const float vx = vix - rVX[gj], vy = viy - rVY[gj], vz = viz - rVZ[gj];
-            const float d = vx * nx + vy * ny + vz * nz;
-            const float tx = vx - d * nx, ty = vy - d * ny, tz = vz - d * nz;
-            fdx = fdx - k.gext * tx;
-            fdy = fdy - k.gext * ty;
-            fdz = fdz - k.gext * tz;
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+            if (i0 + j < np) {
+                const float vx = vix - wx[j], vy = viy - wy[j], vz = viz - wz[j];
+                const float d = vx * nx + vy * ny + vz * nz;
+                const float tx = vx - d * nx, ty = vy - d * ny, tz = vz - d * nz;
+                fdx = fdx - gext * tx;
+                fdy = fdy - gext * ty;
+                fdz = fdz - gext * tz;
+            }
         }
     }
 }
@@ -433,8 +434,10 @@ __device__ __forceinline__ void pair_forces(const int32_t* lst, int base, int np
 // distance (ConForce skips R^2 >= ar^2, DisForce R^2 > ar^2, propagate.cu:421,641)
 // and the candidate rule per partner cell.  Accepted keys go to rec (for the
 // second friction pass); contact (CON) and friction (DIS) are summed as in
-// propagate.cu:411-440 and 633-653.  Returns the number accepted.
-template <bool CON, bool DIS>
+// propagate.cu:411-440 and 633-653.  Candidates are taken CH at a time: keys,
+// then positions and velocities, are loaded before any is tested.
+// Returns the number accepted.
+template <bool CON, bool DIS, int CH>
 __device__ __forceinline__ int accept_forces(const int32_t* cand, int nc, int32_t* rec, int base, float X, float Y,
                                              float Z, const RuleCtx& R, const float* __restrict__ rBB, float vix,
                                              float viy, float viz, float nx, float ny, float nz, const PairConsts& k,
@@ -446,40 +449,59 @@ __device__ __forceinline__ int accept_forces(const int32_t* cand, int nc, int32_
     int np = 0;
     int lastb = -1;
     bool lastok = false;
-    for (int i = 0; i < nc; ++i) {
-        const int key = cand[i * kSlots];
-        const int b = key / kSlots;
-        const int64_t gj = (int64_t)(base + b) * kSlots + (key - b * kSlots);
-        const float dX = X - rX[gj], dY = Y - rY[gj], dZ = Z - rZ[gj];
-        const float R2 = dX * dX + dY * dY + dZ * dZ;
-        if (R2 > k.ar2) continue;
-        if (b != lastb) {
-            lastb = b;
-            lastok = rule_ok(R, rBB + (int64_t)(base + b) * 6, X, Y, Z);
-        }
-        if (!lastok) continue;
-        rec[np * kSlots] = key;
-        ++np;
-        if (CON && R2 < k.ar2) {
-            const float Rl = sqrtf(R2);
-            if (Rl < k.ar) {
-                cfx += k.ka * (k.ar - Rl) / Rl * dX;
-                cfy += k.ka * (k.ar - Rl) / Rl * dY;
-                cfz += k.ka * (k.ar - Rl) / Rl * dZ;
-            }
-            if (Rl <= k.rr) {
-                cfx += k.kr * (k.rr - Rl) / Rl * dX;
-                cfy += k.kr * (k.rr - Rl) / Rl * dY;
-                cfz += k.kr * (k.rr - Rl) / Rl * dZ;
+    for (int i0 = 0; i0 < nc; i0 += CH) {
+        int key[CH];
+#pragma unroll
+        for (int j = 0; j < CH; ++j) key[j] = i0 + j < nc ? cand[(i0 + j) * kSlots] : 0;
+        float px[CH], py[CH], pz[CH], wx[CH], wy[CH], wz[CH];
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            const bool in = i0 + j < nc;
+            const int64_t gj = key_node(base, key[j]);
+            px[j] = in ? rX[gj] : 0.f;
+            py[j] = in ? rY[gj] : 0.f;
+            pz[j] = in ? rZ[gj] : 0.f;
+            if (DIS) {
+                wx[j] = in ? rVX[gj] : 0.f;
+                wy[j] = in ? rVY[gj] : 0.f;
+                wz[j] = in ? rVZ[gj] : 0.f;
             }
         }
-        if (DIS) {
-            const float vx = vix - rVX[gj], vy = viy - rVY[gj], vz = viz - rVZ[gj];
-            const float d = vx * nx + vy * ny + vz * nz;
-            const float tx = vx - d * nx, ty = vy - d * ny, tz = vz - d * nz;
-            fdx = fdx - k.gext * tx;
-            fdy = fdy - k.gext * ty;
-            fdz = fdz - k.gext * tz;
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            if (i0 + j >= nc) continue;
+            const int b = key[j] / kSlots;
+            const float dX = X - px[j], dY = Y - py[j], dZ = Z - pz[j];
+            const float R2 = dX * dX + dY * dY + dZ * dZ;
+            if (R2 > k.ar2) continue;
+            if (b != lastb) {
+                lastb = b;
+                lastok = rule_ok(R, rBB + (int64_t)(base + b) * 6, X, Y, Z);
+            }
+            if (!lastok) continue;
+            rec[np * kSlots] = key[j];
+            ++np;
+            if (CON && R2 < k.ar2) {
+                const float Rl = sqrtf(R2);
+                if (Rl < k.ar) {
+                    cfx += k.ka * (k.ar - Rl) / Rl * dX;
+                    cfy += k.ka * (k.ar - Rl) / Rl * dY;
+                    cfz += k.ka * (k.ar - Rl) / Rl * dZ;
+                }
+                if (Rl <= k.rr) {
+                    cfx += k.kr * (k.rr - Rl) / Rl * dX;
+                    cfy += k.kr * (k.rr - Rl) / Rl * dY;
+                    cfz += k.kr * (k.rr - Rl) / Rl * dZ;
+                }
+            }
+            if (DIS) {
+                const float vx = vix - wx[j], vy = viy - wy[j], vz = viz - wz[j];
+                const float d = vx * nx + vy * ny + vz * nz;
+                const float tx = vx - d * nx, ty = vy - d * ny, tz = vz - d * nz;
+                fdx = fdx - k.gext * tx;
+                fdy = fdy - k.gext * ty;
+                fdz = fdz - k.gext * tz;
+            }
         }
     }
     return np;
@@ -641,16 +663,16 @@ __global__ __launch_bounds__(192) void k_con_dis(Pool p, Range r, int32_t mode, 
                 }
                 float nx, ny, nz;
                 node_normal(sx, sy, sz, mesh, tid, nn3[0], nn3[1], nn3[2], nx, ny, nz);
-                np = accept_forces<true, true>(cand, nc, rec, M->base, X, Y, Z, R, rBB, vix, viy, viz, nx, ny, nz, k,
+                np = accept_forces<true, true, 1>(cand, nc, rec, M->base, X, Y, Z, R, rBB, vix, viy, viz, nx, ny, nz, k,
                                                rX, rY, rZ, rVX, rVY, rVZ, cfx, cfy, cfz, fdx, fdy, fdz);
                 fdx = fdx - P->gamma_visc * vix;  // drag, propagate.cu:658
                 fdy = fdy - P->gamma_visc * viy;
                 fdz = fdz - P->gamma_visc * viz;
             } else if (forces) {
-                np = accept_forces<true, false>(cand, nc, rec, M->base, X, Y, Z, R, rBB, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, k,
+                np = accept_forces<true, false, 1>(cand, nc, rec, M->base, X, Y, Z, R, rBB, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, k,
                                                 rX, rY, rZ, rVX, rVY, rVZ, cfx, cfy, cfz, fdx, fdy, fdz);
             } else {
-                np = accept_forces<false, false>(cand, nc, rec, M->base, X, Y, Z, R, rBB, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
+                np = accept_forces<false, false, 1>(cand, nc, rec, M->base, X, Y, Z, R, rBB, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
                                                  k, rX, rY, rZ, rVX, rVY, rVZ, cfx, cfy, cfz, fdx, fdy, fdz);
             }
             p.NPAIR[gi] = np;
@@ -693,8 +715,10 @@ __global__ __launch_bounds__(192) void k_con_dis(Pool p, Range r, int32_t mode, 
 // ------------------------------------------------------------------------------
 // Bonded part of CalculateConForce alone (springs, pressure, and angles when
 // constrainAngles): propagate.cu:253-337, the same arithmetic and summation
-// order as k_con_dis, in a lean kernel for the standalone bonded entry point.
-// Without angles it streams x in and F_con out (24 B per node).
+// order as k_con_dis, in a lean streaming kernel for the standalone bonded
+// entry point.  Without angles it moves x in and F_con out (24 B per node): the
+// per-lane mesh data stay in registers across the cells a workgroup walks, and
+// the next cell's positions are loaded while the current one is computed.
 // ------------------------------------------------------------------------------
 template <bool ANGLES>
 __global__ __launch_bounds__(192) void k_bonded(Pool p, Range r) {
@@ -703,77 +727,99 @@ __global__ __launch_bounds__(192) void k_bonded(Pool p, Range r) {
     const int tid = threadIdx.x;
     const bool node = tid < kNodes;
     const MeshDev* mesh = p.mesh;
-    for (int slot = r.slot_lo + blockIdx.x; slot < r.slot_hi; slot += gridDim.x) {
-        const int s = p.slot_sys[slot];
-        if (s < 0) continue;
-        SysMeta* M = p.meta + s;
-        if (!sys_live(M)) continue;
-        if (slot - M->base >= M->n) continue;
-        const cs3d_params* P = p.prm + s;
-        if ((P->constrain_angles != 0) != ANGLES) continue;
+    int nb[3] = {0, 0, 0};
+    float r0[3] = {1.f, 1.f, 1.f}, th[3] = {0.f, 0.f, 0.f};
+    if (node) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            nb[i] = mesh->nn[i * kSlots + tid];
+            r0[i] = mesh->R0[i * kSlots + tid];
+            if (ANGLES) th[i] = mesh->theta0[3 * tid + i];
+        }
+    }
+    int slot = r.slot_lo + blockIdx.x;
+    float X = 0.f, Y = 0.f, Z = 0.f;
+    if (slot < r.slot_hi) {
         const int64_t gi = (int64_t)slot * kSlots + tid;
-        const float X = p.X[gi], Y = p.Y[gi], Z = p.Z[gi];
-        sx[tid] = X; sy[tid] = Y; sz[tid] = Z;
-        __syncthreads();
-        uint32_t st = 0;
-        if (ANGLES) {
-            if (node) {
-                const float k = P->angle_k;
-                const float* th = mesh->theta0 + 3 * tid;
-                const int n0 = mesh->nn[tid], n1 = mesh->nn[kSlots + tid], n2 = mesh->nn[2 * kSlots + tid];
+        X = p.X[gi]; Y = p.Y[gi]; Z = p.Z[gi];
+    }
+    for (; slot < r.slot_hi; slot += gridDim.x) {
+        // prefetch the next cell of this workgroup (in bounds: slot < slot_hi <= capacity)
+        const int nslot = slot + gridDim.x;
+        float Xn = 0.f, Yn = 0.f, Zn = 0.f;
+        if (nslot < r.slot_hi) {
+            const int64_t gn = (int64_t)nslot * kSlots + tid;
+            Xn = p.X[gn]; Yn = p.Y[gn]; Zn = p.Z[gn];
+        }
+        const int s = p.slot_sys[slot];
+        bool use = s >= 0;
+        SysMeta* M = use ? p.meta + s : nullptr;
+        const cs3d_params* P = use ? p.prm + s : nullptr;
+        use = use && sys_live(M) && slot - M->base < M->n && (P->constrain_angles != 0) == ANGLES;
+        if (use) {
+            const int64_t gi = (int64_t)slot * kSlots + tid;
+            sx[tid] = X; sy[tid] = Y; sz[tid] = Z;
+            __syncthreads();
+            uint32_t st = 0;
+            if (ANGLES) {
+                if (node) {
+                    const float k = P->angle_k;
 #pragma unroll 1
-                for (int q = 0; q < 3; ++q) {
-                    const int a = q == 1 ? n1 : n0;
-                    const int b = q == 0 ? n1 : n2;
-                    st |= angle_pair(sx[a] - X, sy[a] - Y, sz[a] - Z, sx[b] - X, sy[b] - Y, sz[b] - Z, th[q], k,
-                                     angF + (tid * 3 + q) * 6);
+                    for (int q = 0; q < 3; ++q) {
+                        const int a = q == 1 ? nb[1] : nb[0];
+                        const int b = q == 0 ? nb[1] : nb[2];
+                        const float t0 = q == 0 ? th[0] : (q == 1 ? th[1] : th[2]);
+                        st |= angle_pair(sx[a] - X, sy[a] - Y, sz[a] - Z, sx[b] - X, sy[b] - Y, sz[b] - Z, t0, k,
+                                         angF + (tid * 3 + q) * 6);
+                    }
                 }
+                __syncthreads();
             }
+            if (node) {
+                if (isnan(X) || isnan(Y) || isnan(Z)) st |= CS3D_SYS_NAN;
+                float FX = 0.f, FY = 0.f, FZ = 0.f;
+                const float stiffness = p.E[slot];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    const int N1 = nb[i];
+                    const float dX = sx[N1] - X, dY = sy[N1] - Y, dZ = sz[N1] - Z;
+                    const float R = sqrtf(dX * dX + dY * dY + dZ * dZ);
+                    const float R0 = r0[i];
+                    FX += stiffness * (R - R0) / R0 * dX / R;
+                    FY += stiffness * (R - R0) / R0 * dY / R;
+                    FZ += stiffness * (R - R0) / R0 * dZ / R;
+                }
+                const float rx = X - p.CMX[slot], ry = Y - p.CMY[slot], rz = Z - p.CMZ[slot];
+                const float inv = 1.0f / sqrtf(rx * rx + ry * ry + rz * rz);
+                const float g = 3.0f * p.P[slot];
+                FX += g * (inv * rx);
+                FY += g * (inv * ry);
+                FZ += g * (inv * rz);
+                if (ANGLES) {
+                    const float* a = angF + tid * 18;
+                    float ax = 0.f, ay = 0.f, az = 0.f;
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        ax = ax - (a[q * 6 + 0] + a[q * 6 + 3]);
+                        ay = ay - (a[q * 6 + 1] + a[q * 6 + 4]);
+                        az = az - (a[q * 6 + 2] + a[q * 6 + 5]);
+                    }
+                    const int32_t* gt = mesh->gather + tid * 6;
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) {
+                        const float* f = angF + gt[q];
+                        ax = ax + f[0];
+                        ay = ay + f[1];
+                        az = az + f[2];
+                    }
+                    FX += ax; FY += ay; FZ += az;
+                }
+                p.FCX[gi] = FX; p.FCY[gi] = FY; p.FCZ[gi] = FZ;
+            }
+            if (st) flag(M, st);
             __syncthreads();
         }
-        if (node) {
-            if (isnan(X) || isnan(Y) || isnan(Z)) st |= CS3D_SYS_NAN;
-            float FX = 0.f, FY = 0.f, FZ = 0.f;
-            const float stiffness = p.E[slot];
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                const int N1 = mesh->nn[i * kSlots + tid];
-                const float dX = sx[N1] - X, dY = sy[N1] - Y, dZ = sz[N1] - Z;
-                const float R = sqrtf(dX * dX + dY * dY + dZ * dZ);
-                const float R0 = mesh->R0[i * kSlots + tid];
-                FX += stiffness * (R - R0) / R0 * dX / R;
-                FY += stiffness * (R - R0) / R0 * dY / R;
-                FZ += stiffness * (R - R0) / R0 * dZ / R;
-            }
-            const float rx = X - p.CMX[slot], ry = Y - p.CMY[slot], rz = Z - p.CMZ[slot];
-            const float inv = 1.0f / sqrtf(rx * rx + ry * ry + rz * rz);
-            const float g = 3.0f * p.P[slot];
-            FX += g * (inv * rx);
-            FY += g * (inv * ry);
-            FZ += g * (inv * rz);
-            if (ANGLES) {
-                const float* a = angF + tid * 18;
-                float ax = 0.f, ay = 0.f, az = 0.f;
-#pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    ax = ax - (a[q * 6 + 0] + a[q * 6 + 3]);
-                    ay = ay - (a[q * 6 + 1] + a[q * 6 + 4]);
-                    az = az - (a[q * 6 + 2] + a[q * 6 + 5]);
-                }
-                const int32_t* gt = mesh->gather + tid * 6;
-#pragma unroll
-                for (int q = 0; q < 6; ++q) {
-                    const float* f = angF + gt[q];
-                    ax = ax + f[0];
-                    ay = ay + f[1];
-                    az = az + f[2];
-                }
-                FX += ax; FY += ay; FZ += az;
-            }
-            p.FCX[gi] = FX; p.FCY[gi] = FY; p.FCZ[gi] = FZ;
-        }
-        if (st) flag(M, st);
-        __syncthreads();
+        X = Xn; Y = Yn; Z = Zn;
     }
 }
 
@@ -836,10 +882,8 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
             k.ka = 0.f;
             k.kr = 0.f;
             k.gext = P->viscotic_damping;
-            float cfx = 0.f, cfy = 0.f, cfz = 0.f;
             const int32_t* lst = p.PAIRS + (int64_t)slot * kPMax * kSlots + tid;
-            pair_forces<false, true>(lst, M->base, np, X, Y, Z, vix, viy, viz, nx, ny, nz, k, rX, rY, rZ, rVX, rVY,
-                                     rVZ, cfx, cfy, cfz, fx, fy, fz);
+            pair_friction(lst, M->base, np, vix, viy, viz, nx, ny, nz, k.gext, rVX, rVY, rVZ, fx, fy, fz);
             fx = fx - P->gamma_visc * vix;  // drag, propagate.cu:658
             fy = fy - P->gamma_visc * viy;
             fz = fz - P->gamma_visc * viz;
