@@ -23,6 +23,7 @@ void launch_bonded(const Pool& p, const Range& r, int angles, hipStream_t st);
 void launch_divide(const Pool& p, const Range& r, int mode, hipStream_t st);
 void launch_grid(const Pool& p, const Range& r, int vox_blocks, hipStream_t st);
 void launch_vox_lists(const Pool& p, const Range& r, int vox_blocks, hipStream_t st);
+void launch_vrange(const Pool& p, const Range& r, hipStream_t st);
 void launch_velocity_ab(const Pool& p, const Range& r, hipStream_t st);
 void launch_set_flags(const Pool& p, const Range& r, int rebuild, int search, hipStream_t st);
 void launch_copy_calibrate(const void* src, void* dst, int64_t n4, hipStream_t st);
@@ -311,6 +312,7 @@ extern "C" cs3d_status cs3d_ctx_create(int device, const cs3d_mesh* mesh, int64_
     if ((st = dalloc(c, p.FLAG, capacity_cells)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.BBOX, capacity_cells * 6)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.BBOXN, capacity_cells * 6)) != CS3D_OK) return cleanup(st);
+    if ((st = dalloc(c, p.VRANGE, capacity_cells * 6)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.PAIRS, capacity_cells * kPMax * kSlots)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.NPAIR, capacity_cells * kSlots)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.CAND, capacity_cells * kPMax * kSlots)) != CS3D_OK) return cleanup(st);
@@ -590,9 +592,16 @@ extern "C" cs3d_status cs3d_set(cs3d_ctx* c, cs3d_sys s, cs3d_field f, const voi
         } else {
             std::memcpy(&m.rng, host, bytes);
         }
-        return write_meta(c, s, m);
+        cs3d_status st2 = write_meta(c, s, m);
+        if (st2 != CS3D_OK || f != CS3D_F_GRID) return st2;
+        launch_vrange(c->pool, range_of(c, s), c->stream);  // voxel ranges follow the new grid
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return CS3D_OK;
     }
     HIPCHK(hipMemcpyAsync(L.ptr, host, bytes, hipMemcpyHostToDevice, c->stream));
+    if (f == CS3D_F_BBOX) launch_vrange(c->pool, range_of(c, s), c->stream);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     if (f == CS3D_F_X || f == CS3D_F_Y || f == CS3D_F_Z || f == CS3D_F_BBOX) {  // stored candidates are stale
         SysMeta m;

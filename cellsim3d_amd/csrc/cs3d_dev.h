@@ -6,6 +6,7 @@
 //   cell arrays  float[cap_cells]        CM x/y/z, pressure, stiffness, volume, area, growth scale
 //                float[cap_cells * 6]    bbox used by the contact candidate test (rebuilt with the grid)
 //                float[cap_cells * 6]    bbox of the positions at the end of the latest step
+//                int32[cap_cells * 6]    voxel range of the contact-rule bbox (kept with it)
 //   pair lists   int32[cap_cells * kPMax * 192]  contact partners of each node, (cell, node)
 //                ascending, k-major per cell so lanes access consecutive words
 //                int32[cap_cells * 192]  partner count per node
@@ -86,6 +87,7 @@ struct Pool {
     int8_t* FLAG;
     float* BBOX;
     float* BBOXN;
+    int32_t* VRANGE;  // makeNNlist's voxel range of BBOX (sx, ex, sy, ey, sz, ez) per cell
     int32_t* PAIRS;
     int32_t* NPAIR;
     int32_t* CAND;

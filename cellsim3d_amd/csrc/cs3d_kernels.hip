@@ -102,6 +102,14 @@ __device__ __forceinline__ VoxRange vox_range(const float* b, const SysMeta* M, 
     return v;
 }
 
+// voxel range of a cell's bbox into VRANGE (lanes 0..5 of the caller store)
+__device__ __forceinline__ void store_vrange(const Pool& p, int64_t slot, const VoxRange& v, int lane) {
+    if (lane < 6) {
+        const int w[6] = {v.sx, v.ex, v.sy, v.ey, v.sz, v.ez};
+        p.VRANGE[slot * 6 + lane] = w[lane];
+    }
+}
+
 // Grid geometry from the global extent e = {minx, maxx, miny, maxy, minz, maxz}:
 // host Xdiv/Ydiv/Zdiv (GPUbounce.cu:1331-1334) with the voxel origin at the
 // extent's minimum, and the search bins (edge a little above attraction_range +
@@ -167,6 +175,7 @@ __global__ __launch_bounds__(192) void k_integrate(Pool p, Range r, int32_t flag
             for (int a = 0; a < 6; ++a) b[a] = bn[a];
             if (tid < 6) p.BBOX[(int64_t)slot * 6 + tid] = b[tid];
             const VoxRange v = vox_range(b, M, P);
+            store_vrange(p, slot, v, tid);
             const int nx = v.ex - v.sx + 1, ny = v.ey - v.sy + 1, nz = v.ez - v.sz + 1;
             const int total = (nx > 0 && ny > 0 && nz > 0) ? nx * ny * nz : 0;
             int32_t* counts = p.vox + M->vbase;
@@ -308,41 +317,39 @@ __device__ int find_candidates(const Pool& p, const SysMeta* M, const cs3d_param
 
 // The reference's candidate rule for node (X, Y, Z) in voxel (vx, vy, vz) and
 // cell b: b is in the node's voxel list -- makeNNlist puts b into exactly the
-// voxels of its bbox widened by attraction_range (propagatebound.cu:224-259) --
-// and the node lies within attraction_range of b's bbox (propagate.cu:382-392).
+// voxels of its bbox widened by attraction_range (propagatebound.cu:224-259),
+// whose range is kept per cell in VRANGE -- and the node lies within
+// attraction_range of b's bbox (propagate.cu:382-392).
 struct RuleCtx {
-    float ar, ar2, DL, gx, gy, gz;
-    int xd, yd, zd, vx, vy, vz;
+    float ar2, DL, gx, gy, gz;
+    int xd, yd, zd;
 };
 
-__device__ __forceinline__ bool rule_ok(const RuleCtx& R, const float* __restrict__ bb, float X, float Y, float Z) {
-    const float b0 = bb[0], b1 = bb[1], b2 = bb[2], b3 = bb[3], b4 = bb[4], b5 = bb[5];
-    int sx = (int)((b0 - R.ar - R.gx) / R.DL); if (sx < 0) sx = 0;
-    int ex = (int)((b1 + R.ar - R.gx) / R.DL); if (ex >= R.xd) ex = R.xd - 1;
-    int sy = (int)((b2 - R.ar - R.gy) / R.DL); if (sy < 0) sy = 0;
-    int ey = (int)((b3 + R.ar - R.gy) / R.DL); if (ey >= R.yd) ey = R.yd - 1;
-    int sz = (int)((b4 - R.ar - R.gz) / R.DL); if (sz < 0) sz = 0;
-    int ez = (int)((b5 + R.ar - R.gz) / R.DL); if (ez >= R.zd) ez = R.zd - 1;
-    const bool inlist = sx <= R.vx && R.vx <= ex && sy <= R.vy && R.vy <= ey && sz <= R.vz && R.vz <= ez;
+__device__ __forceinline__ bool rule_ok(const int32_t* __restrict__ vr, const float* __restrict__ bb, int vx, int vy,
+                                        int vz, float X, float Y, float Z, float ar2) {
+    // all twelve words are loaded before any test (8-byte aligned pairs)
+    const int2 r01 = reinterpret_cast<const int2*>(vr)[0], r23 = reinterpret_cast<const int2*>(vr)[1],
+               r45 = reinterpret_cast<const int2*>(vr)[2];
+    const float2 c01 = reinterpret_cast<const float2*>(bb)[0], c23 = reinterpret_cast<const float2*>(bb)[1],
+                 c45 = reinterpret_cast<const float2*>(bb)[2];
+    const bool inlist = (r01.x <= vx) & (vx <= r01.y) & (r23.x <= vy) & (vy <= r23.y) & (r45.x <= vz) & (vz <= r45.y);
+    const float b0 = c01.x, b1 = c01.y, b2 = c23.x, b3 = c23.y, b4 = c45.x, b5 = c45.y;
     float qx = (float)(X - b1 > 0.0f) * (X - b1);
     qx += (float)(b0 - X > 0.0f) * (b0 - X);
     float qy = (float)(Y - b3 > 0.0f) * (Y - b3);
     qy += (float)(b2 - Y > 0.0f) * (b2 - Y);
     float qz = (float)(Z - b5 > 0.0f) * (Z - b5);
     qz += (float)(b4 - Z > 0.0f) * (b4 - Z);
-    return inlist && !(qx * qx + qy * qy + qz * qz > R.ar2);
+    return inlist && !(qx * qx + qy * qy + qz * qz > ar2);
 }
 
-__device__ __forceinline__ RuleCtx rule_ctx(const SysMeta* M, const cs3d_params* P, float X, float Y, float Z) {
+__device__ __forceinline__ RuleCtx rule_ctx(const SysMeta* M, const cs3d_params* P) {
     RuleCtx R;
-    R.ar = P->attraction_range;
-    R.ar2 = R.ar * R.ar;
+    const float ar = P->attraction_range;
+    R.ar2 = ar * ar;
     R.DL = P->grid_dl;
     R.gx = M->gmin[0]; R.gy = M->gmin[1]; R.gz = M->gmin[2];
     R.xd = M->dims[0]; R.yd = M->dims[1]; R.zd = M->dims[2];
-    R.vx = vox1(X, R.gx, R.DL, R.xd);  // the node's voxel, propagate.cu:360-372
-    R.vy = vox1(Y, R.gy, R.DL, R.yd);
-    R.vz = vox1(Z, R.gz, R.DL, R.zd);
     return R;
 }
 
@@ -439,7 +446,8 @@ __device__ __forceinline__ void pair_friction(const int32_t* lst, int base, int 
 // Returns the number accepted.
 template <bool CON, bool DIS, int CH>
 __device__ __forceinline__ int accept_forces(const int32_t* cand, int nc, int32_t* rec, int base, float X, float Y,
-                                             float Z, const RuleCtx& R, const float* __restrict__ rBB, float vix,
+                                             float Z, const RuleCtx& R, const int32_t* __restrict__ vrange,
+                                             const float* __restrict__ rBB, float vix,
                                              float viy, float viz, float nx, float ny, float nz, const PairConsts& k,
                                              const float* __restrict__ rX, const float* __restrict__ rY,
                                              const float* __restrict__ rZ, const float* __restrict__ rVX,
@@ -449,6 +457,7 @@ __device__ __forceinline__ int accept_forces(const int32_t* cand, int nc, int32_
     int np = 0;
     int lastb = -1;
     bool lastok = false;
+    int vx = -1, vy = -1, vz = -1;  // the node's voxel (propagate.cu:360-372), found at first use
     for (int i0 = 0; i0 < nc; i0 += CH) {
         int key[CH];
 #pragma unroll
@@ -475,8 +484,14 @@ __device__ __forceinline__ int accept_forces(const int32_t* cand, int nc, int32_
             const float R2 = dX * dX + dY * dY + dZ * dZ;
             if (R2 > k.ar2) continue;
             if (b != lastb) {
+                if (vx < 0) {
+                    vx = vox1(X, R.gx, R.DL, R.xd);
+                    vy = vox1(Y, R.gy, R.DL, R.yd);
+                    vz = vox1(Z, R.gz, R.DL, R.zd);
+                }
                 lastb = b;
-                lastok = rule_ok(R, rBB + (int64_t)(base + b) * 6, X, Y, Z);
+                lastok = rule_ok(vrange + (int64_t)(base + b) * 6, rBB + (int64_t)(base + b) * 6, vx, vy, vz, X, Y, Z,
+                                 R.ar2);
             }
             if (!lastok) continue;
             rec[np * kSlots] = key[j];
@@ -639,7 +654,7 @@ __global__ __launch_bounds__(192) void k_con_dis(Pool p, Range r, int32_t mode, 
         if (search && node) {
             const int nc = p.NCAND[gi];
             const int32_t* cand = p.CAND + (int64_t)slot * kPMax * kSlots + tid;
-            const RuleCtx R = rule_ctx(M, P, X, Y, Z);
+            const RuleCtx R = rule_ctx(M, P);
             PairConsts k;
             k.ar = P->attraction_range;
             k.ar2 = k.ar * k.ar;
@@ -663,16 +678,16 @@ __global__ __launch_bounds__(192) void k_con_dis(Pool p, Range r, int32_t mode, 
                 }
                 float nx, ny, nz;
                 node_normal(sx, sy, sz, mesh, tid, nn3[0], nn3[1], nn3[2], nx, ny, nz);
-                np = accept_forces<true, true, 1>(cand, nc, rec, M->base, X, Y, Z, R, rBB, vix, viy, viz, nx, ny, nz, k,
+                np = accept_forces<true, true, 1>(cand, nc, rec, M->base, X, Y, Z, R, p.VRANGE, rBB, vix, viy, viz, nx, ny, nz, k,
                                                rX, rY, rZ, rVX, rVY, rVZ, cfx, cfy, cfz, fdx, fdy, fdz);
                 fdx = fdx - P->gamma_visc * vix;  // drag, propagate.cu:658
                 fdy = fdy - P->gamma_visc * viy;
                 fdz = fdz - P->gamma_visc * viz;
             } else if (forces) {
-                np = accept_forces<true, false, 1>(cand, nc, rec, M->base, X, Y, Z, R, rBB, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, k,
+                np = accept_forces<true, false, 1>(cand, nc, rec, M->base, X, Y, Z, R, p.VRANGE, rBB, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, k,
                                                 rX, rY, rZ, rVX, rVY, rVZ, cfx, cfy, cfz, fdx, fdy, fdz);
             } else {
-                np = accept_forces<false, false, 1>(cand, nc, rec, M->base, X, Y, Z, R, rBB, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
+                np = accept_forces<false, false, 1>(cand, nc, rec, M->base, X, Y, Z, R, p.VRANGE, rBB, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
                                                  k, rX, rY, rZ, rVX, rVY, rVZ, cfx, cfy, cfz, fdx, fdy, fdz);
             }
             p.NPAIR[gi] = np;
@@ -1411,6 +1426,10 @@ __global__ void k_insert(Pool p, Range r, int32_t force) {
         if (c >= M->n || (M->status & CS3D_SYS_GRID_TOO_LARGE)) continue;
         const cs3d_params* P = p.prm + s;
         const VoxRange vr = vox_range(p.BBOX + (int64_t)slot * 6, M, P);
+        {
+            const int w[6] = {vr.sx, vr.ex, vr.sy, vr.ey, vr.sz, vr.ez};
+            for (int a = 0; a < 6; ++a) p.VRANGE[(int64_t)slot * 6 + a] = w[a];
+        }
         const int xd = M->dims[0], yd = M->dims[1];
         int32_t* counts = p.vox + M->vbase;
         int32_t* lists = p.vox + M->lbase;
@@ -1460,6 +1479,22 @@ __global__ __launch_bounds__(192) void k_grid_adopt(Pool p, Range r) {
         const SysMeta* M = p.meta + s;
         if (!M->rebuild || slot - M->base >= M->n) continue;
         if (threadIdx.x < 6) p.BBOX[(int64_t)slot * 6 + threadIdx.x] = p.BBOXN[(int64_t)slot * 6 + threadIdx.x];
+        if (!(M->status & CS3D_SYS_GRID_TOO_LARGE))
+            store_vrange(p, slot, vox_range(p.BBOXN + (int64_t)slot * 6, M, p.prm + s), threadIdx.x);
+    }
+}
+
+// voxel ranges of the current bboxes (after a caller sets BBOX or the grid)
+__global__ void k_vrange(Pool p, Range r) {
+    for (int slot = r.slot_lo + blockIdx.x * blockDim.x + threadIdx.x; slot < r.slot_hi;
+         slot += gridDim.x * blockDim.x) {
+        const int s = p.slot_sys[slot];
+        if (s < 0) continue;
+        const SysMeta* M = p.meta + s;
+        if (slot - M->base >= M->n || (M->status & CS3D_SYS_GRID_TOO_LARGE)) continue;
+        const VoxRange v = vox_range(p.BBOX + (int64_t)slot * 6, M, p.prm + s);
+        const int w[6] = {v.sx, v.ex, v.sy, v.ey, v.sz, v.ez};
+        for (int a = 0; a < 6; ++a) p.VRANGE[(int64_t)slot * 6 + a] = w[a];
     }
 }
 
@@ -1562,6 +1597,10 @@ void launch_grid(const Pool& p, const Range& r, int vox_blocks, hipStream_t st) 
     hipLaunchKernelGGL(k_grid_dims, dim3(vox_blocks, ns > 0 ? ns : 1), dim3(256), 0, st, p, r);
     hipLaunchKernelGGL(k_insert, dim3(cb > 0 ? cb : 1), dim3(256), 0, st, p, r, 0);
     hipLaunchKernelGGL(k_sort, dim3(vox_blocks, ns > 0 ? ns : 1), dim3(256), 0, st, p, r, 0);
+}
+void launch_vrange(const Pool& p, const Range& r, hipStream_t st) {
+    const int n = r.slot_hi - r.slot_lo;
+    hipLaunchKernelGGL(k_vrange, dim3(n > 0 ? (n + 255) / 256 : 1), dim3(256), 0, st, p, r);
 }
 // voxel lists of the current grid (the fused step keeps only the counts)
 void launch_vox_lists(const Pool& p, const Range& r, int vox_blocks, hipStream_t st) {
