@@ -28,9 +28,10 @@ SYS_FATAL_MASK = 0x17F
 # cs3d_field
 (F_X, F_Y, F_Z, F_VX, F_VY, F_VZ, F_FCX, F_FCY, F_FCZ, F_FDX, F_FDY, F_FDZ, F_VAX, F_VAY, F_VAZ,
  F_CMX, F_CMY, F_CMZ, F_BBOX, F_PRESSURE, F_STIFFNESS, F_VOLUME, F_AREA, F_DIVFLAG, F_GRID,
- F_VOXCOUNT, F_VOXLIST, F_RANMAR, F_PAIRCOUNT, F_PAIRS, F_GROWTH_SCALE, F__COUNT) = range(32)
+ F_VOXCOUNT, F_VOXLIST, F_RANMAR, F_PAIRCOUNT, F_PAIRS, F_GROWTH_SCALE, F_FRX, F_FRY, F_FRZ, F__COUNT) = range(35)
 
-NODE_FIELDS = {F_X, F_Y, F_Z, F_VX, F_VY, F_VZ, F_FCX, F_FCY, F_FCZ, F_FDX, F_FDY, F_FDZ, F_VAX, F_VAY, F_VAZ}
+NODE_FIELDS = {F_X, F_Y, F_Z, F_VX, F_VY, F_VZ, F_FCX, F_FCY, F_FCZ, F_FDX, F_FDY, F_FDZ, F_VAX, F_VAY, F_VAZ,
+               F_FRX, F_FRY, F_FRZ}
 CELL_FIELDS = {F_CMX, F_CMY, F_CMZ, F_PRESSURE, F_STIFFNESS, F_VOLUME, F_AREA, F_GROWTH_SCALE}
 
 
@@ -86,7 +87,9 @@ class Params(C.Structure):
         ("grid_interval", C.c_int32),
         ("daughter_com_fix", C.c_int32),
         ("pair_skin", C.c_float),
-        ("reserved", C.c_int32 * 6),
+        ("rand_scale_factor", C.c_float),
+        ("noise_seed", C.c_uint32),
+        ("reserved", C.c_int32 * 4),
     ]
 
     def copy(self):
@@ -162,4 +165,6 @@ def default_params():
     p.grid_interval = 1
     p.daughter_com_fix = 0
     p.pair_skin = 0.05
+    p.rand_scale_factor = 0.001
+    p.noise_seed = 1
     return p

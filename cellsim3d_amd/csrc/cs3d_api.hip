@@ -92,6 +92,7 @@ struct cs3d_ctx {
     bool use_graph = true;
     int graph_steps = 10;
     std::map<int, std::pair<hipGraphExec_t, int>> graphs;  // target -> (exec, steps)
+    cs3d_mesh mesh_host;  // the tables the context was created with (checkpoint check)
 };
 
 namespace {
@@ -226,6 +227,9 @@ FieldLoc field_loc(cs3d_ctx* c, cs3d_sys s, cs3d_field f) {
         return L;
     }
     switch (f) {
+        case CS3D_F_FRX: L.ptr = p.FRX + nb; L.bytes = nbytes; break;
+        case CS3D_F_FRY: L.ptr = p.FRY + nb; L.bytes = nbytes; break;
+        case CS3D_F_FRZ: L.ptr = p.FRZ + nb; L.bytes = nbytes; break;
         case CS3D_F_CMX: L.ptr = p.CMX + h.base; L.bytes = h.cap * 4; break;
         case CS3D_F_CMY: L.ptr = p.CMY + h.base; L.bytes = h.cap * 4; break;
         case CS3D_F_CMZ: L.ptr = p.CMZ + h.base; L.bytes = h.cap * 4; break;
@@ -283,6 +287,7 @@ extern "C" cs3d_status cs3d_ctx_create(int device, const cs3d_mesh* mesh, int64_
         if (mesh->faces[i] < 0 || mesh->faces[i] >= kNodes) return fail(CS3D_INVALID_ARG, "mesh face index out of range");
 
     cs3d_ctx* c = new cs3d_ctx();
+    c->mesh_host = *mesh;
     c->device = device;
     c->cap_cells = capacity_cells;
     c->vox_total = voxel_budget;
@@ -303,7 +308,7 @@ extern "C" cs3d_status cs3d_ctx_create(int device, const cs3d_mesh* mesh, int64_
     const int64_t ns = capacity_cells * kSlots;
     cs3d_status st = CS3D_OK;
     float** nodes[] = {&p.X, &p.Y, &p.Z, &p.VX, &p.VY, &p.VZ, &p.VAX, &p.VAY, &p.VAZ,
-                       &p.FCX, &p.FCY, &p.FCZ, &p.FDX, &p.FDY, &p.FDZ};
+                       &p.FCX, &p.FCY, &p.FCZ, &p.FDX, &p.FDY, &p.FDZ, &p.FRX, &p.FRY, &p.FRZ};
     for (float** a : nodes)
         if ((st = dalloc(c, *a, ns)) != CS3D_OK) return cleanup(st);
     float** cells[] = {&p.CMX, &p.CMY, &p.CMZ, &p.P, &p.E, &p.VOL, &p.AREA, &p.GS};
@@ -402,7 +407,8 @@ extern "C" cs3d_status cs3d_system_add_raw(cs3d_ctx* c, const cs3d_params* prm, 
     HIPCHK(hipSetDevice(c->device));
     const Pool& p = c->pool;
     const int64_t nb = base * kSlots, nbytes = (int64_t)max_cells * kSlots * 4;
-    float* node[] = {p.X, p.Y, p.Z, p.VX, p.VY, p.VZ, p.VAX, p.VAY, p.VAZ, p.FCX, p.FCY, p.FCZ, p.FDX, p.FDY, p.FDZ};
+    float* node[] = {p.X,   p.Y,   p.Z,   p.VX,  p.VY,  p.VZ,  p.VAX, p.VAY, p.VAZ,
+                     p.FCX, p.FCY, p.FCZ, p.FDX, p.FDY, p.FDZ, p.FRX, p.FRY, p.FRZ};
     for (float* a : node) HIPCHK(hipMemsetAsync(a + nb, 0, nbytes, c->stream));
     float* cellz[] = {p.CMX, p.CMY, p.CMZ, p.VOL, p.AREA};
     for (float* a : cellz) HIPCHK(hipMemsetAsync(a + base, 0, (size_t)max_cells * 4, c->stream));
@@ -610,6 +616,176 @@ extern "C" cs3d_status cs3d_set(cs3d_ctx* c, cs3d_sys s, cs3d_field f, const voi
         m.need_search = 1;
         return write_meta(c, s, m);
     }
+    return CS3D_OK;
+}
+
+// ---- checkpoint / restart -------------------------------------------------------
+// File layout (little-endian): "CS3DCKP1", u32 version, u32 sizeof(cs3d_params),
+// cs3d_params, i32 n, i32 saved_cap, i64 step, i64 horizon, u32 status,
+// cs3d_ranmar_state, grid geometry {f32 gmin[3], i32 dims[3], f32 bmin[3], f32 bh,
+// i32 bdims[3]}, u32 mesh checksum, then for cells 0..n-1: 15 node arrays
+// (x, y, z, v, F_con, F_dis, F_ran; 192 floats per cell each), 6 cell arrays (CM, P,
+// stiffness, growth scale) and the contact-rule bbox and its voxel range (6 each).
+namespace {
+const char kCkptMagic[8] = {'C', 'S', '3', 'D', 'C', 'K', 'P', '1'};
+constexpr uint32_t kCkptVersion = 2;
+
+uint32_t mesh_checksum(const cs3d_ctx* c) {
+    // FNV-1a over the mesh tables the context was built with
+    uint32_t h = 2166136261u;
+    const unsigned char* b = reinterpret_cast<const unsigned char*>(&c->mesh_host);
+    for (size_t i = 0; i < sizeof(cs3d_mesh); ++i) h = (h ^ b[i]) * 16777619u;
+    return h;
+}
+
+struct CkptGrid {
+    float gmin[3];
+    int32_t dims[3];
+    float bmin[3];
+    float bh;
+    int32_t bdims[3];
+};
+
+template <class T> bool put(FILE* f, const T& v) { return std::fwrite(&v, sizeof(T), 1, f) == 1; }
+template <class T> bool get(FILE* f, T& v) { return std::fread(&v, sizeof(T), 1, f) == 1; }
+}  // namespace
+
+extern "C" cs3d_status cs3d_system_save(cs3d_ctx* c, cs3d_sys s, const char* path) {
+    if (!valid_sys(c, s) || !path) return fail(CS3D_INVALID_ARG, "bad save arguments");
+    const Pool& p = c->pool;
+    const Range r = range_of(c, s);
+    // a grid rebuilt at the end of the last step is completed first (its bboxes
+    // become current), as a read of the grid fields does
+    launch_vox_lists(p, r, vox_blocks_of(c, s), c->stream);
+    launch_set_flags(p, r, 0, -1, c->stream);
+    HIPCHK(hipGetLastError());
+    SysMeta m;
+    cs3d_status st = read_meta(c, s, m);
+    if (st != CS3D_OK) return st;
+    const SysHost& h = c->sys[s];
+    const int64_t n = m.n, base = h.base;
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(CS3D_INVALID_ARG, std::string("cannot open ") + path);
+    bool ok = std::fwrite(kCkptMagic, 1, 8, f) == 8 && put(f, kCkptVersion) && put(f, (uint32_t)sizeof(cs3d_params)) &&
+              put(f, h.prm) && put(f, (int32_t)n) && put(f, (int32_t)h.cap) && put(f, m.step) && put(f, m.horizon) &&
+              put(f, m.status) && put(f, m.rng);
+    CkptGrid g;
+    std::memcpy(g.gmin, m.gmin, sizeof(g.gmin));
+    std::memcpy(g.dims, m.dims, sizeof(g.dims));
+    std::memcpy(g.bmin, m.bmin, sizeof(g.bmin));
+    g.bh = m.bh;
+    std::memcpy(g.bdims, m.bdims, sizeof(g.bdims));
+    ok = ok && put(f, g) && put(f, mesh_checksum(c));
+    const float* node[] = {p.X,   p.Y,   p.Z,   p.VX,  p.VY,  p.VZ,  p.FCX, p.FCY, p.FCZ,
+                           p.FDX, p.FDY, p.FDZ, p.FRX, p.FRY, p.FRZ};
+    std::vector<float> buf((size_t)std::max<int64_t>(n, 1) * kSlots);
+    for (const float* a : node) {
+        if (!ok) break;
+        if (hipMemcpy(buf.data(), a + base * kSlots, (size_t)n * kSlots * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+            std::fclose(f);
+            return fail(CS3D_HIP_ERROR, "checkpoint read");
+        }
+        ok = std::fwrite(buf.data(), 4, (size_t)n * kSlots, f) == (size_t)n * kSlots;
+    }
+    const float* cell[] = {p.CMX, p.CMY, p.CMZ, p.P, p.E, p.GS};
+    for (const float* a : cell) {
+        if (!ok) break;
+        if (hipMemcpy(buf.data(), a + base, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+            std::fclose(f);
+            return fail(CS3D_HIP_ERROR, "checkpoint read");
+        }
+        ok = std::fwrite(buf.data(), 4, (size_t)n, f) == (size_t)n;
+    }
+    if (ok) {
+        std::vector<float> bb((size_t)std::max<int64_t>(n, 1) * 6);
+        std::vector<int32_t> vr((size_t)std::max<int64_t>(n, 1) * 6);
+        ok = hipMemcpy(bb.data(), p.BBOX + base * 6, (size_t)n * 24, hipMemcpyDeviceToHost) == hipSuccess &&
+             hipMemcpy(vr.data(), p.VRANGE + base * 6, (size_t)n * 24, hipMemcpyDeviceToHost) == hipSuccess &&
+             std::fwrite(bb.data(), 4, (size_t)n * 6, f) == (size_t)n * 6 &&
+             std::fwrite(vr.data(), 4, (size_t)n * 6, f) == (size_t)n * 6;
+    }
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? CS3D_OK : fail(CS3D_INVALID_ARG, std::string("cannot write ") + path);
+}
+
+extern "C" cs3d_status cs3d_system_load(cs3d_ctx* c, const char* path, int32_t max_cells, int64_t voxels,
+                                        cs3d_sys* out) {
+    if (!c || !path || !out) return fail(CS3D_INVALID_ARG, "bad load arguments");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(CS3D_INVALID_ARG, std::string("cannot open ") + path);
+    char magic[8];
+    uint32_t ver = 0, psize = 0, mcheck = 0;
+    cs3d_params prm;
+    int32_t n = 0, scap = 0;
+    int64_t step = 0, horizon = -1;
+    uint32_t status = 0;
+    cs3d_ranmar_state rng;
+    CkptGrid g;
+    bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kCkptMagic, 8) == 0 && get(f, ver) &&
+              ver == kCkptVersion && get(f, psize) && psize == sizeof(cs3d_params) && get(f, prm) && get(f, n) &&
+              get(f, scap) && get(f, step) && get(f, horizon) && get(f, status) && get(f, rng) && get(f, g) &&
+              get(f, mcheck);
+    if (!ok) {
+        std::fclose(f);
+        return fail(CS3D_INVALID_ARG, std::string("not a checkpoint of this engine: ") + path);
+    }
+    if (mcheck != mesh_checksum(c)) {
+        std::fclose(f);
+        return fail(CS3D_INVALID_ARG, "checkpoint was written with a different mesh");
+    }
+    if (max_cells <= 0) max_cells = scap;
+    cs3d_sys s = -1;
+    cs3d_status st = cs3d_system_add_raw(c, &prm, n, max_cells, voxels, &s);
+    if (st != CS3D_OK) {
+        std::fclose(f);
+        return st;
+    }
+    const Pool& p = c->pool;
+    const int64_t base = c->sys[s].base;
+    std::vector<float> buf((size_t)std::max(n, 1) * kSlots);
+    float* node[] = {p.X, p.Y, p.Z, p.VX, p.VY, p.VZ, p.FCX, p.FCY, p.FCZ, p.FDX, p.FDY, p.FDZ, p.FRX, p.FRY, p.FRZ};
+    for (float* a : node) {
+        if (!ok) break;
+        ok = std::fread(buf.data(), 4, (size_t)n * kSlots, f) == (size_t)n * kSlots &&
+             hipMemcpy(a + base * kSlots, buf.data(), (size_t)n * kSlots * 4, hipMemcpyHostToDevice) == hipSuccess;
+    }
+    float* cell[] = {p.CMX, p.CMY, p.CMZ, p.P, p.E, p.GS};
+    for (float* a : cell) {
+        if (!ok) break;
+        ok = std::fread(buf.data(), 4, (size_t)n, f) == (size_t)n &&
+             hipMemcpy(a + base, buf.data(), (size_t)n * 4, hipMemcpyHostToDevice) == hipSuccess;
+    }
+    if (ok) {
+        std::vector<float> bb((size_t)std::max(n, 1) * 6);
+        std::vector<int32_t> vr((size_t)std::max(n, 1) * 6);
+        ok = std::fread(bb.data(), 4, (size_t)n * 6, f) == (size_t)n * 6 &&
+             std::fread(vr.data(), 4, (size_t)n * 6, f) == (size_t)n * 6 &&
+             hipMemcpy(p.BBOX + base * 6, bb.data(), (size_t)n * 24, hipMemcpyHostToDevice) == hipSuccess &&
+             hipMemcpy(p.VRANGE + base * 6, vr.data(), (size_t)n * 24, hipMemcpyHostToDevice) == hipSuccess;
+    }
+    std::fclose(f);
+    SysMeta m;
+    if (ok) ok = read_meta(c, s, m) == CS3D_OK;
+    if (ok) {
+        m.step = step;
+        m.horizon = horizon;
+        m.status = status;
+        m.rng = rng;
+        std::memcpy(m.gmin, g.gmin, sizeof(g.gmin));
+        std::memcpy(m.dims, g.dims, sizeof(g.dims));
+        std::memcpy(m.bmin, g.bmin, sizeof(g.bmin));
+        m.bh = g.bh;
+        std::memcpy(m.bdims, g.bdims, sizeof(g.bdims));
+        m.rebuild = 0;
+        m.need_search = 1;  // candidate lists are rebuilt at the next step
+        m.search_next = 0;
+        ok = (int64_t)m.dims[0] * m.dims[1] * m.dims[2] <= m.vbudget && write_meta(c, s, m) == CS3D_OK;
+    }
+    if (!ok) {
+        cs3d_system_remove(c, s);
+        return fail(CS3D_INVALID_ARG, std::string("truncated or inconsistent checkpoint: ") + path);
+    }
+    *out = s;
     return CS3D_OK;
 }
 

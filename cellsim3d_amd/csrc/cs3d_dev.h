@@ -1,7 +1,7 @@
 // cs3d_dev.h - data layout shared by the HIP kernels and the host side of libcs3d.
 //
 // HBM layout (one context = one pool on one GPU):
-//   node arrays  float[cap_cells * 192]  x, y, z, v, v_after_integrate, F_con, F_dis
+//   node arrays  float[cap_cells * 192]  x, y, z, v, v_after_integrate, F_con, F_dis, F_ran
 //                (SoA, 192-slot stride per cell as the reference's d_X, GPUbounce.cu:365-373)
 //   cell arrays  float[cap_cells]        CM x/y/z, pressure, stiffness, volume, area, growth scale
 //                float[cap_cells * 6]    bbox used by the contact candidate test (rebuilt with the grid)
@@ -82,6 +82,7 @@ struct Pool {
     float *VAX, *VAY, *VAZ;  // velocity after Integrate (read by the first DisForce)
     float *FCX, *FCY, *FCZ;
     float *FDX, *FDY, *FDZ;
+    float *FRX, *FRY, *FRZ;  // random force drawn in the latest step (d_fRanList)
     float *CMX, *CMY, *CMZ;
     float *P, *E, *VOL, *AREA, *GS;
     int8_t* FLAG;

@@ -79,12 +79,14 @@ extern "C" cs3d_status cs3d_params_default(cs3d_params* p) {
     p->voxel_capacity = 32;
     p->grid_interval = 1;
     p->pair_skin = 0.05f;
+    p->rand_scale_factor = 0.001f;
+    p->noise_seed = 1;
     return CS3D_OK;
 }
 
 extern "C" cs3d_status cs3d_params_check(const cs3d_params* p) {
     if (!p) return fail(CS3D_INVALID_ARG, "params is NULL");
-    if (p->add_rands) return fail(CS3D_UNSUPPORTED, "add_rands = 1 needs the reference's cuRAND stream, which is not reproduced");
+    if (p->add_rands && !(p->rand_scale_factor >= 0)) return fail(CS3D_INVALID_ARG, "rand_scale_factor must be >= 0");
     if (!(p->mass > 0)) return fail(CS3D_INVALID_ARG, "particle_mass must be > 0");
     if (!(p->dt > 0)) return fail(CS3D_INVALID_ARG, "time_interval must be > 0");
     if (!(p->grid_dl > 0)) return fail(CS3D_INVALID_ARG, "grid_dl must be > 0");

@@ -79,9 +79,71 @@ __device__ __forceinline__ float o2f(int i) { return __int_as_float(i >= 0 ? i :
 
 // Integrate's velocity half-kick of one component (propagate.cu:698-705): the
 // sum in float, the rest in double (0.5 is a double literal), rounded to float.
-__device__ __forceinline__ float half_kick(float v, float fc, float fd, float dt, float rdt, float m) {
-    const float fr = 0.0f;  // random force disabled (add_rands = 0)
+// fr is the random force drawn in the previous step (0 without add_rands).
+__device__ __forceinline__ float half_kick(float v, float fc, float fd, float fr, float dt, float rdt, float m) {
     return (float)((double)v + 0.5 * (double)(dt * fc + dt * fd + rdt * fr) / (double)m);
+}
+
+// ---- random force (CalculateRanForce, propagate.cu:667-676) --------------------
+// The reference draws curand_normal from per-node cuRAND states (closed library,
+// not reproduced).  Here: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel
+// random numbers: as easy as 1, 2, 3", SC'11) with counter {cell * 192 + node,
+// step (low, high), attempt} and key {noise_seed, 0x43533344}, turned into
+// standard normals by the Marsaglia polar method.  Every operation is integer or
+// correctly rounded float (the logarithm is a fixed polynomial), so the host and
+// the device produce the same bits.
+__device__ __forceinline__ void philox4x32_10(uint32_t* c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c[0]), lo0 = 0xD2511F53u * c[0];
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]), lo1 = 0xCD9E8D57u * c[2];
+        const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// natural logarithm of s in (0, 1): s = m 2^e with m in [sqrt(1/2), sqrt(2)),
+// ln m = 2 atanh((m - 1) / (m + 1)) by its series to z^9, e ln 2 in two parts
+__device__ __forceinline__ float ln01(float s) {
+    const uint32_t b = __float_as_uint(s);
+    int e = (int)((b >> 23) & 0xffu) - 126;
+    float m = __uint_as_float((b & 0x7fffffu) | 0x3f000000u);
+    if (m < 0.70710678f) {
+        m = m + m;
+        e -= 1;
+    }
+    const float z = (m - 1.0f) / (m + 1.0f);
+    const float z2 = z * z;
+    float q = 0.11111111f;
+    q = q * z2 + 0.14285715f;
+    q = q * z2 + 0.2f;
+    q = q * z2 + 0.33333334f;
+    q = q * z2 + 1.0f;
+    const float fe = (float)e;
+    return fe * 0.693145751953125f + (2.0f * z * q + fe * 1.4286068e-06f);
+}
+
+// three standard normals of node `node` of a tissue at step `step`
+__device__ __forceinline__ void noise3(uint32_t seed, uint32_t node, int64_t step, float* z) {
+    int got = 0;
+    z[0] = z[1] = z[2] = 0.f;
+    for (uint32_t attempt = 0; got < 3 && attempt < 32; ++attempt) {
+        uint32_t c[4] = {node, (uint32_t)step, (uint32_t)((uint64_t)step >> 32), attempt};
+        philox4x32_10(c, seed, 0x43533344u);
+#pragma unroll
+        for (int pr = 0; pr < 2; ++pr) {
+            const float v1 = 2.0f * (((float)(c[2 * pr] >> 9) + 0.5f) * 1.1920929e-07f) - 1.0f;
+            const float v2 = 2.0f * (((float)(c[2 * pr + 1] >> 9) + 0.5f) * 1.1920929e-07f) - 1.0f;
+            const float s = v1 * v1 + v2 * v2;
+            if (got < 3 && s > 0.0f && s < 1.0f) {
+                const float f = sqrtf(-2.0f * ln01(s) / s);
+                z[got++] = v1 * f;
+                if (got < 3) z[got++] = v2 * f;
+            }
+        }
+    }
 }
 
 // makeNNlist's voxel range of a cell (propagatebound.cu:224-245): the voxels of
@@ -193,9 +255,10 @@ __global__ __launch_bounds__(192) void k_integrate(Pool p, Range r, int32_t flag
             const float dt = P->dt, m = P->mass;
             const float rdt = sqrtf(dt);
             float vx = p.VX[gi], vy = p.VY[gi], vz = p.VZ[gi];
-            vx = half_kick(vx, p.FCX[gi], p.FDX[gi], dt, rdt, m);
-            vy = half_kick(vy, p.FCY[gi], p.FDY[gi], dt, rdt, m);
-            vz = half_kick(vz, p.FCZ[gi], p.FDZ[gi], dt, rdt, m);
+            const bool rnd = P->add_rands != 0;  // the previous step's random force (d_fRanList)
+            vx = half_kick(vx, p.FCX[gi], p.FDX[gi], rnd ? p.FRX[gi] : 0.f, dt, rdt, m);
+            vy = half_kick(vy, p.FCY[gi], p.FDY[gi], rnd ? p.FRY[gi] : 0.f, dt, rdt, m);
+            vz = half_kick(vz, p.FCZ[gi], p.FDZ[gi], rnd ? p.FRZ[gi] : 0.f, dt, rdt, m);
             x = x + vx * dt;
             y = y + vy * dt;
             z = z + vz * dt;
@@ -709,12 +772,19 @@ __global__ __launch_bounds__(192) void k_con_dis(Pool p, Range r, int32_t mode, 
             if (fused) {
                 const float dt = P->dt, m = P->mass;
                 const float rdt = sqrtf(dt);
-                const float fr = 0.0f;
+                float frx = 0.f, fry = 0.f, frz = 0.f;
+                if (P->add_rands) {  // CalculateRanForce for this step (propagate.cu:667-676)
+                    float zr[3];
+                    noise3(P->noise_seed, (uint32_t)(c * kSlots + tid), M->step + 1, zr);
+                    const float sc = P->rand_scale_factor;
+                    frx = sc * zr[0]; fry = sc * zr[1]; frz = sc * zr[2];
+                    p.FRX[gi] = frx; p.FRY[gi] = fry; p.FRZ[gi] = frz;
+                }
                 const float vix = svx[tid], viy = svy[tid], viz = svz[tid];
                 // VelocityUpdateA (propagate.cu:721-723) then VelocityUpdateB (:732-734)
-                float vx = (float)((double)vix + 0.5 * (double)(dt * FX + rdt * fr) / (double)m);
-                float vy = (float)((double)viy + 0.5 * (double)(dt * FY + rdt * fr) / (double)m);
-                float vz = (float)((double)viz + 0.5 * (double)(dt * FZ + rdt * fr) / (double)m);
+                float vx = (float)((double)vix + 0.5 * (double)(dt * FX + rdt * frx) / (double)m);
+                float vy = (float)((double)viy + 0.5 * (double)(dt * FY + rdt * fry) / (double)m);
+                float vz = (float)((double)viz + 0.5 * (double)(dt * FZ + rdt * frz) / (double)m);
                 vx = (float)((double)vx + 0.5 * (double)dt * (double)fdx / (double)m);
                 vy = (float)((double)vy + 0.5 * (double)dt * (double)fdy / (double)m);
                 vz = (float)((double)vz + 0.5 * (double)dt * (double)fdz / (double)m);
@@ -909,9 +979,10 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
                 // values now in V, F_con and F_dis (bit-identical to k_integrate): search
                 // again next step when that is pair_skin/2 or more from the last search
                 const float dt = P->dt, m = P->mass, rdt = sqrtf(dt);
-                const float qx = X + half_kick(vix, p.FCX[gi], fx, dt, rdt, m) * dt - p.XBX[gi];
-                const float qy = Y + half_kick(viy, p.FCY[gi], fy, dt, rdt, m) * dt - p.XBY[gi];
-                const float qz = Z + half_kick(viz, p.FCZ[gi], fz, dt, rdt, m) * dt - p.XBZ[gi];
+                const bool rnd = P->add_rands != 0;
+                const float qx = X + half_kick(vix, p.FCX[gi], fx, rnd ? p.FRX[gi] : 0.f, dt, rdt, m) * dt - p.XBX[gi];
+                const float qy = Y + half_kick(viy, p.FCY[gi], fy, rnd ? p.FRY[gi] : 0.f, dt, rdt, m) * dt - p.XBY[gi];
+                const float qz = Z + half_kick(viz, p.FCZ[gi], fz, rnd ? p.FRZ[gi] : 0.f, dt, rdt, m) * dt - p.XBZ[gi];
                 const float h = 0.5f * P->pair_skin;
                 far = !(qx * qx + qy * qy + qz * qz <= h * h);
             }
@@ -1519,12 +1590,15 @@ __global__ __launch_bounds__(192) void k_velocity_ab(Pool p, Range r) {
         if (slot - M->base >= M->n || tid >= kNodes) continue;
         const cs3d_params* P = p.prm + s;
         const int64_t gi = (int64_t)slot * kSlots + tid;
-        const float dt = P->dt, m = P->mass, rdt = sqrtf(dt), fr = 0.0f;
+        const float dt = P->dt, m = P->mass, rdt = sqrtf(dt);
         float v[3] = {p.VX[gi], p.VY[gi], p.VZ[gi]};
         const float fc[3] = {p.FCX[gi], p.FCY[gi], p.FCZ[gi]};
         const float fd[3] = {p.FDX[gi], p.FDY[gi], p.FDZ[gi]};
+        const bool rnd = P->add_rands != 0;  // the stored random force (d_fRanList)
+        const float fr[3] = {rnd ? p.FRX[gi] : 0.f, rnd ? p.FRY[gi] : 0.f, rnd ? p.FRZ[gi] : 0.f};
 #pragma unroll
-        for (int a = 0; a < 3; ++a) v[a] = (float)((double)v[a] + 0.5 * (double)(dt * fc[a] + rdt * fr) / (double)m);
+        for (int a = 0; a < 3; ++a)
+            v[a] = (float)((double)v[a] + 0.5 * (double)(dt * fc[a] + rdt * fr[a]) / (double)m);
 #pragma unroll
         for (int a = 0; a < 3; ++a) v[a] = (float)((double)v[a] + 0.5 * (double)dt * (double)fd[a] / (double)m);
         p.VX[gi] = v[0]; p.VY[gi] = v[1]; p.VZ[gi] = v[2];

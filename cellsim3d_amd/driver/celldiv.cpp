@@ -8,7 +8,9 @@
 // C180_pentahexa in the working directory or --mesh-dir) and outputs: the
 // trajectory (binary or ASCII, core.trajFileName), the forces CSV
 // (core.forces_file when core.write_cont_force), and the empty psfil and
-// velocity2.xyz the reference creates.  All simulation work runs on the GPU
+// velocity2.xyz the reference creates.  The random force (add_rands) draws from a
+// counter-based Philox stream instead of the reference's cuRAND states.  All
+// simulation work runs on the GPU
 // through the C ABI of libcs3d (include/cs3d.h); this file only parses,
 // places, and writes.
 //
@@ -199,6 +201,12 @@ int read_params(const char* path, Settings& s) {
     }
     p.add_rands = rnd["add_rands"].as_bool();
     s.rand_seed = rnd["rand_seed"].as_int();
+    p.rand_scale_factor = rnd["rand_scale_factor"].as_float();
+    if (rnd["rand_dist"].as_int() != 0) {
+        std::printf("ERROR: rand_dist %d is not implemented (the reference implements 0 only)\n",
+                    rnd["rand_dist"].as_int());
+        return -1;
+    }
 
     if (s.rad_frac < 0.4f || s.rad_frac > 0.8f) {
         std::printf("radFrac not in [0.4, 0.8] setting to 1.\n");
@@ -246,8 +254,7 @@ int read_params(const char* path, Settings& s) {
 // refused with a message rather than silently ignored.
 int refuse_unsupported(const Settings& s) {
     const char* why = nullptr;
-    if (s.p.add_rands) why = "rand_params.add_rands = 1 (the reference's cuRAND stream is not reproduced)";
-    else if (s.do_adaptive_dt) why = "core.doAdaptive_dt = 1";
+    if (s.do_adaptive_dt) why = "core.doAdaptive_dt = 1";
     else if (s.correct_com) why = "core.correct_com = 1";
     else if (s.p.use_different_stiffnesses) why = "stiffnessParams.useDifferentStiffnesses = 1 (softer-cell selection)";
     else if (s.use_pbcs) why = "boxParams.usePBCs = 1 (unavailable in the reference as well)";
@@ -535,6 +542,12 @@ int main(int argc, char** argv) {
     }
     s.p.ranmar_ij = ij;
     s.p.ranmar_kl = kl;
+    // random force: the reference seeds cuRAND MT19937 with rand_seed or the time
+    // (GPUbounce.cu:714-728); this engine keys its counter-based stream the same way
+    s.p.noise_seed = s.rand_seed > 0 ? (uint32_t)s.rand_seed : (uint32_t)std::time(nullptr);
+    if (s.p.add_rands)
+        std::printf("random force on: Philox4x32-10 normals (not the reference's cuRAND stream), scale %g\n",
+                    s.p.rand_scale_factor);
     cs3d_ranmar_state rng;
     if (cs3d_ranmar_init(&rng, ij, kl) != CS3D_OK) {
         std::printf("RANMAR seeds out of range: %d, %d\n", ij, kl);

@@ -45,6 +45,8 @@ SIGNATURES = {
     "cs3d_system_set_horizon": ([vp, C.c_int32, C.c_int64], st_t),
     "cs3d_system_info": ([vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_uint32)], st_t),
     "cs3d_system_set_counts": ([vp, C.c_int32, C.c_int32, C.c_int64], st_t),
+    "cs3d_system_save": ([vp, C.c_int32, C.c_char_p], st_t),
+    "cs3d_system_load": ([vp, C.c_char_p, C.c_int32, C.c_int64, C.POINTER(C.c_int32)], st_t),
     "cs3d_get": ([vp, C.c_int32, C.c_int, vp, C.c_size_t], st_t),
     "cs3d_set": ([vp, C.c_int32, C.c_int, vp, C.c_size_t], st_t),
     "cs3d_field_bytes": ([vp, C.c_int32, C.c_int], C.c_size_t),
@@ -169,6 +171,16 @@ class Context:
         self.systems[s.id] = s
         return s
 
+    def load_system(self, path, max_cells=0, voxels=0):
+        """A system restored from a checkpoint written by System.save."""
+        out = C.c_int32()
+        check(lib().cs3d_system_load(self.h, str(path).encode(), int(max_cells), int(voxels), C.byref(out)),
+              "cs3d_system_load")
+        s = System(self, out.value, None, 0)
+        s.max_cells = lib().cs3d_field_bytes(self.h, out.value, abi.F_PRESSURE) // 4
+        self.systems[s.id] = s
+        return s
+
     def step(self, n_steps, target=abi.ALL):
         sid = target.id if isinstance(target, System) else target
         check(lib().cs3d_step(self.h, sid, int(n_steps)), "cs3d_step")
@@ -215,6 +227,10 @@ class System:
         self.id = sid
         self.params = params
         self.max_cells = max_cells
+
+    def save(self, path):
+        """Checkpoint of the complete state (continues bit-identically after load)."""
+        check(lib().cs3d_system_save(self.ctx.h, self.id, str(path).encode()), "cs3d_system_save")
 
     def remove(self):
         check(lib().cs3d_system_remove(self.ctx.h, self.id), "cs3d_system_remove")

@@ -105,7 +105,7 @@ typedef struct cs3d_params {
     int32_t ranmar_ij, ranmar_kl;  /* RANMAR seeds (GPUbounce.cu:1777-1810: ij = rand_seed, kl = 100*rand_seed) */
     int32_t do_pop_model;          /* population.doPopModel */
     float max_pop;                 /* population.max_pop */
-    int32_t add_rands;             /* rand_params.add_rands: must be 0 (cuRAND is not reproduced) */
+    int32_t add_rands;             /* rand_params.add_rands: random force (propagate.cu:667-676, 681-725) */
     float angle_k;                 /* bending constant, 1000 in the reference (propagate.cu:335) */
     float grid_dl;                 /* voxel edge DL, 2.9 in the reference (GPUbounce.cu:756) */
     int32_t voxel_capacity;        /* cells per voxel list, 32 in the reference (propagatebound.cu:259) */
@@ -115,7 +115,12 @@ typedef struct cs3d_params {
     float pair_skin;               /* engine: contact candidates are searched within attraction_range +
                                       pair_skin and re-searched once a node has moved pair_skin/2; every step
                                       applies the exact reference tests, so results do not depend on it */
-    int32_t reserved[6];
+    float rand_scale_factor;       /* rand_params.rand_scale_factor: scale of the normal random force */
+    uint32_t noise_seed;           /* key of the random-force stream.  The reference draws from cuRAND
+                                      (closed, not reproduced); this engine draws standard normals with the
+                                      Marsaglia polar method from Philox4x32-10 (Salmon et al., SC'11),
+                                      counter (cell * 192 + node, step), reproducible per tissue, node and step */
+    int32_t reserved[4];
 } cs3d_params;
 
 /* Fields readable/writable with cs3d_get / cs3d_set.  Sizes are per system
@@ -141,6 +146,7 @@ typedef enum cs3d_field {
     CS3D_F_PAIRS,                              /* int32 40 x 192 per cell: partner keys cell*192 + node,
                                                   ascending, k-major (engine-specific) */
     CS3D_F_GROWTH_SCALE,                       /* per-cell multiplier of the pressure increment (default 1) */
+    CS3D_F_FRX, CS3D_F_FRY, CS3D_F_FRZ,        /* random force of the latest step, d_fRanList (add_rands) */
     CS3D_F__COUNT
 } cs3d_field;
 
@@ -205,6 +211,14 @@ cs3d_status cs3d_system_set_horizon(cs3d_ctx* ctx, cs3d_sys sys, int64_t horizon
 cs3d_status cs3d_system_info(cs3d_ctx* ctx, cs3d_sys sys, int32_t* n_cells, int64_t* step, uint32_t* status);
 /* Sets the live cell count and step counter (state injection). */
 cs3d_status cs3d_system_set_counts(cs3d_ctx* ctx, cs3d_sys sys, int32_t n_cells, int64_t step);
+/* Checkpoint / restart of one system's complete state (positions, velocities,
+ * forces, per-cell state, stale centres of mass, the contact-rule grid,
+ * RANMAR stream, counters).  A system loaded into a context with the same mesh
+ * continues bit-identically to the saved one.  The reference reads a Restart
+ * key but never implements it (GPUbounce.cu:1963); this has no counterpart.
+ * load: max_cells <= 0 keeps the saved capacity; voxels as cs3d_system_add. */
+cs3d_status cs3d_system_save(cs3d_ctx* ctx, cs3d_sys sys, const char* path);
+cs3d_status cs3d_system_load(cs3d_ctx* ctx, const char* path, int32_t max_cells, int64_t voxels, cs3d_sys* out);
 cs3d_status cs3d_get(cs3d_ctx* ctx, cs3d_sys sys, cs3d_field f, void* host, size_t bytes);
 cs3d_status cs3d_set(cs3d_ctx* ctx, cs3d_sys sys, cs3d_field f, const void* host, size_t bytes);
 size_t cs3d_field_bytes(cs3d_ctx* ctx, cs3d_sys sys, cs3d_field f);

@@ -299,13 +299,71 @@ uint32_t dis_force(int n, const cs3d_params* p, const int32_t* nnT, const int32_
     return status;
 }
 
+// ---- random force -----------------------------------------------------------
+// The reference's CalculateRanForce (propagate.cu:667-676) draws curand_normal
+// from cuRAND states, a closed library that is not reproduced.  The engine's
+// documented replacement: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) with
+// counter {cell * 192 + node, step low, step high, attempt} and key {seed,
+// 0x43533344}, standard normals by the Marsaglia polar method, the logarithm
+// as ln m = 2 atanh((m-1)/(m+1)) to z^9 after splitting off the binary exponent.
+inline uint32_t mulhi32(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) >> 32); }
+
+void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t m0 = 0xD2511F53u, m1 = 0xCD9E8D57u;
+        const uint32_t h0 = mulhi32(m0, c[0]), l0 = m0 * c[0];
+        const uint32_t h1 = mulhi32(m1, c[2]), l1 = m1 * c[2];
+        const uint32_t x0 = h1 ^ c[1] ^ k0, x2 = h0 ^ c[3] ^ k1;
+        c[0] = x0; c[1] = l1; c[2] = x2; c[3] = l0;
+        k0 += 0x9E3779B9u;  // Weyl constants of Philox
+        k1 += 0xBB67AE85u;
+    }
+}
+
+float ln_unit(float s) {
+    uint32_t b;
+    std::memcpy(&b, &s, 4);
+    int e = (int)((b >> 23) & 0xffu) - 126;
+    const uint32_t mb = (b & 0x7fffffu) | 0x3f000000u;
+    float m;
+    std::memcpy(&m, &mb, 4);
+    if (m < 0.70710678f) { m = m + m; e -= 1; }
+    const float z = (m - 1.0f) / (m + 1.0f), z2 = z * z;
+    float q = 0.11111111f;
+    q = q * z2 + 0.14285715f;
+    q = q * z2 + 0.2f;
+    q = q * z2 + 0.33333334f;
+    q = q * z2 + 1.0f;
+    const float fe = (float)e;
+    return fe * 0.693145751953125f + (2.0f * z * q + fe * 1.4286068e-06f);
+}
+
+void normals3(uint32_t seed, uint32_t node, int64_t step, float z[3]) {
+    int got = 0;
+    z[0] = z[1] = z[2] = 0.f;
+    for (uint32_t attempt = 0; got < 3 && attempt < 32; ++attempt) {
+        uint32_t c[4] = {node, (uint32_t)step, (uint32_t)((uint64_t)step >> 32), attempt};
+        philox10(c, seed, 0x43533344u);
+        for (int pr = 0; pr < 2; ++pr) {
+            const float v1 = 2.0f * (((float)(c[2 * pr] >> 9) + 0.5f) * 1.1920929e-07f) - 1.0f;
+            const float v2 = 2.0f * (((float)(c[2 * pr + 1] >> 9) + 0.5f) * 1.1920929e-07f) - 1.0f;
+            const float s = v1 * v1 + v2 * v2;
+            if (got < 3 && s > 0.0f && s < 1.0f) {
+                const float f = std::sqrt(-2.0f * ln_unit(s) / s);
+                z[got++] = v1 * f;
+                if (got < 3) z[got++] = v2 * f;
+            }
+        }
+    }
+}
+
 // ---- Integrate + ForwardTime, propagate.cu:681-712, 738-763 -----------------
 template <class T>
 void integrate(int n, const cs3d_params* p, T* X, T* Y, T* Z, T* VX, T* VY, T* VZ, const T* FCX, const T* FCY,
-               const T* FCZ, const T* FDX, const T* FDY, const T* FDZ) {
+               const T* FCZ, const T* FDX, const T* FDY, const T* FDZ, const T* FRX = nullptr,
+               const T* FRY = nullptr, const T* FRZ = nullptr) {
     const T dt = (T)p->dt, m = (T)p->mass;
     const T root_dt = std::sqrt(dt);
-    const T fr = 0;  // random force disabled (add_rands = 0)
 #pragma omp parallel for schedule(static)
     for (int c = 0; c < n; ++c) {
         for (int i = 0; i < NN; ++i) {
@@ -313,7 +371,9 @@ void integrate(int n, const cs3d_params* p, T* X, T* Y, T* Z, T* VX, T* VY, T* V
             T* V[3] = {VX + gi, VY + gi, VZ + gi};
             T* P[3] = {X + gi, Y + gi, Z + gi};
             const T FC[3] = {FCX[gi], FCY[gi], FCZ[gi]}, FD[3] = {FDX[gi], FDY[gi], FDZ[gi]};
+            const T FR[3] = {FRX ? FRX[gi] : T(0), FRY ? FRY[gi] : T(0), FRZ ? FRZ[gi] : T(0)};
             for (int a = 0; a < 3; ++a) {
+                const T fr = FR[a];
                 if (is_f32<T>())
                     *V[a] = (T)((double)*V[a] + 0.5 * (double)(dt * FC[a] + dt * FD[a] + root_dt * fr) / (double)m);
                 else
@@ -327,17 +387,19 @@ void integrate(int n, const cs3d_params* p, T* X, T* Y, T* Z, T* VX, T* VY, T* V
 // ---- VelocityUpdateA then VelocityUpdateB, propagate.cu:714-736 --------------
 template <class T>
 void velocity_ab(int n, const cs3d_params* p, T* VX, T* VY, T* VZ, const T* FCX, const T* FCY, const T* FCZ,
-                 const T* FDX, const T* FDY, const T* FDZ) {
+                 const T* FDX, const T* FDY, const T* FDZ, const T* FRX = nullptr, const T* FRY = nullptr,
+                 const T* FRZ = nullptr) {
     const T dt = (T)p->dt, m = (T)p->mass;
     const T root_dt = std::sqrt(dt);
-    const T fr = 0;
 #pragma omp parallel for schedule(static)
     for (int c = 0; c < n; ++c) {
         for (int i = 0; i < NN; ++i) {
             int64_t gi = (int64_t)c * S + i;
             T* V[3] = {VX + gi, VY + gi, VZ + gi};
             const T FC[3] = {FCX[gi], FCY[gi], FCZ[gi]}, FD[3] = {FDX[gi], FDY[gi], FDZ[gi]};
+            const T FR[3] = {FRX ? FRX[gi] : T(0), FRY ? FRY[gi] : T(0), FRZ ? FRZ[gi] : T(0)};
             for (int a = 0; a < 3; ++a) {
+                const T fr = FR[a];
                 if (is_f32<T>())
                     *V[a] = (T)((double)*V[a] + 0.5 * (double)(dt * FC[a] + root_dt * fr) / (double)m);
                 else
@@ -713,6 +775,9 @@ extern "C" void or_velocity_ab(int n, const cs3d_params* p, float* VX, float* VY
     velocity_ab<float>(n, p, VX, VY, VZ, FCX, FCY, FCZ, FDX, FDY, FDZ);
 }
 extern "C" float or_rgrowth(const cs3d_params* p, int n_cells) { return rgrowth<float>(p, n_cells); }
+extern "C" void or_philox(uint32_t* c, uint32_t k0, uint32_t k1) { philox10(c, k0, k1); }
+extern "C" float or_ln_unit(float s) { return ln_unit(s); }
+extern "C" void or_normals3(uint32_t seed, uint32_t node, int64_t step, float* z) { normals3(seed, node, step, z); }
 extern "C" void or_pressure_update(int n, const cs3d_params* p, float* P, const float* stiffness,
                                    const float* scale, int step, float rg) {
     pressure_update<float>(n, p, P, stiffness, scale, step, rg);
@@ -736,7 +801,7 @@ template <class T> struct Sim : SimBase {
     cs3d_params p;
     cs3d_mesh m;
     std::vector<T> R0, th0;
-    std::vector<T> X, Y, Z, VX, VY, VZ, FCX, FCY, FCZ, FDX, FDY, FDZ, VAX, VAY, VAZ;
+    std::vector<T> X, Y, Z, VX, VY, VZ, FCX, FCY, FCZ, FDX, FDY, FDZ, VAX, VAY, VAZ, FRX, FRY, FRZ;
     std::vector<T> CMx, CMy, CMz, bb, P, E, vol, area, gscale, patch;
     std::vector<int8_t> flag;
     int32_t budget = 0;
@@ -750,7 +815,8 @@ template <class T> struct Sim : SimBase {
         const float* z) : p(pp), m(mm) {
         n = n0; cap = cap0; budget = bud;
         size_t ns = (size_t)cap * S;
-        for (auto* v : {&X, &Y, &Z, &VX, &VY, &VZ, &FCX, &FCY, &FCZ, &FDX, &FDY, &FDZ, &VAX, &VAY, &VAZ})
+        for (auto* v : {&X, &Y, &Z, &VX, &VY, &VZ, &FCX, &FCY, &FCZ, &FDX, &FDY, &FDZ, &VAX, &VAY, &VAZ, &FRX, &FRY,
+                        &FRZ})
             v->assign(ns, T(0));
         for (auto* v : {&CMx, &CMy, &CMz, &P, &E, &vol, &area}) v->assign(cap, T(0));
         gscale.assign(cap, T(1));
@@ -853,15 +919,29 @@ template <class T> struct Sim : SimBase {
         uint32_t st = 0;
         for (int s = 0; s < k; ++s) {
             ++stepno;
+            const bool rnd = p.add_rands != 0;  // Integrate uses the previous step's draw (d_fRanList)
             integrate<T>(n, &p, X.data(), Y.data(), Z.data(), VX.data(), VY.data(), VZ.data(), FCX.data(), FCY.data(),
-                         FCZ.data(), FDX.data(), FDY.data(), FDZ.data());
+                         FCZ.data(), FDX.data(), FDY.data(), FDZ.data(), rnd ? FRX.data() : nullptr,
+                         rnd ? FRY.data() : nullptr, rnd ? FRZ.data() : nullptr);
             VAX = VX; VAY = VY; VAZ = VZ;
             T rg = rgrowth<T>(&p, n);
             pressure_update<T>(n, &p, P.data(), E.data(), gscale.data(), stepno, rg);
             st |= conforce();
             st |= disforce();
+            if (rnd) {  // CalculateRanForce, GPUbounce.cu:1133-1137
+                for (int c = 0; c < n; ++c)
+                    for (int i = 0; i < NN; ++i) {
+                        float zr[3];
+                        normals3(p.noise_seed, (uint32_t)(c * S + i), stepno, zr);
+                        const int64_t gi = (int64_t)c * S + i;
+                        FRX[gi] = (T)(p.rand_scale_factor * zr[0]);
+                        FRY[gi] = (T)(p.rand_scale_factor * zr[1]);
+                        FRZ[gi] = (T)(p.rand_scale_factor * zr[2]);
+                    }
+            }
             velocity_ab<T>(n, &p, VX.data(), VY.data(), VZ.data(), FCX.data(), FCY.data(), FCZ.data(), FDX.data(),
-                           FDY.data(), FDZ.data());
+                           FDY.data(), FDZ.data(), rnd ? FRX.data() : nullptr, rnd ? FRY.data() : nullptr,
+                           rnd ? FRZ.data() : nullptr);
             st |= disforce();
             com<T>(n, X.data(), Y.data(), Z.data(), CMx.data(), CMy.data(), CMz.data());
             bool divided = false;
@@ -888,6 +968,7 @@ template <class T> struct Sim : SimBase {
             case CS3D_F_BBOX: return &bb; case CS3D_F_PRESSURE: return &P; case CS3D_F_STIFFNESS: return &E;
             case CS3D_F_VOLUME: return &vol; case CS3D_F_AREA: return &area;
             case CS3D_F_GROWTH_SCALE: return &gscale;
+            case CS3D_F_FRX: return &FRX; case CS3D_F_FRY: return &FRY; case CS3D_F_FRZ: return &FRZ;
             default: return nullptr;
         }
     }

@@ -75,6 +75,10 @@ def lib():
         L.or_sim_get_f64.argtypes = [vp, C.c_int, vp, C.c_size_t]
         L.or_sim_get_f64.restype = C.c_size_t
         L.or_sim_set_counts.argtypes = [vp, C.c_int, C.c_int64]
+        L.or_philox.argtypes = [np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"), C.c_uint32, C.c_uint32]
+        L.or_ln_unit.argtypes = [C.c_float]
+        L.or_ln_unit.restype = C.c_float
+        L.or_normals3.argtypes = [C.c_uint32, C.c_uint32, C.c_int64, f32p]
         L.or_set_threads.argtypes = [C.c_int]
         L.or_get_threads.restype = C.c_int
         _lib = L
@@ -204,6 +208,19 @@ class Sim:
             return
         a = np.ascontiguousarray(arr)
         lib().or_sim_set(self.h, f, a.ctypes.data_as(vp), a.nbytes)
+
+
+def philox(ctr, key):
+    """Philox4x32-10 of a 4-word counter and 2-word key."""
+    c = np.array(ctr, dtype=np.uint32)
+    lib().or_philox(c, int(key[0]), int(key[1]))
+    return c
+
+
+def normals3(seed, node, step):
+    z = np.zeros(3, dtype=np.float32)
+    lib().or_normals3(int(seed), int(node), int(step), z)
+    return z
 
 
 def set_threads(n):

@@ -156,7 +156,8 @@ def test_random_placement_follows_the_ranmar_stream(tmp_path):
     (lambda d: d["boxParams"].update(box_len_x=1.5, box_len_y=1.5, box_len_z=1.5), 27),  # floor(V/0.786) < n
     (lambda d: d["stiffnessParams"].__setitem__("closenessToCenter", 2.0), 255),
     (lambda d: d["walls"].__setitem__("useWalls", 1), 255),            # walls and rigid box together
-    (lambda d: d["rand_params"].__setitem__("add_rands", 1), 255),     # cuRAND stream not reproduced
+    (lambda d: d["rand_params"].__setitem__("rand_dist", 1), 255),     # only the uniform default exists
+    (lambda d: d["core"].__setitem__("doAdaptive_dt", 1), 255),        # not provided
     (lambda d: d.pop("divParams"), 255),
     (lambda d: d["core"].update(dt_tol=1.0), 196),                     # exit(-60)
 ])

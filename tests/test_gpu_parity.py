@@ -220,3 +220,39 @@ def test_graph_equals_eager(oracle_mesh):
             assert np.array_equal(s1.get(f), s2.get(f))
     c1.close()
     c2.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_random_force_matches_oracle(oracle_mesh, graph):
+    """add_rands=1: the counter-based normals (Philox4x32-10 + polar method) are
+    drawn per (cell, node, step) on the device and in the oracle; the random
+    force enters Integrate with the previous step's draw and VelocityUpdateA
+    with this step's, so positions and velocities must agree as without noise,
+    and the FR fields themselves must be bit-identical."""
+    A = _abi()
+    p, X, Y, Z, n, mc = _case("tissue", n=27, seed=5)
+    p.add_rands = 1
+    p.rand_scale_factor = np.float32(0.05)
+    p.noise_seed = 11
+    sim, ctx, s = _pair(oracle_mesh, p, X, Y, Z, n, mc)
+    ctx.set_graph(graph)
+    for k in (1, 19):
+        sim.step(k)
+        s.step(k)
+        ctx.sync()
+        n1, st1, status = s.info()
+        assert n1 == sim.n and st1 == sim.stepno
+        assert status & A.SYS_FATAL_MASK == 0
+        for f in (A.F_FRX, A.F_FRY, A.F_FRZ):
+            a, b = s.get(f)[:n * 192], sim.get(f)[:n * 192]
+            assert np.array_equal(a, b), "random force field %d differs at step %d" % (f, st1)
+        _cmp(sim, s, [A.F_X, A.F_Y, A.F_Z, A.F_VX, A.F_VY, A.F_VZ, A.F_PRESSURE], n,
+             label="add_rands steps=%d" % st1)
+    # the noise moved the system: a run without it differs
+    p0 = _case("tissue", n=27, seed=5)[0]
+    sim0, ctx0, s0 = _pair(oracle_mesh, p0, X, Y, Z, n, mc)
+    s0.step(20)
+    ctx0.sync()
+    assert not np.array_equal(s0.get(A.F_X)[:n * 192], s.get(A.F_X)[:n * 192])
+    ctx0.close()
+    ctx.close()

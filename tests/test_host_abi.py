@@ -76,10 +76,12 @@ def test_params_check_rejects(field, value):
     assert engine.lib().cs3d_last_error()
 
 
-def test_params_check_rejects_random_forcing():
+def test_params_check_random_forcing():
     p = engine.default_params()
     p.add_rands = 1
-    assert engine.lib().cs3d_params_check(C.byref(p)) == abi.UNSUPPORTED
+    assert engine.lib().cs3d_params_check(C.byref(p)) == abi.OK
+    p.rand_scale_factor = -1.0
+    assert engine.lib().cs3d_params_check(C.byref(p)) == abi.INVALID_ARG
 
 
 def test_mesh_build_matches_oracle_rest_state(gp30):

@@ -654,3 +654,35 @@ def test_division_half_spaces_and_bookkeeping(gp30, oracle_mesh):
         assert np.sum(np.abs((xp - cm - h * nrm) @ nrm) < 1e-5) > 10
         # daughter CM stays stale (0) until the next CenterOfMass (Appendix hazard A5)
         assert CMx[d] == 0 and CMy[d] == 0 and CMz[d] == 0
+
+
+# ------------------------------------------------------------------ random force
+def test_philox_known_answers():
+    """Philox4x32-10 known-answer vectors published with Random123 (Salmon et al.,
+    SC'11): counter, key -> output."""
+    kat = [
+        ([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+        ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+        ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+         [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+    ]
+    for ctr, key, out in kat:
+        assert [int(v) for v in O.philox(ctr, key)] == out
+
+
+def test_log_of_the_polar_method():
+    import ctypes as C
+    s = np.concatenate([np.linspace(1e-6, 1 - 1e-6, 20001), 2.0 ** -np.arange(1, 48)]).astype(np.float32)
+    got = np.array([O.lib().or_ln_unit(C.c_float(v)) for v in s], dtype=np.float64)
+    ref = np.log(s.astype(np.float64))
+    assert np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-30)) < 4e-7
+
+
+def test_normals_are_standard_normal_and_reproducible():
+    from scipy import stats
+    z = np.array([O.normals3(7, node, step) for node in range(3000) for step in (1, 2)]).ravel()
+    assert abs(z.mean()) < 0.03 and abs(z.std() - 1.0) < 0.03
+    assert stats.kstest(z, "norm").pvalue > 1e-3
+    assert np.array_equal(O.normals3(7, 5, 11), O.normals3(7, 5, 11))
+    assert not np.array_equal(O.normals3(7, 5, 11), O.normals3(7, 5, 12))
+    assert not np.array_equal(O.normals3(7, 5, 11), O.normals3(8, 5, 11))
