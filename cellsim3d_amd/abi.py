@@ -7,6 +7,8 @@ import ctypes as C
 
 NODES = 180
 SLOTS = 192
+BIN_CAP = 32  # CS3D_BIN_CAP
+BIN_WORDS = 1 + 4 * BIN_CAP  # int32 words per search bin
 FACES = 92
 ALL = -1
 
@@ -23,12 +25,13 @@ SYS_VOLUME_NONFINITE = 1 << 5
 SYS_BAD_NORMAL = 1 << 6
 SYS_NN_OVER_10 = 1 << 7
 SYS_CAND_OVERFLOW = 1 << 8
-SYS_FATAL_MASK = 0x17F
+SYS_BIN_OVERFLOW = 1 << 9
+SYS_FATAL_MASK = 0x37F
 
 # cs3d_field
 (F_X, F_Y, F_Z, F_VX, F_VY, F_VZ, F_FCX, F_FCY, F_FCZ, F_FDX, F_FDY, F_FDZ, F_VAX, F_VAY, F_VAZ,
  F_CMX, F_CMY, F_CMZ, F_BBOX, F_PRESSURE, F_STIFFNESS, F_VOLUME, F_AREA, F_DIVFLAG, F_GRID,
- F_VOXCOUNT, F_VOXLIST, F_RANMAR, F_PAIRCOUNT, F_PAIRS, F_GROWTH_SCALE, F_FRX, F_FRY, F_FRZ, F__COUNT) = range(35)
+ F_VOXCOUNT, F_VOXLIST, F_RANMAR, F_PAIRCOUNT, F_PAIRS, F_GROWTH_SCALE, F_FRX, F_FRY, F_FRZ, F_SYSCM, F__COUNT) = range(36)
 
 NODE_FIELDS = {F_X, F_Y, F_Z, F_VX, F_VY, F_VZ, F_FCX, F_FCY, F_FCZ, F_FDX, F_FDY, F_FDZ, F_VAX, F_VAY, F_VAZ,
                F_FRX, F_FRY, F_FRZ}
@@ -89,7 +92,8 @@ class Params(C.Structure):
         ("pair_skin", C.c_float),
         ("rand_scale_factor", C.c_float),
         ("noise_seed", C.c_uint32),
-        ("reserved", C.c_int32 * 4),
+        ("correct_com", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
     def copy(self):

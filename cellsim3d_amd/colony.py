@@ -54,10 +54,11 @@ def main(argv=None):
     m = cfg["mesh"]
     p = cfg["params"]
     p.grid_interval = args.grid_interval
-    p.voxel_capacity = 64
+    # a growing colony compresses its core: up to ~100 cells' boxes meet in one voxel
+    p.voxel_capacity = 128
     gm = engine.build_mesh(meshmod.tables(m), configs.rest_positions(m))
-    ctx = engine.Context(gm, capacity_cells=cap, voxel_budget=(32 * cap + 4096) * 65,
-                         bin_budget=(640 * cap + 16384) * 65)
+    ctx = engine.Context(gm, capacity_cells=cap, voxel_budget=(32 * cap + 4096) * (p.voxel_capacity + 1),
+                         bin_budget=(640 * cap + 16384) * abi.BIN_WORDS)
     s = ctx.add_system(p, cfg["X"], cfg["Y"], cfg["Z"], 1, max_cells=cap, voxels=32 * cap + 4096)
     s.set(abi.F_GROWTH_SCALE, cfg["growth_scale"])
     samples = []
@@ -72,7 +73,11 @@ def main(argv=None):
         dt = time.perf_counter() - t0
         n_new, step, status = s.info()
         if status & abi.SYS_FATAL_MASK:
-            raise SystemExit("colony stopped: status 0x%x at step %d with %d cells" % (status, step, n_new))
+            vol = s.get(abi.F_VOLUME)[:n_new]
+            pc = s.get(abi.F_PAIRCOUNT).reshape(-1, 192)[:n_new, :180]
+            raise SystemExit("colony stopped: status 0x%x at step %d with %d cells; cell volume / V0 in [%.3f, %.3f], "
+                             "at most %d contact partners per node" % (status, step, n_new, vol.min() / cfg["v0"],
+                                                                       vol.max() / cfg["v0"], int(pc.max())))
         node_updates += 0.5 * (n + n_new) * 180.0 * args.chunk
         samples.append({"step": int(step), "cells": int(n_new), "ms_per_step": dt / args.chunk * 1e3,
                         "node_updates_per_s": 0.5 * (n + n_new) * 180.0 * args.chunk / dt})

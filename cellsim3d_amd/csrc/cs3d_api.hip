@@ -21,6 +21,7 @@ void launch_dis(const Pool& p, const Range& r, int mode, int64_t step_override, 
                 hipStream_t st);
 void launch_bonded(const Pool& p, const Range& r, int angles, hipStream_t st);
 void launch_divide(const Pool& p, const Range& r, int mode, hipStream_t st);
+void launch_com_shift(const Pool& p, const Range& r, hipStream_t st);
 void launch_grid(const Pool& p, const Range& r, int vox_blocks, hipStream_t st);
 void launch_vox_lists(const Pool& p, const Range& r, int vox_blocks, hipStream_t st);
 void launch_vrange(const Pool& p, const Range& r, hipStream_t st);
@@ -162,6 +163,12 @@ cs3d_status write_meta(cs3d_ctx* c, cs3d_sys s, const SysMeta& m) {
     return CS3D_OK;
 }
 
+bool any_correct_com(const cs3d_ctx* c, const Range& r) {
+    for (int s = r.sys_lo; s < r.sys_hi; ++s)
+        if (c->sys[s].used && c->sys[s].prm.correct_com) return true;
+    return false;
+}
+
 // One step of the loop in GPUbounce.cu:1055-1506 as five launches:
 //   integrate  Integrate + ForwardTime + PressureUpdate; per-cell part of a grid
 //              rebuilt at the end of the last step; search bins (search steps)
@@ -171,6 +178,7 @@ cs3d_status write_meta(cs3d_ctx* c, cs3d_sys s, const SysMeta& m) {
 //   dis_post   CalculateDisForce (end velocities) + CenterOfMass [+ volumes] +
 //              bounding_boxes + next-step search decision
 //   divide     division phase, step bookkeeping, extent and grid geometry
+//              [+ CorrectCoMMotion, a sixth launch, for systems with correct_com]
 void enqueue_step_phase(cs3d_ctx* c, const Range& r, int phase) {
     const Pool& p = c->pool;
     switch (phase) {
@@ -180,7 +188,10 @@ void enqueue_step_phase(cs3d_ctx* c, const Range& r, int phase) {
         case CS3D_PH_SEARCH: launch_search(p, r, 0, c->stream); break;
         case CS3D_PH_CON_DIS: launch_con(p, r, M_STEP, true, c->stream); break;
         case CS3D_PH_DIS_POST: launch_dis(p, r, M_STEP, -1, 1, false, c->stream); break;
-        case CS3D_PH_DIVIDE: launch_divide(p, r, M_STEP, c->stream); break;
+        case CS3D_PH_DIVIDE:
+            launch_divide(p, r, M_STEP, c->stream);
+            if (any_correct_com(c, r)) launch_com_shift(p, r, c->stream);
+            break;
         default: break;
     }
 }
@@ -246,6 +257,7 @@ FieldLoc field_loc(cs3d_ctx* c, cs3d_sys s, cs3d_field f) {
         case CS3D_F_GROWTH_SCALE: L.ptr = p.GS + h.base; L.bytes = h.cap * 4; break;
         case CS3D_F_GRID: L.bytes = 32; break;
         case CS3D_F_RANMAR: L.bytes = sizeof(cs3d_ranmar_state); break;
+        case CS3D_F_SYSCM: L.bytes = 12; break;
         default: break;
     }
     return L;
@@ -442,7 +454,7 @@ extern "C" cs3d_status cs3d_system_add_raw(cs3d_ctx* c, const cs3d_params* prm, 
     m.bbase = breg;
     m.ebase = (breg + bb + 3) / 4 * 4;
     m.bh = prm->attraction_range > 0 ? prm->attraction_range * 1.001f : 1.0f;
-    m.bdims[0] = m.bdims[1] = m.bdims[2] = 1;
+    m.bdims[0] = m.bdims[1] = m.bdims[2] = 3;  // at least 3 per axis: the 27 periodic neighbours are distinct
     m.need_search = 1;
     for (int a = 0; a < 3; ++a) {  // minmaxpre sentinels (propagatebound.cu:161-166)
         float lo = 1.0E8f, hi = -1.0E8f;
@@ -550,7 +562,7 @@ extern "C" cs3d_status cs3d_get(cs3d_ctx* c, cs3d_sys s, cs3d_field f, void* hos
     FieldLoc L = field_loc(c, s, f);
     if (L.bytes == 0) return fail(CS3D_INVALID_ARG, "unknown field");
     if (bytes > L.bytes) return fail(CS3D_INVALID_ARG, "requested more bytes than the field holds");
-    if (f == CS3D_F_GRID || f == CS3D_F_RANMAR) {
+    if (f == CS3D_F_GRID || f == CS3D_F_RANMAR || f == CS3D_F_SYSCM) {
         SysMeta m;
         cs3d_status st = read_meta(c, s, m);
         if (st != CS3D_OK) return st;
@@ -559,8 +571,10 @@ extern "C" cs3d_status cs3d_get(cs3d_ctx* c, cs3d_sys s, cs3d_field f, void* hos
             std::memcpy(g, m.gmin, 12);
             g[3] = m.dims[0]; g[4] = m.dims[1]; g[5] = m.dims[2]; g[6] = m.vbudget; g[7] = m.vcap;
             std::memcpy(host, g, bytes);
-        } else {
+        } else if (f == CS3D_F_RANMAR) {
             std::memcpy(host, &m.rng, bytes);
+        } else {
+            std::memcpy(host, m.sys_cm, bytes);
         }
         return CS3D_OK;
     }
@@ -583,6 +597,7 @@ extern "C" cs3d_status cs3d_set(cs3d_ctx* c, cs3d_sys s, cs3d_field f, const voi
     FieldLoc L = field_loc(c, s, f);
     if (L.bytes == 0) return fail(CS3D_INVALID_ARG, "unknown field");
     if (bytes > L.bytes) return fail(CS3D_INVALID_ARG, "more bytes than the field holds");
+    if (f == CS3D_F_SYSCM) return fail(CS3D_INVALID_ARG, "the system centre is read-only");
     if (f == CS3D_F_GRID || f == CS3D_F_RANMAR) {
         SysMeta m;
         cs3d_status st = read_meta(c, s, m);
@@ -623,12 +638,12 @@ extern "C" cs3d_status cs3d_set(cs3d_ctx* c, cs3d_sys s, cs3d_field f, const voi
 // File layout (little-endian): "CS3DCKP1", u32 version, u32 sizeof(cs3d_params),
 // cs3d_params, i32 n, i32 saved_cap, i64 step, i64 horizon, u32 status,
 // cs3d_ranmar_state, grid geometry {f32 gmin[3], i32 dims[3], f32 bmin[3], f32 bh,
-// i32 bdims[3]}, u32 mesh checksum, then for cells 0..n-1: 15 node arrays
+// i32 bdims[3]}, f32 system centre[3], u32 mesh checksum, then for cells 0..n-1: 15 node arrays
 // (x, y, z, v, F_con, F_dis, F_ran; 192 floats per cell each), 6 cell arrays (CM, P,
 // stiffness, growth scale) and the contact-rule bbox and its voxel range (6 each).
 namespace {
 const char kCkptMagic[8] = {'C', 'S', '3', 'D', 'C', 'K', 'P', '1'};
-constexpr uint32_t kCkptVersion = 2;
+constexpr uint32_t kCkptVersion = 3;
 
 uint32_t mesh_checksum(const cs3d_ctx* c) {
     // FNV-1a over the mesh tables the context was built with
@@ -675,7 +690,7 @@ extern "C" cs3d_status cs3d_system_save(cs3d_ctx* c, cs3d_sys s, const char* pat
     std::memcpy(g.bmin, m.bmin, sizeof(g.bmin));
     g.bh = m.bh;
     std::memcpy(g.bdims, m.bdims, sizeof(g.bdims));
-    ok = ok && put(f, g) && put(f, mesh_checksum(c));
+    ok = ok && put(f, g) && put(f, m.sys_cm) && put(f, mesh_checksum(c));
     const float* node[] = {p.X,   p.Y,   p.Z,   p.VX,  p.VY,  p.VZ,  p.FCX, p.FCY, p.FCZ,
                            p.FDX, p.FDY, p.FDZ, p.FRX, p.FRY, p.FRZ};
     std::vector<float> buf((size_t)std::max<int64_t>(n, 1) * kSlots);
@@ -721,10 +736,11 @@ extern "C" cs3d_status cs3d_system_load(cs3d_ctx* c, const char* path, int32_t m
     uint32_t status = 0;
     cs3d_ranmar_state rng;
     CkptGrid g;
+    float sys_cm[3];
     bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kCkptMagic, 8) == 0 && get(f, ver) &&
               ver == kCkptVersion && get(f, psize) && psize == sizeof(cs3d_params) && get(f, prm) && get(f, n) &&
               get(f, scap) && get(f, step) && get(f, horizon) && get(f, status) && get(f, rng) && get(f, g) &&
-              get(f, mcheck);
+              get(f, sys_cm) && get(f, mcheck);
     if (!ok) {
         std::fclose(f);
         return fail(CS3D_INVALID_ARG, std::string("not a checkpoint of this engine: ") + path);
@@ -776,6 +792,8 @@ extern "C" cs3d_status cs3d_system_load(cs3d_ctx* c, const char* path, int32_t m
         std::memcpy(m.bmin, g.bmin, sizeof(g.bmin));
         m.bh = g.bh;
         std::memcpy(m.bdims, g.bdims, sizeof(g.bdims));
+        std::memcpy(m.sys_cm, sys_cm, sizeof(sys_cm));
+        m.shift_pending = 0;
         m.rebuild = 0;
         m.need_search = 1;  // candidate lists are rebuilt at the next step
         m.search_next = 0;

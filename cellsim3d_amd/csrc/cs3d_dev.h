@@ -30,7 +30,7 @@ namespace cs3d {
 constexpr int kSlots = CS3D_SLOTS;   // 192
 constexpr int kNodes = CS3D_NODES;   // 180
 constexpr int kPMax = 40;            // contact candidates / partners stored per node
-constexpr int kBCap = 16;            // nodes per search bin
+constexpr int kBCap = CS3D_BIN_CAP;  // nodes per search bin
 constexpr int kMaxSys = 4096;        // systems per context
 
 // Per-system state kept on the device.
@@ -61,6 +61,9 @@ struct SysMeta {
     int32_t bdims[3];   // bins per axis
     int32_t need_search;  // 1 = search contact candidates this step (a node moved pair_skin/2)
     int32_t search_next;  // set during a step when the next step's drift takes a node pair_skin/2 away
+    float sys_cm[3];      // system centre subtracted at the end of the latest step (correct_com)
+    int32_t shift_pending;  // 1 = the step that just ended computed sys_cm and its nodes are still to be
+                            // shifted (set by the division phase, consumed by the shift kernel)
     cs3d_ranmar_state rng;
 };
 

@@ -137,6 +137,27 @@ extern "C" cs3d_status cs3d_mesh_build(const int32_t* nn, const int32_t* sign, c
 }
 
 // rmarin, marsaglia.h:54-113
+// The device CenterOfMass tree: slots t and t+128 (t < 128, slots >= 180 count as
+// zero), then t and t+64, then pairs at distance 32, 16, ..., 1; one division by 180.
+extern "C" cs3d_status cs3d_cell_centres(const float* x, const float* y, const float* z, int32_t n_cells,
+                                         float* cmx, float* cmy, float* cmz) {
+    if (!x || !y || !z || !cmx || !cmy || !cmz || n_cells < 0) return fail(CS3D_INVALID_ARG, "bad cell_centres args");
+    const float* in[3] = {x, y, z};
+    float* out[3] = {cmx, cmy, cmz};
+    for (int32_t c = 0; c < n_cells; ++c) {
+        for (int a = 0; a < 3; ++a) {
+            const float* v = in[a] + (int64_t)c * CS3D_SLOTS;
+            float t[128];
+            for (int i = 0; i < 128; ++i) t[i] = v[i] + (i + 128 < CS3D_NODES ? v[i + 128] : 0.0f);
+            for (int i = 0; i < 64; ++i) t[i] = t[i] + t[i + 64];
+            for (int o = 32; o > 0; o >>= 1)
+                for (int i = 0; i < o; ++i) t[i] = t[i] + t[i + o];
+            out[a][c] = t[0] / 180.0f;
+        }
+    }
+    return CS3D_OK;
+}
+
 extern "C" cs3d_status cs3d_ranmar_init(cs3d_ranmar_state* st, int32_t ij, int32_t kl) {
     if (!st) return fail(CS3D_INVALID_ARG, "state is NULL");
     if (ij < 0 || ij > 31328 || kl < 0 || kl > 30081) return fail(CS3D_INVALID_ARG, "RANMAR seeds out of range");

@@ -56,12 +56,18 @@ __device__ __forceinline__ int vox1(float x, float mn, float DL, int div) {
 }
 
 // bin coordinate of one axis (clamping keeps |i - j| <= 1 for nodes within one bin edge)
+// Search bins are periodic: bin coordinate floor((x - origin) / h) modulo the bin
+// count of the axis.  Regions farther apart than the bin grid alias into the same
+// bins, which only adds entries the distance test rejects, so the bin edge never
+// has to grow with the system's extent (outlying cells cannot coarsen the bins of
+// a dense tissue).
 __device__ __forceinline__ int bin1(float x, float mn, float h, int div) {
-    int s = (int)floorf((x - mn) / h);
-    if (s < 0) s = 0;
-    if (s >= div) s = div - 1;
-    return s;
+    float q = floorf((x - mn) / h);
+    q = q == q ? fminf(fmaxf(q, -1.0e9f), 1.0e9f) : 0.0f;  // NaN positions go to bin 0 (flagged elsewhere)
+    int s = (int)q % div;
+    return s < 0 ? s + div : s;
 }
+__device__ __forceinline__ int wrap1(int s, int div) { return s < 0 ? s + div : (s >= div ? s - div : s); }
 
 __device__ __forceinline__ int bin_of(const SysMeta* M, float x, float y, float z) {
     const int bx = bin1(x, M->bmin[0], M->bh, M->bdims[0]);
@@ -188,19 +194,20 @@ __device__ int64_t set_grid_geometry(SysMeta* M, const cs3d_params* P, const flo
     M->gmin[0] = e[0]; M->gmin[1] = e[2]; M->gmin[2] = e[4];
     M->dims[0] = ok ? xd : 1; M->dims[1] = ok ? yd : 1; M->dims[2] = ok ? zd : 1;
     if (!ok) atomicOr(&M->status, (uint32_t)CS3D_SYS_GRID_TOO_LARGE);
+    // search bins of edge attraction_range + pair_skin covering the extent, or
+    // fewer along the longest axes when the budget is short (periodic bins)
     float h = (P->attraction_range + P->pair_skin) * 1.001f;
     if (!(h > 0.f)) h = 1.0f;
     const float sz[3] = {e[1] - e[0], e[3] - e[2], e[5] - e[4]};
-    int bd[3] = {5, 5, 5};
-    for (int it = 0; it < 64; ++it) {
-        int64_t nb = 1;
-        for (int a = 0; a < 3; ++a) {
-            const float q = sz[a] / h;
-            bd[a] = (q >= 0.f && q < 1.0e7f) ? (int)q + 5 : 5;
-            nb *= bd[a];
-        }
-        if (nb <= M->bbudget) break;
-        h *= 1.25f;
+    int bd[3];
+    for (int a = 0; a < 3; ++a) {
+        const float q = sz[a] / h;
+        bd[a] = (q >= 0.f && q < 1.0e6f) ? (int)q + 5 : 1000000;
+    }
+    for (int it = 0; it < 200; ++it) {
+        if ((int64_t)bd[0] * bd[1] * bd[2] <= M->bbudget) break;
+        const int a = bd[0] >= bd[1] && bd[0] >= bd[2] ? 0 : (bd[1] >= bd[2] ? 1 : 2);
+        bd[a] = max(3, (int)(bd[a] * 0.9f));
     }
     for (int a = 0; a < 3; ++a) {
         M->bmin[a] = e[2 * a] - 2.0f * h;
@@ -285,7 +292,7 @@ __global__ __launch_bounds__(192) void k_integrate(Pool p, Range r, int32_t flag
                 float4* ent = reinterpret_cast<float4*>(p.bins + M->ebase);
                 ent[(int64_t)b * kBCap + k] = make_float4(x, y, z, __int_as_float(c * kSlots + tid));
             } else {
-                flag(M, CS3D_SYS_CAND_OVERFLOW);
+                flag(M, CS3D_SYS_BIN_OVERFLOW);
             }
         }
         if (tid == 0 && (flags & I_PRESSURE)) {
@@ -338,15 +345,13 @@ __device__ int find_candidates(const Pool& p, const SysMeta* M, const cs3d_param
     // loaded together (independent loads), longer bins finish in a tail loop
     constexpr int kHead = 4;
     for (int row = 0; row < 9; ++row) {
-        const int y3 = by + row % 3 - 1, z3 = bz + row / 3 - 1;
-        const bool rin = y3 >= 0 && y3 < bdy && z3 >= 0 && z3 < bdz;
+        const int y3 = wrap1(by + row % 3 - 1, bdy), z3 = wrap1(bz + row / 3 - 1, bdz);
         int bi[3], cn[3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            const int x3 = bx + q - 1;
-            const bool in = rin && x3 >= 0 && x3 < bdx;
-            bi[q] = in ? (z3 * bdy + y3) * bdx + x3 : 0;
-            cn[q] = in ? min(counts[bi[q]], kBCap) : 0;
+            const int x3 = wrap1(bx + q - 1, bdx);
+            bi[q] = (z3 * bdy + y3) * bdx + x3;
+            cn[q] = min(counts[bi[q]], kBCap);
         }
         float4 head[3][kHead];  // loads of empty slots are masked off
 #pragma unroll
@@ -1213,11 +1218,11 @@ __global__ __launch_bounds__(256) void k_sys_divide(Pool p, Range r, int32_t mod
     SysMeta* M = p.meta + s;
     const bool step_mode = mode == M_STEP;
     if (!M->active) {
-        if (tid == 0 && step_mode) M->rebuild = 0;
+        if (tid == 0 && step_mode) { M->rebuild = 0; M->shift_pending = 0; }
         return;
     }
     if (step_mode && !sys_live(M)) {
-        if (tid == 0) M->rebuild = 0;
+        if (tid == 0) { M->rebuild = 0; M->shift_pending = 0; }
         return;
     }
     const cs3d_params* P = p.prm + s;
@@ -1372,7 +1377,29 @@ __global__ __launch_bounds__(256) void k_sys_divide(Pool p, Range r, int32_t mod
         int32_t* counts = p.vox + M->vbase;
         for (int64_t v = tid; v < nclear; v += blockDim.x) counts[v] = 0;
     }
+    const bool shift = step_mode && P->correct_com;
+    if (shift) {
+        // calc_sys_CM (GPUbounce.cu:2504-2520): float sums of the cell centres in
+        // cell order (daughters of this step still hold 0), divided by the count;
+        // the nodes are shifted by k_com_shift (CorrectCoMMotion, propagate.cu:766-776)
+        float acc[3] = {0.f, 0.f, 0.f};
+        for (int c0 = 0; c0 < ncur; c0 += 128) {
+            if (tid < 128 && c0 + tid < ncur) {
+                redc[0][tid] = p.CMX[base + c0 + tid];
+                redc[1][tid] = p.CMY[base + c0 + tid];
+                redc[2][tid] = p.CMZ[base + c0 + tid];
+            }
+            __syncthreads();
+            if (tid < 3) {
+                const int m = min(128, ncur - c0);
+                for (int k = 0; k < m; ++k) acc[tid] = acc[tid] + redc[tid][k];
+            }
+            __syncthreads();
+        }
+        if (tid < 3) M->sys_cm[tid] = acc[tid] / (float)ncur;
+    }
     if (tid == 0) {
+        M->shift_pending = shift ? 1 : 0;
         if (bad) atomicOr(&M->status, (uint32_t)bad);
         M->n_events = events;
         M->n = ncur;
@@ -1385,6 +1412,26 @@ __global__ __launch_bounds__(256) void k_sys_divide(Pool p, Range r, int32_t mod
         } else if (events > 0) {
             M->need_search = 1;
         }
+    }
+}
+
+// CorrectCoMMotion (propagate.cu:766-776) for the systems whose division phase
+// just computed the system centre: every node of every cell minus sys_cm.  The
+// reference's kernel also shifts the 12 padding slots of each cell; they are no
+// mesh nodes and stay 0 here.  The bboxes and cell centres of the step are left
+// as they are, as in the reference (they were computed before the shift).
+__global__ __launch_bounds__(192) void k_com_shift(Pool p, Range r) {
+    const int tid = threadIdx.x;
+    if (tid >= kNodes) return;
+    for (int slot = r.slot_lo + blockIdx.x; slot < r.slot_hi; slot += gridDim.x) {
+        const int s = p.slot_sys[slot];
+        if (s < 0) continue;
+        const SysMeta* M = p.meta + s;
+        if (!M->shift_pending || slot - M->base >= M->n) continue;
+        const int64_t gi = (int64_t)slot * kSlots + tid;
+        p.X[gi] = p.X[gi] - M->sys_cm[0];
+        p.Y[gi] = p.Y[gi] - M->sys_cm[1];
+        p.Z[gi] = p.Z[gi] - M->sys_cm[2];
     }
 }
 
@@ -1661,6 +1708,9 @@ void launch_dis(const Pool& p, const Range& r, int mode, int64_t step_override, 
 void launch_divide(const Pool& p, const Range& r, int mode, hipStream_t st) {
     const int ns = r.sys_hi - r.sys_lo;
     hipLaunchKernelGGL(k_sys_divide, dim3(ns > 0 ? ns : 1), dim3(256), 0, st, p, r, mode);
+}
+void launch_com_shift(const Pool& p, const Range& r, hipStream_t st) {
+    hipLaunchKernelGGL(k_com_shift, dim3(grid_cells(r, p.wg_per_cu)), dim3(kSlots), 0, st, p, r);
 }
 void launch_grid(const Pool& p, const Range& r, int vox_blocks, hipStream_t st) {
     const int n = r.slot_hi - r.slot_lo;

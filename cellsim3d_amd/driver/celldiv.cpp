@@ -60,7 +60,13 @@ struct Settings {
     int new_cell_count_int = 0;
     bool count_only_internal = false;
     float rad_frac = 0.f;
+    std::string mit_ind_file;
+    bool overwrite_mit_ind = false;
+    bool during_growth = false;
+    bool choose_random_cell_indices = false;
     int use_walls = 0;
+    std::string perp_axis;
+    float d_axis = 0.f;
     float closeness_to_center = 0.f;
     float fraction_of_softer_cells = 0.f;
     int number_of_softer_cells = 0;
@@ -133,8 +139,10 @@ int read_params(const char* path, Settings& s) {
         std::printf("ERROR: Cannot load counting parameters\nExiting");
         return -1;
     }
+    s.mit_ind_file = cnt["mit-index_file_name"].as_string();
     s.count_only_internal = cnt["count_only_internal_cells?"].as_bool();
     s.rad_frac = cnt["radius_cutoff"].as_float();
+    s.overwrite_mit_ind = cnt["overwrite_mit_ind_file?"].as_bool();
     s.new_cell_count_int = cnt["cell_count_int"].as_int();
 
     const jsonlite::Value& pop = root["population"];
@@ -151,6 +159,8 @@ int read_params(const char* path, Settings& s) {
         return -1;
     }
     s.use_walls = walls["useWalls"].as_int();
+    s.perp_axis = walls["perpAxis"].as_string();
+    s.d_axis = walls["dAxis"].as_float();
     p.thresh_dist = walls["threshDist"].as_float();
 
     const jsonlite::Value& div = root["divParams"];
@@ -173,6 +183,8 @@ int read_params(const char* path, Settings& s) {
     s.number_of_softer_cells = stf["numberOfSofterCells"].as_int();
     s.closeness_to_center = stf["closenessToCenter"].as_float();
     s.fraction_of_softer_cells = stf["fractionOfSofterCells"].as_float();
+    s.during_growth = stf["duringGrowth"].as_bool();
+    s.choose_random_cell_indices = stf["chooseRandomCellIndices"].as_bool();
     p.daught_same_stiffness = stf["daughtSameStiffness"].as_bool();
     p.recalc_r0 = stf["recalc_r0"].as_bool();
 
@@ -255,8 +267,6 @@ int read_params(const char* path, Settings& s) {
 int refuse_unsupported(const Settings& s) {
     const char* why = nullptr;
     if (s.do_adaptive_dt) why = "core.doAdaptive_dt = 1";
-    else if (s.correct_com) why = "core.correct_com = 1";
-    else if (s.p.use_different_stiffnesses) why = "stiffnessParams.useDifferentStiffnesses = 1 (softer-cell selection)";
     else if (s.use_pbcs) why = "boxParams.usePBCs = 1 (unavailable in the reference as well)";
     if (!why) return 0;
     std::printf("ERROR: not supported by this driver: %s\n", why);
@@ -416,6 +426,131 @@ int place_cells(const Settings& s, int n, int side, std::vector<float> ix, std::
     return 0;
 }
 
+// The host copy of the system centre, sysCMx/y/z (GPUbounce.cu:240, initial
+// value 1): calc_sys_CM overwrites it; getRmax2 reads whatever it holds.
+struct SysCentre {
+    float x = 1.0f, y = 1.0f, z = 1.0f;
+};
+
+// calc_sys_CM (GPUbounce.cu:2504-2520): float sums in cell order, divided by the count
+void calc_sys_cm(SysCentre& sc, const float* cx, const float* cy, const float* cz, int n) {
+    sc.x = 0; sc.y = 0; sc.z = 0;
+    for (int c = 0; c < n; ++c) { sc.x += cx[c]; sc.y += cy[c]; sc.z += cz[c]; }
+    sc.x = sc.x / n; sc.y = sc.y / n; sc.z = sc.z / n;
+}
+
+// getRmax2 (GPUbounce.cu:2523-2535)
+float get_rmax2(const SysCentre& sc, const float* cx, const float* cy, const float* cz, int n) {
+    float r2 = 0;
+    for (int c = 0; c < n; ++c) {
+        const float dx = cx[c] - sc.x, dy = cy[c] - sc.y, dz = cz[c] - sc.z;
+        r2 = std::max(r2, dx * dx + dy * dy + dz * dz);
+    }
+    return r2;
+}
+
+// rGrowth of the step loop (GPUbounce.cu:1077-1088), from the cell count at the
+// start of the step
+float growth_rate_now(const cs3d_params& p, int n) {
+    if (p.do_pop_model == 1) {
+        float r = p.growth_rate * (1 - (n * 1.0 / p.max_pop));
+        return r < 0 ? 0.f : r;
+    }
+    return p.growth_rate;
+}
+
+// Wall set-up (GPUbounce.cu:808-861): only the checks have an effect, since the
+// contact kernel applies no wall force.  The axis must be z; the wall gap
+// dAxis * 2 * (3 divVol / (4 pi))^(1/3) must not be smaller than the z spread the
+// reference measures (its sort covers the first n entries of the z array only).
+int check_walls(const Settings& s, int n, const std::vector<float>& Z) {
+    if (s.use_walls != 1) return 0;
+    if (s.perp_axis.empty() || (s.perp_axis[0] != 'Z' && s.perp_axis[0] != 'z')) {
+        std::printf(" Invalid wall axis selection %s\nExiting...", s.perp_axis.c_str());
+        return -1;
+    }
+    std::vector<float> tz(Z.begin(), Z.begin() + (size_t)n * S);
+    std::sort(tz.begin(), tz.begin() + n);
+    float radius = 3.0 * s.p.division_vol / 4.0;
+    radius = radius / 3.14159;
+    radius = std::pow(radius, 0.33333333333);
+    const float gap = s.d_axis * 2 * radius;
+    if (gap < (tz[n] - tz[0])) {
+        std::printf("Distance between walls is too small\nExiting...");
+        std::printf("Starting system size= %f\nWall gap = %f", tz[n] - tz[0], gap);
+        return -1;
+    }
+    return 0;
+}
+
+// Softer cells chosen at set-up (GPUbounce.cu:615-692): within closenessToCenter of
+// the largest centre distance, else numberOfSofterCells (or the fraction) cells,
+// the first ones in a rigid box and RANMAR-drawn indices otherwise.  E holds one
+// stiffness per cell slot.  Returns 12516 when more cells are asked for than exist.
+int soft_cells_at_setup(const Settings& s, int n, const std::vector<float>& X, const std::vector<float>& Y,
+                        const std::vector<float>& Z, cs3d_ranmar_state* rng, std::vector<float>& E) {
+    E.assign(s.max_cells, s.p.stiffness1);
+    if (!s.p.use_different_stiffnesses) return 0;
+    std::printf("Making some cells softer... \n");
+    if (s.closeness_to_center > 0.f && s.closeness_to_center < 1.f) {
+        std::printf("Only making cells within %f of max radius softer\n", s.closeness_to_center);
+        std::vector<float> cx(n), cy(n), cz(n);
+        cs3d_cell_centres(X.data(), Y.data(), Z.data(), n, cx.data(), cy.data(), cz.data());
+        float sx = 0.f, sy = 0.f, sz = 0.f;
+        for (int c = 0; c < n; ++c) { sx = sx + cx[c]; sy = sy + cy[c]; sz = sz + cz[c]; }
+        const float inv = 1 / (float)n;  // float3 / float is (1/a)*b (inc/VectorFunctions.hpp:84-92)
+        sx = inv * sx; sy = inv * sy; sz = inv * sz;
+        std::printf("COM = (%f, %f, %f)\n", sx, sy, sz);
+        float rmax = 0;
+        std::vector<float> mags(n);
+        for (int c = 0; c < n; ++c) {
+            const float dx = cx[c] - sx, dy = cy[c] - sy, dz = cz[c] - sz;
+            mags[c] = sqrtf(dx * dx + dy * dy + dz * dz);
+            rmax = std::max(rmax, mags[c]);
+        }
+        int made = 0;
+        for (int c = 0; c < n; ++c)
+            if (mags[c] / rmax <= s.closeness_to_center) { E[c] = s.p.stiffness2; ++made; }
+        std::printf("Made %d cells softer\n", made);
+        return 0;
+    }
+    std::printf("Choosing softer cells randomly\n");
+    int count = s.number_of_softer_cells;
+    if (s.fraction_of_softer_cells > 0.f && s.fraction_of_softer_cells < 1.f)
+        count = (int)std::round(s.fraction_of_softer_cells * (float)n);
+    if (count > n) {
+        std::printf("ERROR: Too many softer cells requested\n");
+        return 12516;
+    }
+    if (s.p.use_rigid_box) {
+        for (int c = 0; c < count; ++c) E[c] = s.p.stiffness2;
+    } else {
+        for (int c = 0; c < count; ++c) {
+            float r;
+            cs3d_ranmar(rng, &r, 1);
+            const int ind = (int)std::round(r * n);  // may be n: a slot not yet in use
+            if (ind < s.max_cells) E[ind] = s.p.stiffness2;
+        }
+    }
+    std::printf("Made %d cells softer\n", count);
+    return 0;
+}
+
+// correct_com at set-up (GPUbounce.cu:785-806): the mean of the cell centres is
+// subtracted from every mesh node (the reference also shifts the 12 padding
+// slots of each cell, which are no mesh nodes; they stay 0 here)
+void correct_com_at_setup(int n, std::vector<float>& X, std::vector<float>& Y, std::vector<float>& Z,
+                          SysCentre& sc) {
+    std::vector<float> cx(n), cy(n), cz(n);
+    cs3d_cell_centres(X.data(), Y.data(), Z.data(), n, cx.data(), cy.data(), cz.data());
+    calc_sys_cm(sc, cx.data(), cy.data(), cz.data(), n);
+    for (int c = 0; c < n; ++c)
+        for (int a = 0; a < NN; ++a) {
+            const size_t g = (size_t)c * S + a;
+            X[g] = X[g] - sc.x; Y[g] = Y[g] - sc.y; Z[g] = Z[g] - sc.z;
+        }
+}
+
 struct Output {
     FILE* ascii = nullptr;
     cs3d_traj* bin = nullptr;
@@ -431,16 +566,26 @@ bool get_field(cs3d_ctx* ctx, cs3d_sys sys, cs3d_field f, void* host, size_t byt
     return true;
 }
 
-// write_traj (GPUbounce.cu:2412-2444) / WriteBinaryTraj (:2446-2477) of one frame
+// write_traj (GPUbounce.cu:2412-2444) / WriteBinaryTraj (:2446-2477) of one frame;
+// with variable stiffness each cell carries its type (0/H: stiffness1, 1/C: other)
 bool write_positions(const Settings& s, Output& out, int step, int frame, int n, const std::vector<float>& X,
-                     const std::vector<float>& Y, const std::vector<float>& Z) {
+                     const std::vector<float>& Y, const std::vector<float>& Z, const std::vector<float>& E) {
+    const bool var = s.p.use_different_stiffnesses != 0;
     if (out.bin) {
-        if (cs3d_traj_append(out.bin, step, frame, n, X.data(), Y.data(), Z.data(), nullptr) != CS3D_OK) return false;
+        std::vector<int32_t> types;
+        if (var)
+            for (int c = 0; c < n; ++c) types.push_back(E[c] == s.p.stiffness1 ? 0 : 1);
+        if (cs3d_traj_append(out.bin, step, frame, n, X.data(), Y.data(), Z.data(), var ? types.data() : nullptr) !=
+            CS3D_OK)
+            return false;
     } else if (out.ascii) {
         std::fprintf(out.ascii, "%d\n", n * 192);
         std::fprintf(out.ascii, "Step: %d frame: %d\n", step, step / s.traj_write_int);
         for (int c = 0; c < n; ++c) {
-            std::fprintf(out.ascii, "cell: %d\n", c);
+            if (!var) std::fprintf(out.ascii, "cell: %d\n", c);
+            else if (E[c] == s.p.stiffness1) std::fprintf(out.ascii, "cell: %d H\n", c);
+            else if (E[c] == s.p.stiffness2) std::fprintf(out.ascii, "cell: %d C\n", c);
+            else std::fprintf(out.ascii, "cell: %d UnknownStiffness\n", c);
             for (int q = 0; q < S; ++q)
                 std::fprintf(out.ascii, "%.7f,  %.7f,  %.7f\n", X[(size_t)c * S + q], Y[(size_t)c * S + q],
                              Z[(size_t)c * S + q]);
@@ -480,12 +625,113 @@ bool write_forces(cs3d_ctx* ctx, cs3d_sys sys, Output& out, int step, int n, con
 }
 
 bool write_frame(cs3d_ctx* ctx, cs3d_sys sys, const Settings& s, Output& out, int step, int frame, int n) {
-    std::vector<float> X((size_t)n * S), Y((size_t)n * S), Z((size_t)n * S);
+    std::vector<float> X((size_t)n * S), Y((size_t)n * S), Z((size_t)n * S), E(std::max(n, 1));
     if (n > 0 && (!get_field(ctx, sys, CS3D_F_X, X.data(), X.size() * 4) ||
                   !get_field(ctx, sys, CS3D_F_Y, Y.data(), Y.size() * 4) ||
-                  !get_field(ctx, sys, CS3D_F_Z, Z.data(), Z.size() * 4)))
+                  !get_field(ctx, sys, CS3D_F_Z, Z.data(), Z.size() * 4) ||
+                  (s.p.use_different_stiffnesses && !get_field(ctx, sys, CS3D_F_STIFFNESS, E.data(), (size_t)n * 4))))
         return false;
-    return write_positions(s, out, step, frame, n, X, Y, Z) && write_forces(ctx, sys, out, step, n, X, Y, Z);
+    return write_positions(s, out, step, frame, n, X, Y, Z, E) && write_forces(ctx, sys, out, step, n, X, Y, Z);
+}
+
+bool get_centres(cs3d_ctx* ctx, cs3d_sys sys, int n, std::vector<float>& cx, std::vector<float>& cy,
+                 std::vector<float>& cz) {
+    cx.resize(std::max(n, 1)); cy.resize(std::max(n, 1)); cz.resize(std::max(n, 1));
+    return n == 0 || (get_field(ctx, sys, CS3D_F_CMX, cx.data(), (size_t)n * 4) &&
+                      get_field(ctx, sys, CS3D_F_CMY, cy.data(), (size_t)n * 4) &&
+                      get_field(ctx, sys, CS3D_F_CMZ, cz.data(), (size_t)n * 4));
+}
+
+// num_cells_far (GPUbounce.cu:2537-2565): of the num_div newest cells, those whose
+// centre lies beyond radius_cutoff of the largest centre distance (a daughter's
+// centre is still 0 in the step that made it)
+bool num_cells_far(cs3d_ctx* ctx, cs3d_sys sys, const Settings& s, int n, int num_div, SysCentre& sc, int& far) {
+    far = 0;
+    if (num_div == 0 || n < 50) return true;
+    std::vector<float> cx, cy, cz;
+    if (!get_centres(ctx, sys, n, cx, cy, cz)) return false;
+    calc_sys_cm(sc, cx.data(), cy.data(), cz.data(), n);
+    const float rmax2 = get_rmax2(sc, cx.data(), cy.data(), cz.data(), n);
+    for (int c = n - num_div; c < n; ++c) {
+        const float dx = cx[c] - sc.x, dy = cy[c] - sc.y, dz = cz[c] - sc.z;
+        const float dr2 = dx * dx + dy * dy + dz * dz;
+        if (dr2 > s.rad_frac * s.rad_frac * rmax2) ++far;
+    }
+    return true;
+}
+
+// Softer cells chosen when growth ends (GPUbounce.cu:1340-1429, useDifferentStiffnesses
+// without duringGrowth): distinct RANMAR-drawn indices, or the first cells (in index
+// order) within (1 - closenessToCenter) of the largest centre distance.  The
+// largest distance is taken about the system centre held from the last
+// calc_sys_CM, as the reference does, before the centre is recomputed.
+int soft_cells_after_growth(cs3d_ctx* ctx, cs3d_sys sys, Settings& s, int n, SysCentre& sc) {
+    std::printf("Now making some cells softer...\n");
+    if (s.fraction_of_softer_cells > 0) s.number_of_softer_cells = (int)roundf(s.fraction_of_softer_cells * n);
+    const int want = s.number_of_softer_cells;
+    std::printf("Will make %d cells softer\n", want);
+    std::vector<float> E(std::max(n, 1));
+    if (n > 0 && !get_field(ctx, sys, CS3D_F_STIFFNESS, E.data(), (size_t)n * 4)) return -1;
+    if (s.choose_random_cell_indices) {
+        if (want > n + 1) {  // the reference's draw loop would never end
+            std::printf("ERROR: %d distinct softer cells requested from %d indices\n", want, n + 1);
+            return -1;
+        }
+        cs3d_ranmar_state rng;
+        if (!get_field(ctx, sys, CS3D_F_RANMAR, &rng, sizeof(rng))) return -1;
+        std::vector<int> chosen;
+        std::printf("Make cells with indices ");
+        while ((int)chosen.size() < want) {
+            float r;
+            cs3d_ranmar(&rng, &r, 1);
+            const int ind = (int)roundf(r * n);
+            if (std::find(chosen.begin(), chosen.end(), ind) == chosen.end()) {
+                chosen.push_back(ind);
+                std::printf("%d ", ind);
+            }
+        }
+        for (int ind : chosen)
+            if (ind < n) E[ind] = s.p.stiffness2;  // index n names no cell (the copy covers n cells)
+        if (cs3d_set(ctx, sys, CS3D_F_RANMAR, &rng, sizeof(rng)) != CS3D_OK) return -1;
+    } else {
+        std::vector<float> cx, cy, cz;
+        if (!get_centres(ctx, sys, n, cx, cy, cz)) return -1;
+        const float rmax2 = get_rmax2(sc, cx.data(), cy.data(), cz.data(), n);
+        calc_sys_cm(sc, cx.data(), cy.data(), cz.data(), n);
+        const float f = 1 - s.closeness_to_center;
+        std::printf("Made cells with indices ");
+        int made = 0;
+        for (int c = 0; made < want && c < n; ++c) {
+            const float dx = cx[c] - sc.x, dy = cy[c] - sc.y, dz = cz[c] - sc.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 <= f * f * rmax2) {
+                std::printf("%d, ", c);
+                ++made;
+                E[c] = s.p.stiffness2;
+            }
+        }
+    }
+    if (n > 0 && cs3d_set(ctx, sys, CS3D_F_STIFFNESS, E.data(), (size_t)n * 4) != CS3D_OK) return -1;
+    std::printf("\b\b softer\n");
+    return 0;
+}
+
+// mitotic-index file (GPUbounce.cu:1515-1547): one line per counting interval
+// (Time_steps / cell_count_int + 1 lines), new cells over the cells that did not
+// divide, 0 where either count is 0.  The reference reads its last line one past
+// the end of its arrays; it is 0 here.
+bool write_mit_index(const Settings& s, const std::vector<int>& dividing, const std::vector<int>& total) {
+    FILE* f = std::fopen(s.mit_ind_file.c_str(), s.overwrite_mit_ind ? "w" : "a");
+    if (!f) {
+        std::printf("Failed to open mit-index.dat\n");
+        std::exit(1);
+    }
+    for (size_t i = 0; i < dividing.size(); ++i) {
+        if (dividing[i] != 0 && total[i] != 0) std::fprintf(f, "%f\n", (float)dividing[i] / total[i]);
+        else std::fprintf(f, "%f\n", 0.0);
+    }
+    std::fclose(f);
+    return true;
 }
 
 }  // namespace
@@ -570,6 +816,13 @@ int main(int argc, char** argv) {
         std::printf("ERROR: mesh: %s\n", cs3d_last_error());
         return -1;
     }
+    std::vector<float> E;
+    const int rs = soft_cells_at_setup(s, n_cells, X, Y, Z, &rng, E);
+    if (rs != 0) return rs;
+    SysCentre sc;
+    if (s.correct_com) correct_com_at_setup(n_cells, X, Y, Z, sc);
+    s.p.correct_com = s.correct_com ? 1 : 0;
+    if (check_walls(s, n_cells, Z) != 0) return -1;
 
     // outputs, GPUbounce.cu:332-335, 768-778, 1007-1048
     if (FILE* f = std::fopen("psfil", "w")) std::fclose(f);
@@ -600,7 +853,7 @@ int main(int argc, char** argv) {
     if (FILE* f = std::fopen("velocity2.xyz", "w")) std::fclose(f);
     // frame 0: the start positions (the device start-up does not move nodes)
     int frame = 1;
-    if (!write_positions(s, out, 0, frame, n_cells, X, Y, Z)) return -1;
+    if (!write_positions(s, out, 0, frame, n_cells, X, Y, Z, E)) return -1;
     if (setup_only) {  // parameters, mesh, placement and frame 0 only; no device
         if (out.bin) cs3d_traj_close(out.bin, n_cells);
         if (out.ascii) std::fclose(out.ascii);
@@ -613,7 +866,7 @@ int main(int argc, char** argv) {
     // clears (GPUbounce.cu:1329), search bins sized like the engine's default
     const int64_t vcells = std::max<int64_t>(1 << 18, 32 * (int64_t)s.max_cells);
     const int64_t vwords = vcells * (1 + s.p.voxel_capacity);
-    const int64_t bwords = std::max<int64_t>(1 << 22, (int64_t)s.max_cells * 640 * 65);
+    const int64_t bwords = std::max<int64_t>(1 << 22, (int64_t)s.max_cells * 640 * (1 + 4 * CS3D_BIN_CAP));
     cs3d_ctx* ctx = nullptr;
     cs3d_status st = cs3d_ctx_create(gpuid, &mesh, s.max_cells, vwords, bwords, &ctx);
     if (st != CS3D_OK) {
@@ -629,16 +882,39 @@ int main(int argc, char** argv) {
         cs3d_ctx_destroy(ctx);
         return -1;
     }
-    // the RANMAR stream continues where placement left it (one stream, as in the reference)
+    // the RANMAR stream continues where placement and soft-cell selection left it
+    // (one stream, as in the reference)
     cs3d_set(ctx, sys, CS3D_F_RANMAR, &rng, sizeof(rng));
+    if (s.p.use_different_stiffnesses) cs3d_set(ctx, sys, CS3D_F_STIFFNESS, E.data(), (size_t)n_cells * 4);
 
     const int total = s.time_steps + s.equi_steps;
+    const int I = s.new_cell_count_int;
+    // growth ends at the first step past Time_steps + 1 (GPUbounce.cu:1340)
+    const int growth_end = s.time_steps + 2;
+    const bool select_after_growth = s.p.use_different_stiffnesses && !s.during_growth && growth_end <= total;
+    std::vector<int> dividing(s.time_steps / I + 1, 0), totals(s.time_steps / I + 1, 0);
+    int window_new = 0;  // cells counted since the last counting step
+    bool growth_done = false;
+    // stops: trajectory frames; k*I - 1 and k*I (the counting step needs the count
+    // at its start); every growth step when only internal cells are counted; the
+    // step before growth ends and the one that ends it
+    auto next_stop = [&](int at) {
+        int nx = std::min(total, (at / s.traj_write_int + 1) * s.traj_write_int);
+        if (at < s.time_steps) {
+            if (s.count_only_internal) nx = std::min(nx, at + 1);
+            const int b = (at / I + 1) * I;
+            if (b <= s.time_steps) nx = std::min(nx, b - 1 > at ? b - 1 : b);
+        }
+        if (select_after_growth && at < growth_end) nx = std::min(nx, at < growth_end - 1 ? growth_end - 1 : growth_end);
+        return nx;
+    };
     int n = n_cells;
     int rcode = 0;
     if (!write_forces(ctx, sys, out, 0, n, X, Y, Z)) rcode = -1;
     int step = 0;
     while (rcode == 0 && step < total) {
-        const int next = std::min(total, (step / s.traj_write_int + 1) * s.traj_write_int);
+        const int next = next_stop(step);
+        const int n_start = n;
         st = cs3d_step(ctx, sys, next - step);
         if (st == CS3D_OK) st = cs3d_sync(ctx);
         if (st != CS3D_OK) {
@@ -668,6 +944,37 @@ int main(int argc, char** argv) {
             rcode = -1;
             break;
         }
+        if (step <= s.time_steps) {  // mitotic-index counting (GPUbounce.cu:1288-1302)
+            int born = n - n_start;
+            if (s.count_only_internal && born > 0 && growth_rate_now(s.p, n_start) > 0) {
+                int far = 0;
+                if (!num_cells_far(ctx, sys, s, n, born, sc, far)) { rcode = -1; break; }
+                born -= far;
+            }
+            window_new += born;
+            // the counting step runs only inside the division block (rGrowth > 0)
+            if (step % I == 0) {
+                if (growth_rate_now(s.p, n_start) > 0) {
+                    dividing[(step - 1) / I] = window_new;
+                    totals[(step - 1) / I] = n - window_new;
+                }
+                window_new = 0;
+            }
+        }
+        if (select_after_growth && step == growth_end - 1 && s.correct_com) {
+            // the system centre of the last correct_com, read by getRmax2 below
+            float c3[3];
+            if (!get_field(ctx, sys, CS3D_F_SYSCM, c3, sizeof(c3))) { rcode = -1; break; }
+            sc.x = c3[0]; sc.y = c3[1]; sc.z = c3[2];
+        }
+        if (!growth_done && step >= growth_end) {
+            std::printf("Cell growth halted.\nProceeding with MD simulation without growth...\n");
+            growth_done = true;
+        }
+        if (select_after_growth && step == growth_end && soft_cells_after_growth(ctx, sys, s, n, sc) != 0) {
+            rcode = -1;
+            break;
+        }
         if (step % s.traj_write_int == 0) {
             ++frame;
             if (!write_frame(ctx, sys, s, out, step, frame, n)) rcode = -1;
@@ -677,6 +984,7 @@ int main(int argc, char** argv) {
     if (out.ascii) std::fclose(out.ascii);
     if (out.forces) std::fclose(out.forces);
     cs3d_ctx_destroy(ctx);
+    if (rcode == 0) write_mit_index(s, dividing, totals);
     std::printf("%d cells after %d steps\n", n, step);
     return rcode;
 }

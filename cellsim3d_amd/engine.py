@@ -33,6 +33,7 @@ SIGNATURES = {
     "cs3d_params_default": ([PP], st_t),
     "cs3d_params_check": ([PP], st_t),
     "cs3d_mesh_build": ([vp, vp, vp, vp, vp, vp, MP], st_t),
+    "cs3d_cell_centres": ([vp, vp, vp, C.c_int32, vp, vp, vp], st_t),
     "cs3d_ranmar_init": ([RP, C.c_int32, C.c_int32], st_t),
     "cs3d_ranmar": ([RP, vp, C.c_int32], st_t),
     "cs3d_ctx_create": ([C.c_int, MP, C.c_int64, C.c_int64, C.c_int64, C.POINTER(vp)], st_t),
@@ -124,12 +125,22 @@ def build_mesh(tables, rest):
     return m
 
 
+def cell_centres(X, Y, Z, n):
+    """Centres of n cells on the host, bit-identical to the device CenterOfMass."""
+    x, y, z = (np.ascontiguousarray(a, dtype=np.float32) for a in (X, Y, Z))
+    if min(len(x), len(y), len(z)) < n * abi.SLOTS:
+        raise ValueError("cell_centres: arrays shorter than n * 192")
+    out = [np.zeros(max(n, 1), np.float32) for _ in range(3)]
+    check(lib().cs3d_cell_centres(_ptr(x), _ptr(y), _ptr(z), int(n), *(_ptr(o) for o in out)), "cs3d_cell_centres")
+    return [o[:n] for o in out]
+
+
 class Context:
     def __init__(self, mesh, capacity_cells, voxel_budget=None, device=0, bin_budget=None):
         if voxel_budget is None:
             voxel_budget = max(1 << 20, capacity_cells * 32 * 40)
         if bin_budget is None:
-            bin_budget = max(1 << 22, capacity_cells * 640 * 65)
+            bin_budget = max(1 << 22, capacity_cells * 640 * abi.BIN_WORDS)
         h = vp()
         check(lib().cs3d_ctx_create(device, C.byref(mesh), int(capacity_cells), int(voxel_budget), int(bin_budget),
                                     C.byref(h)), "cs3d_ctx_create")

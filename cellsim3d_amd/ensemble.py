@@ -46,7 +46,7 @@ def node_updates(tissues, steps):
 def _context(gm, tissues):
     cap = sum(t["max_cells"] for t in tissues)
     vox = sum(max(4096, 32 * t["max_cells"]) * (1 + t["params"].voxel_capacity) for t in tissues)
-    bins = sum(min(max(16384, 640 * t["max_cells"]), 1 << 40) for t in tissues) * 65
+    bins = sum(min(max(16384, 640 * t["max_cells"]), 1 << 40) for t in tissues) * abi.BIN_WORDS
     return engine.Context(gm, capacity_cells=cap, voxel_budget=vox, bin_budget=bins)
 
 

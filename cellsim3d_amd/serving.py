@@ -29,12 +29,12 @@ from . import mesh as meshmod
 # device bytes per cell slot (node arrays, contact lists, per-cell scalars),
 # and the voxel and search-bin words cs3d_system_add reserves per system:
 # max(4096, 32 * cells) voxels of (1 + capacity) words, max(16384, 640 * cells)
-# bins of 65 words
+# bins of abi.BIN_WORDS words
 SLOT_BYTES = (15 + 3) * 192 * 4 + 2 * 40 * 192 * 4 + 2 * 192 * 4 + 16 * 4 + 48 + 1
 VOX_WORDS_PER_CELL = 32 * 65
-BIN_WORDS_PER_CELL = 640 * 65
+BIN_WORDS_PER_CELL = 640 * abi.BIN_WORDS
 VOX_WORDS_MIN = 4096 * 65
-BIN_WORDS_MIN = 16384 * 65
+BIN_WORDS_MIN = 16384 * abi.BIN_WORDS
 SMALL_SYSTEMS = 64  # room for this many systems at the minimum reservations
 
 

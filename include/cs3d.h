@@ -26,6 +26,8 @@ extern "C" {
 
 #define CS3D_NODES 180   /* real nodes per C180 cell */
 #define CS3D_SLOTS 192   /* node slots per cell (stride), GPUbounce.cu:365 */
+#define CS3D_BIN_CAP 32  /* nodes per contact-search bin; a bin takes 1 + 4 * CS3D_BIN_CAP int32 words of the
+                            context's bin budget */
 #define CS3D_FACES 92    /* 12 pentagons + 80 hexagons, GPUbounce.cu:1858-1886 */
 #define CS3D_ALL (-1)    /* cs3d_step target: every active system in one batch */
 
@@ -51,9 +53,10 @@ typedef enum cs3d_status {
 #define CS3D_SYS_BAD_NORMAL       (1u << 6)  /* postscriptinit.cu:215-219 */
 #define CS3D_SYS_NN_OVER_10       (1u << 7)  /* diagnostic: > 10 candidate cells for a node; the reference
                                                 writes past localNNs[10] there (propagate.cu:222,398-403) */
-#define CS3D_SYS_CAND_OVERFLOW    (1u << 8)  /* a cell touches more candidate cells or voxels than the engine's
-                                                per-cell lists hold */
-#define CS3D_SYS_FATAL_MASK       (0x17Fu)   /* every bit except the NN_OVER_10 diagnostic */
+#define CS3D_SYS_CAND_OVERFLOW    (1u << 8)  /* a node has more contact candidates (nodes of other cells within
+                                                attraction_range + pair_skin) than its list holds (40) */
+#define CS3D_SYS_BIN_OVERFLOW     (1u << 9)  /* more nodes in one contact-search bin than it holds (16) */
+#define CS3D_SYS_FATAL_MASK       (0x37Fu)   /* every bit except the NN_OVER_10 diagnostic */
 
 /* Mesh topology and rest state shared by all cells.
  * nn[k*192+i] (k=0..2) and sign[i]: C180NN, 0-based (GPUbounce.cu:1826-1833).
@@ -120,7 +123,10 @@ typedef struct cs3d_params {
                                       (closed, not reproduced); this engine draws standard normals with the
                                       Marsaglia polar method from Philox4x32-10 (Salmon et al., SC'11),
                                       counter (cell * 192 + node, step), reproducible per tissue, node and step */
-    int32_t reserved[4];
+    int32_t correct_com;           /* core.correct_com: at the end of every step the system centre (mean of
+                                      the cell centres, summed in cell order) is subtracted from every node
+                                      position (GPUbounce.cu:1431-1441, propagate.cu:766-776) */
+    int32_t reserved[3];
 } cs3d_params;
 
 /* Fields readable/writable with cs3d_get / cs3d_set.  Sizes are per system
@@ -147,6 +153,8 @@ typedef enum cs3d_field {
                                                   ascending, k-major (engine-specific) */
     CS3D_F_GROWTH_SCALE,                       /* per-cell multiplier of the pressure increment (default 1) */
     CS3D_F_FRX, CS3D_F_FRY, CS3D_F_FRZ,        /* random force of the latest step, d_fRanList (add_rands) */
+    CS3D_F_SYSCM,                              /* 3 floats, read-only: the system centre subtracted at the end
+                                                  of the latest step (sysCMx/y/z; correct_com), else 0 */
     CS3D_F__COUNT
 } cs3d_field;
 
@@ -177,6 +185,13 @@ cs3d_status cs3d_params_check(const cs3d_params* p);
  * faces 92*7 ints. */
 cs3d_status cs3d_mesh_build(const int32_t* nn, const int32_t* sign, const int32_t* faces,
                             const float* x, const float* y, const float* z, cs3d_mesh* out);
+/* CenterOfMass (centermass.cu:8-50) of n_cells cells on the host: the same
+ * 256-slot summation tree as the device kernel, so the results are bit-identical
+ * to CS3D_F_CMX/Y/Z after a device CenterOfMass.  x/y/z hold 192 slots per cell.
+ * Used by the host driver where the reference copies d_CMx to the host at set-up
+ * (soft-cell selection, GPUbounce.cu:623-640; correct_com, :785-792). */
+cs3d_status cs3d_cell_centres(const float* x, const float* y, const float* z, int32_t n_cells, float* cmx,
+                              float* cmy, float* cmz);
 /* RANMAR: rmarin (marsaglia.h:54-113) and ranmar (marsaglia.h:115-144). */
 cs3d_status cs3d_ranmar_init(cs3d_ranmar_state* st, int32_t ij, int32_t kl);
 cs3d_status cs3d_ranmar(cs3d_ranmar_state* st, float* out, int32_t n);

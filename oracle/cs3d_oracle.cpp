@@ -810,6 +810,7 @@ template <class T> struct Sim : SimBase {
     std::vector<int32_t> counts, lists;
     cs3d_ranmar_state rng;
     uint32_t status = 0;
+    float sys_cm[3] = {0.f, 0.f, 0.f};
 
     Sim(const cs3d_params& pp, const cs3d_mesh& mm, int n0, int cap0, int32_t bud, const float* x, const float* y,
         const float* z) : p(pp), m(mm) {
@@ -952,6 +953,20 @@ template <class T> struct Sim : SimBase {
             }
             int gi = p.grid_interval > 0 ? p.grid_interval : 1;
             if (divided || stepno % gi == 0) st |= rebuild_grid();
+            if (p.correct_com) {
+                // calc_sys_CM (GPUbounce.cu:2504-2520): sums in cell order, then one
+                // division by the count; CorrectCoMMotion (propagate.cu:766-776) on the
+                // mesh nodes (GPUbounce.cu:1431-1441)
+                T c3[3] = {T(0), T(0), T(0)};
+                for (int c = 0; c < n; ++c) { c3[0] += CMx[c]; c3[1] += CMy[c]; c3[2] += CMz[c]; }
+                for (int a = 0; a < 3; ++a) c3[a] = c3[a] / (T)n;
+                for (int c = 0; c < n; ++c)
+                    for (int i = 0; i < NN; ++i) {
+                        const int64_t g = (int64_t)c * S + i;
+                        X[g] = X[g] - c3[0]; Y[g] = Y[g] - c3[1]; Z[g] = Z[g] - c3[2];
+                    }
+                for (int a = 0; a < 3; ++a) sys_cm[a] = (float)c3[a];
+            }
             status |= st;
             if (st & CS3D_SYS_FATAL_MASK) break;
         }
@@ -978,6 +993,11 @@ template <class T> struct Sim : SimBase {
             float* o = (float*)out;
             for (size_t i = 0; i < cnt; ++i) o[i] = (float)(*v)[i];
             return cnt * sizeof(float);
+        }
+        if (f == CS3D_F_SYSCM) {
+            size_t cnt = std::min(bytes, sizeof(sys_cm));
+            std::memcpy(out, sys_cm, cnt);
+            return cnt;
         }
         if (f == CS3D_F_DIVFLAG) {
             size_t cnt = std::min(bytes, flag.size());

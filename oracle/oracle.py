@@ -180,6 +180,8 @@ class Sim:
             return self.max_cells, np.int8
         if f == abi.F_GRID:
             return 8, np.int32
+        if f == abi.F_SYSCM:
+            return 3, np.float32
         if f == abi.F_VOXCOUNT:
             return self.budget, np.int32
         if f == abi.F_VOXLIST:

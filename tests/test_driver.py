@@ -72,20 +72,38 @@ def _setup(tmp_path, edit=None, n=9, mesh=None):
     return r, d
 
 
-def _read_traj(path):
+def _read_frames(path):
+    """All frames of a binary trajectory (GPUbounce.cu:1007-1018, 2446-2477):
+    header, then per frame (step, frame, n, X, Y, Z, types or None)."""
     b = open(path, "rb").read()
     hdr = struct.unpack_from("<3i", b, 0)
-    step, frame, n = struct.unpack_from("<3i", b, 12)
-    off = 24
-    X, Y, Z = (np.zeros((n, 192), np.float32) for _ in range(3))
-    for c in range(n):
-        (idx,) = struct.unpack_from("<i", b, off)
-        assert idx == c
-        off += 4
-        for A in (X, Y, Z):
-            A[c] = np.frombuffer(b, dtype="<f4", count=192, offset=off)
-            off += 768
+    var = hdr[1]
+    off = 12
+    frames = []
+    while off < len(b):
+        step, frame, n = struct.unpack_from("<3i", b, off)
+        off += 12
+        X, Y, Z = (np.zeros((n, 192), np.float32) for _ in range(3))
+        types = np.zeros(n, np.int32) if var else None
+        for c in range(n):
+            (idx,) = struct.unpack_from("<i", b, off)
+            assert idx == c
+            off += 4
+            for A in (X, Y, Z):
+                A[c] = np.frombuffer(b, dtype="<f4", count=192, offset=off)
+                off += 768
+            if var:
+                (types[c],) = struct.unpack_from("<i", b, off)
+                off += 4
+        frames.append((step, frame, n, X, Y, Z, types))
     assert off == len(b)
+    return hdr, frames
+
+
+def _read_traj(path):
+    hdr, frames = _read_frames(path)
+    assert len(frames) == 1
+    step, frame, n, X, Y, Z, _ = frames[0]
     return hdr, (step, frame, n), X, Y, Z
 
 
@@ -181,3 +199,96 @@ def test_reads_the_reference_parameter_file(tmp_path):
                        text=True, timeout=60)
     assert r.returncode == 0, r.stdout
     assert "parsed successfully" in r.stdout and "16 cells placed" in r.stdout
+
+
+# ------------------------------------------------------------------ soft cells, correct_com, walls
+def _lattice_centres(n, side, box=60.0):
+    c = np.array([[3.0 * (k // side) + 1.5 + box / 2, 3.0 * (k % side) + 1.5 + box / 2] for k in range(n)])
+    return c
+
+
+def _soft(edit_stf, rigid=1):
+    def edit(d):
+        d["stiffnessParams"].update(useDifferentStiffnesses=1, **edit_stf)
+        d["boxParams"]["useRigidSimulationBox"] = rigid
+    return edit
+
+
+def test_soft_cells_within_a_radius_of_the_centre(tmp_path):
+    """GPUbounce.cu:623-659: cells whose centre lies within closenessToCenter of the
+    largest centre distance (about the mean centre) become soft; the frame carries
+    the cell types."""
+    r, _ = _setup(tmp_path, _soft({"closenessToCenter": 0.5}), n=16)
+    assert r.returncode == 0, r.stdout
+    hdr, frames = _read_frames(tmp_path / "t.xyz")
+    assert hdr[1] == 1
+    types = frames[0][6]
+    c = _lattice_centres(16, 4)
+    d = np.linalg.norm(c - c.mean(0), axis=1)
+    assert np.array_equal(types, (d / d.max() <= 0.5).astype(np.int32))
+    assert types.sum() == 4  # the inner 2 x 2 block of the 4 x 4 lattice
+
+
+def test_soft_cells_first_ones_in_a_rigid_box(tmp_path):
+    """GPUbounce.cu:661-681: round(fraction * n) cells; the first ones in a rigid box."""
+    r, _ = _setup(tmp_path, _soft({"fractionOfSofterCells": 0.3}), n=16)
+    assert r.returncode == 0, r.stdout
+    _, frames = _read_frames(tmp_path / "t.xyz")
+    assert list(frames[0][6]) == [1] * 5 + [0] * 11
+
+
+def test_soft_cells_drawn_from_the_ranmar_stream(tmp_path):
+    """GPUbounce.cu:682-689: without a rigid box each soft cell index is
+    round(u * n) for the next RANMAR number (repeats allowed)."""
+    def edit(d):
+        _soft({"fractionOfSofterCells": 0.0, "numberOfSofterCells": 6}, rigid=0)(d)
+    r, _ = _setup(tmp_path, edit, n=16)
+    assert r.returncode == 0, r.stdout
+    _, frames = _read_frames(tmp_path / "t.xyz")
+    rng = O.Ranmar(7, 700)
+    want = np.zeros(16, np.int32)
+    for u in rng.draw(6):
+        i = int(np.round(np.float32(u) * np.float32(16)))
+        if i < 16:
+            want[i] = 1
+    assert np.array_equal(frames[0][6], want)
+
+
+def test_too_many_soft_cells_is_an_error(tmp_path):
+    r, _ = _setup(tmp_path, _soft({"fractionOfSofterCells": 0.0, "numberOfSofterCells": 17}), n=16)
+    assert r.returncode == 12516 % 256 and "Too many softer cells" in r.stdout
+
+
+def test_correct_com_centres_the_start_positions(tmp_path):
+    """GPUbounce.cu:785-806: the start positions minus the mean of the cell
+    centres (float sums in cell order); mesh nodes only."""
+    r0, _ = _setup(tmp_path, n=9)
+    _, _, X0, Y0, Z0 = _read_traj(tmp_path / "t.xyz")
+    r1, _ = _setup(tmp_path, lambda d: d["core"].__setitem__("correct_com", 1), n=9)
+    assert r0.returncode == 0 and r1.returncode == 0, r1.stdout
+    _, _, X1, Y1, Z1 = _read_traj(tmp_path / "t.xyz")
+    from cellsim3d_amd import engine
+    cm = engine.cell_centres(X0.ravel(), Y0.ravel(), Z0.ravel(), 9)
+    for A0, A1, c in zip((X0, Y0, Z0), (X1, Y1, Z1), cm):
+        acc = np.float32(0)
+        for v in c:
+            acc = np.float32(acc + v)
+        sc = np.float32(acc / np.float32(9))
+        assert np.array_equal(A1[:, :180], (A0[:, :180] - sc).astype(np.float32))
+        assert np.all(A1[:, 180:] == 0)
+        assert abs(np.asarray(A1[:, :180], np.float64).mean()) < 1e-5
+
+
+@pytest.mark.parametrize("walls,code", [
+    ({"useWalls": 1, "perpAxis": "x", "dAxis": 2.0}, 255),   # only z walls exist
+    ({"useWalls": 1, "perpAxis": "z", "dAxis": 0.01}, 255),  # the gap is narrower than the cells
+    ({"useWalls": 1, "perpAxis": "Z", "dAxis": 5.0}, 0),
+])
+def test_wall_setup_checks(tmp_path, walls, code):
+    """GPUbounce.cu:808-861 (walls need a free box: the rigid box and walls exclude
+    each other)."""
+    def edit(d):
+        d["walls"].update(walls)
+        d["boxParams"]["useRigidSimulationBox"] = 0
+    r, _ = _setup(tmp_path, edit, n=9)
+    assert r.returncode == code, r.stdout

@@ -1,0 +1,100 @@
+"""The celldiv host driver end to end on the GPU: mitotic-index file
+(GPUbounce.cu:1288-1302, 1515-1547), soft cells chosen when growth ends
+(:1340-1429) and counting of internal cells only (:2537-2565).  The cell counts
+the driver books are checked against the counts in its own trajectory frames."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+import test_driver as D
+
+pytestmark = pytest.mark.gpu
+
+CELLDIV = os.path.join(ROOT, "cellsim3d_amd", "celldiv")
+
+
+def _run(tmp_path, edit, n):
+    from cellsim3d_amd import configs
+    v0 = configs.mesh_volume(configs.unit_mesh())
+
+    def full_edit(d):
+        d["core"].update(division_Vol=float(2.0 * v0), div_time_steps=600, non_div_time_steps=50, trajWriteInt=2,
+                         MaxNoofC180s=256)
+        d["counting"].update({"cell_count_int": 100, "overwrite_mit_ind_file?": 1})
+        edit(d)
+    d = json.loads(D._strip_comments(D.INP))
+    full_edit(d)
+    (tmp_path / "inp.json").write_text(json.dumps(d))
+    from cellsim3d_amd import mesh as M
+    M.write_reference_files(str(tmp_path), configs.unit_mesh())
+    print("celldiv %d cells in %s" % (n, tmp_path), flush=True)
+    r = subprocess.run([CELLDIV, str(n), "inp.json", "0"], cwd=str(tmp_path), capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    hdr, frames = D._read_frames(tmp_path / "t.xyz")
+    counts = {f[0]: f[2] for f in frames}
+    mi = [float(x) for x in (tmp_path / "m.dat").read_text().split()]
+    return d, frames, counts, mi, r.stdout
+
+
+def _expected_mi(counts, steps=600, every=100):
+    out = []
+    for k in range(1, steps // every + 1):
+        new = counts[k * every] - counts[(k - 1) * every]
+        tot = counts[k * every] - new
+        out.append(float("%f" % (np.float32(new) / np.float32(tot))) if new and tot else 0.0)
+    return out + [0.0]
+
+
+def test_mitotic_index_and_soft_cells_after_growth(tmp_path):
+    def edit(d):
+        d["stiffnessParams"].update(useDifferentStiffnesses=1, duringGrowth=0, closenessToCenter=0.01,
+                                    numberOfSofterCells=2, fractionOfSofterCells=0.0, chooseRandomCellIndices=0,
+                                    daughtSameStiffness=1)
+    d, frames, counts, mi, out = _run(tmp_path, edit, 4)
+    assert counts[600] > counts[0], "no division: the test does not exercise the counting"
+    assert mi == _expected_mi(counts)
+    # no soft cell at set-up (every centre is at the largest distance); at step 602
+    # the first two cells within (1 - 0.01) of the largest distance about the stale
+    # system centre (1, 1, 1) become soft
+    by_step = {f[0]: f for f in frames}
+    assert by_step[0][6].sum() == 0 and by_step[600][6].sum() == 0
+    from cellsim3d_amd import engine
+    step, _, n, X, Y, Z, _ = by_step[602]
+    cm = engine.cell_centres(X.ravel(), Y.ravel(), Z.ravel(), n)
+    r2max = np.float32(0)
+    for c in range(n):
+        dx, dy, dz = (np.float32(cm[a][c] - np.float32(1)) for a in range(3))
+        r2max = max(r2max, np.float32(np.float32(dx * dx + dy * dy) + dz * dz))
+    sc = [np.float32(0)] * 3
+    for a in range(3):
+        for c in range(n):
+            sc[a] = np.float32(sc[a] + cm[a][c])
+        sc[a] = np.float32(sc[a] / np.float32(n))
+    f = np.float32(1) - np.float32(0.01)
+    want = np.zeros(n, np.int32)
+    for c in range(n):
+        dx, dy, dz = (np.float32(cm[a][c] - sc[a]) for a in range(3))
+        if np.float32(np.float32(dx * dx + dy * dy) + dz * dz) <= np.float32(f * f) * r2max and want.sum() < 2:
+            want[c] = 1
+    assert want.sum() == 2
+    assert np.array_equal(by_step[604][6], want) and np.array_equal(frames[-1][6], want)
+    assert "Cell growth halted" in out
+
+
+def test_counting_only_internal_cells(tmp_path):
+    """With count_only_internal_cells? a new cell counts only if its centre is
+    within radius_cutoff of the largest distance; a daughter's centre is 0 in
+    the step that made it, so in a colony far from the origin every daughter is
+    far and the index stays 0, while plain counting sees the divisions."""
+    d, frames, counts, mi, _ = _run(tmp_path, lambda d: None, 64)
+    assert counts[600] > counts[0] >= 50
+    assert mi == _expected_mi(counts) and max(mi) > 0
+    d, frames, counts2, mi2, _ = _run(tmp_path, lambda d: d["counting"].update(
+        {"count_only_internal_cells?": 1, "radius_cutoff": 0.6}), 64)
+    assert counts2 == counts
+    assert mi2 == [0.0] * 7

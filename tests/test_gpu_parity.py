@@ -256,3 +256,30 @@ def test_random_force_matches_oracle(oracle_mesh, graph):
     assert not np.array_equal(s0.get(A.F_X)[:n * 192], s.get(A.F_X)[:n * 192])
     ctx0.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_correct_com_matches_oracle(oracle_mesh, graph):
+    """correct_com = 1 (GPUbounce.cu:1431-1441): the division phase sums the cell
+    centres in cell order and a sixth launch shifts every node; positions and the
+    system centre must agree with the oracle bit for bit, across a division."""
+    A = _abi()
+    from cellsim3d_amd import configs
+    cfg = configs.plain_small(seed=2)
+    p = cfg["params"]
+    p.division_vol = np.float32(1.02 * cfg["v0"])
+    p.correct_com = 1
+    n, mc = cfg["n"], 64
+    sim, ctx, s = _pair(oracle_mesh, p, cfg["X"], cfg["Y"], cfg["Z"], n, mc)
+    ctx.set_graph(graph)
+    for k in (1, 30, 270):
+        sim.step(k)
+        s.step(k)
+        ctx.sync()
+        n1, st1, status = s.info()
+        assert n1 == sim.n and st1 == sim.stepno and status & A.SYS_FATAL_MASK == 0
+        assert np.array_equal(s.get(A.F_SYSCM), sim.get(A.F_SYSCM)), "system centre at step %d" % st1
+        _cmp(sim, s, [A.F_X, A.F_Y, A.F_Z, A.F_VX, A.F_VY, A.F_VZ, A.F_CMX, A.F_PRESSURE], n1,
+             label="correct_com steps=%d" % st1)
+    assert sim.n > n, "no division: the test does not cover daughters"
+    ctx.close()

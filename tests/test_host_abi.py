@@ -148,3 +148,22 @@ def test_trajectory_byte_layout(tmp_path, var):
     engine.check(L.cs3d_traj_close(h, 3), "close")
     got = open(path, "rb").read()
     assert got == _traj_reference_bytes(10000, var, 7, frames, 3)
+
+
+def test_cell_centres_match_the_device_tree():
+    """cs3d_cell_centres (host) against the oracle's CenterOfMass restatement
+    (centermass.cu:8-50) bit for bit, and against the float64 mean."""
+    import oracle_kernels as K
+    rng = np.random.default_rng(2)
+    n = 7
+    X, Y, Z = (np.zeros(n * abi.SLOTS, np.float32) for _ in range(3))
+    for A, off in ((X, 3.0), (Y, -40.0), (Z, 1e3)):
+        A.reshape(n, abi.SLOTS)[:, :180] = (off + rng.normal(0, 2, (n, 180))).astype(np.float32)
+    got = engine.cell_centres(X, Y, Z, n)
+    ref = K.com(n, X, Y, Z)
+    for a in range(3):
+        assert np.array_equal(got[a], ref[a])
+        mean = np.asarray((X, Y, Z)[a], np.float64).reshape(n, abi.SLOTS)[:, :180].mean(1)
+        assert np.allclose(got[a], mean, rtol=1e-6, atol=1e-6)
+    with pytest.raises(ValueError):
+        engine.cell_centres(X[:100], Y, Z, n)

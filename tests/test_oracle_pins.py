@@ -686,3 +686,41 @@ def test_normals_are_standard_normal_and_reproducible():
     assert np.array_equal(O.normals3(7, 5, 11), O.normals3(7, 5, 11))
     assert not np.array_equal(O.normals3(7, 5, 11), O.normals3(7, 5, 12))
     assert not np.array_equal(O.normals3(7, 5, 11), O.normals3(8, 5, 11))
+
+
+# ------------------------------------------------------------------ correct_com
+def test_correct_com_shifts_every_node_by_the_mean_centre(gp30):
+    """GPUbounce.cu:1431-1441: at the end of each step the mean of the cell centres
+    (computed before the shift) is subtracted from every node.  In double precision:
+    the centres recomputed from the shifted nodes plus the system centre give back
+    the stored pre-shift centres, the stored centre is their mean, and the shifted
+    system is centred at 0."""
+    nn, sign, faces = M.tables(gp30)
+    from cellsim3d_amd import engine  # host mesh build only (no device)
+    mesh = engine.build_mesh((nn, sign, faces), configs.rest_positions(gp30))
+    cents = [(4.0, 5.0, 6.0), (9.0, 5.5, 6.0), (4.5, 10.0, 7.0)]
+    X, Y, Z = configs.place(gp30, cents)
+    n = len(cents)
+    p = configs.base_params(3)
+    p.correct_com = 1
+    p.growth_rate = 0.0
+    sim = O.Sim(p, mesh, X, Y, Z, n, max_cells=n, precision=1)
+    rng = np.random.default_rng(1)
+    for f in (abi.F_VX, abi.F_VY, abi.F_VZ):  # a drift so the centre moves
+        v = np.zeros(n * S)
+        v.reshape(n, S)[:, :180] = 0.5 + rng.normal(0, 0.05, (n, 180))
+        sim.set(f, v.astype(np.float32))
+    sim.init()
+    for k in range(3):
+        sim.step(1)
+        x = [sim.get64(f).reshape(n, S)[:, :180] for f in (abi.F_X, abi.F_Y, abi.F_Z)]
+        cm = [sim.get64(f)[:n] for f in (abi.F_CMX, abi.F_CMY, abi.F_CMZ)]
+        sc = sim.get(abi.F_SYSCM).astype(np.float64)
+        for a in range(3):
+            fresh = x[a].mean(1)
+            assert np.allclose(fresh + sc[a], cm[a], rtol=0, atol=1e-6), (k, a)
+            assert abs(cm[a].mean() - sc[a]) < 1e-6 * max(1.0, abs(sc[a]))
+            assert abs(fresh.mean()) < 1e-6
+        assert np.all(sim.get(abi.F_X).reshape(n, S)[:, 180:] == 0)
+        if k == 0:
+            assert np.all(np.abs(sc) > 1.0)  # the first step moves the system to the origin
