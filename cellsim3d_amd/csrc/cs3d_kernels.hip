@@ -226,6 +226,17 @@ __global__ __launch_bounds__(192) void k_integrate(Pool p, Range r, int32_t flag
     const int tid = threadIdx.x;
     const bool node = tid < kNodes;
     for (int slot = r.slot_lo + blockIdx.x; slot < r.slot_hi; slot += gridDim.x) {
+        // the cell's node data are loaded before the system checks (every slot of the
+        // range is allocated), so their latency overlaps the metadata reads
+        const int64_t gi = (int64_t)slot * kSlots + tid;
+        float x = p.X[gi], y = p.Y[gi], z = p.Z[gi];
+        const bool kick = flags & I_KICK;
+        float vx = 0.f, vy = 0.f, vz = 0.f, fcx = 0.f, fcy = 0.f, fcz = 0.f, fdx = 0.f, fdy = 0.f, fdz = 0.f;
+        if (kick) {
+            vx = p.VX[gi]; vy = p.VY[gi]; vz = p.VZ[gi];
+            fcx = p.FCX[gi]; fcy = p.FCY[gi]; fcz = p.FCZ[gi];
+            fdx = p.FDX[gi]; fdy = p.FDY[gi]; fdz = p.FDZ[gi];
+        }
         const int s = p.slot_sys[slot];
         if (s < 0) continue;
         SysMeta* M = p.meta + s;
@@ -233,7 +244,6 @@ __global__ __launch_bounds__(192) void k_integrate(Pool p, Range r, int32_t flag
         const int c = slot - M->base;
         if (c >= M->n) continue;
         const cs3d_params* P = p.prm + s;
-        const int64_t gi = (int64_t)slot * kSlots + tid;
         if ((flags & I_GRIDCOUNT) && M->rebuild && !(M->status & CS3D_SYS_GRID_TOO_LARGE)) {
             // the grid was rebuilt at the end of the last step: this cell's new bbox
             // becomes the one the contact rule uses, and its voxels are counted
@@ -257,15 +267,13 @@ __global__ __launch_bounds__(192) void k_integrate(Pool p, Range r, int32_t flag
             if (__any(over) && (tid & 63) == __ffsll((unsigned long long)__ballot(over)) - 1)
                 atomicOr(&M->status, (uint32_t)CS3D_SYS_VOXEL_OVERFLOW);
         }
-        float x = p.X[gi], y = p.Y[gi], z = p.Z[gi];
-        if (node && (flags & I_KICK)) {
+        if (node && kick) {
             const float dt = P->dt, m = P->mass;
             const float rdt = sqrtf(dt);
-            float vx = p.VX[gi], vy = p.VY[gi], vz = p.VZ[gi];
             const bool rnd = P->add_rands != 0;  // the previous step's random force (d_fRanList)
-            vx = half_kick(vx, p.FCX[gi], p.FDX[gi], rnd ? p.FRX[gi] : 0.f, dt, rdt, m);
-            vy = half_kick(vy, p.FCY[gi], p.FDY[gi], rnd ? p.FRY[gi] : 0.f, dt, rdt, m);
-            vz = half_kick(vz, p.FCZ[gi], p.FDZ[gi], rnd ? p.FRZ[gi] : 0.f, dt, rdt, m);
+            vx = half_kick(vx, fcx, fdx, rnd ? p.FRX[gi] : 0.f, dt, rdt, m);
+            vy = half_kick(vy, fcy, fdy, rnd ? p.FRY[gi] : 0.f, dt, rdt, m);
+            vz = half_kick(vz, fcz, fdz, rnd ? p.FRZ[gi] : 0.f, dt, rdt, m);
             x = x + vx * dt;
             y = y + vy * dt;
             z = z + vz * dt;
@@ -643,6 +651,13 @@ __global__ __launch_bounds__(192) void k_con_dis(Pool p, Range r, int32_t mode, 
     const bool forces = mode != M_PAIRS;
     const bool search = mode != M_BONDED;
     for (int slot = r.slot_lo + blockIdx.x; slot < r.slot_hi; slot += gridDim.x) {
+        // node data first (every slot of the range is allocated): their latency
+        // overlaps the metadata reads
+        const int64_t gi = (int64_t)slot * kSlots + tid;
+        const float X = rX[gi], Y = rY[gi], Z = rZ[gi];
+        float vX = 0.f, vY = 0.f, vZ = 0.f;
+        if (fused) { vX = rVX[gi]; vY = rVY[gi]; vZ = rVZ[gi]; }
+        const int nc_pre = search ? p.NCAND[gi] : 0;
         const int s = p.slot_sys[slot];
         if (s < 0) continue;
         SysMeta* M = p.meta + s;
@@ -650,10 +665,8 @@ __global__ __launch_bounds__(192) void k_con_dis(Pool p, Range r, int32_t mode, 
         const int c = slot - M->base;
         if (c >= M->n) continue;
         const cs3d_params* P = p.prm + s;
-        const int64_t gi = (int64_t)slot * kSlots + tid;
-        const float X = rX[gi], Y = rY[gi], Z = rZ[gi];
         sx[tid] = X; sy[tid] = Y; sz[tid] = Z;
-        if (fused) { svx[tid] = rVX[gi]; svy[tid] = rVY[gi]; svz[tid] = rVZ[gi]; }
+        if (fused) { svx[tid] = vX; svy[tid] = vY; svz[tid] = vZ; }
         __syncthreads();
         uint32_t st = 0;
         const bool angles = forces && P->constrain_angles;
@@ -720,7 +733,7 @@ __global__ __launch_bounds__(192) void k_con_dis(Pool p, Range r, int32_t mode, 
             }
         }
         if (search && node) {
-            const int nc = p.NCAND[gi];
+            const int nc = nc_pre;
             const int32_t* cand = p.CAND + (int64_t)slot * kPMax * kSlots + tid;
             const RuleCtx R = rule_ctx(M, P);
             PairConsts k;
@@ -935,6 +948,18 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
     const bool do_dis = mode == M_STEP || mode == M_DIS_V || mode == M_DIS_VA;
     const bool fused = mode == M_STEP;
     for (int slot = r.slot_lo + blockIdx.x; slot < r.slot_hi; slot += gridDim.x) {
+        // node data first (every slot of the range is allocated): their latency
+        // overlaps the metadata reads
+        const int64_t gi = (int64_t)slot * kSlots + tid;
+        const float X = rX[gi], Y = rY[gi], Z = rZ[gi];
+        float vX = 0.f, vY = 0.f, vZ = 0.f;
+        if (do_dis) { vX = rVX[gi]; vY = rVY[gi]; vZ = rVZ[gi]; }
+        const int np = (do_dis && node) ? p.NPAIR[gi] : 0;
+        float fcX = 0.f, fcY = 0.f, fcZ = 0.f, xbX = 0.f, xbY = 0.f, xbZ = 0.f;
+        if (fused && do_dis && node) {
+            fcX = p.FCX[gi]; fcY = p.FCY[gi]; fcZ = p.FCZ[gi];
+            xbX = p.XBX[gi]; xbY = p.XBY[gi]; xbZ = p.XBZ[gi];
+        }
         const int s = p.slot_sys[slot];
         if (s < 0) continue;
         SysMeta* M = p.meta + s;
@@ -942,11 +967,8 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
         const int c = slot - M->base;
         if (c >= M->n) continue;
         const cs3d_params* P = p.prm + s;
-        const int64_t gi = (int64_t)slot * kSlots + tid;
-        const float X = rX[gi], Y = rY[gi], Z = rZ[gi];
         sx[tid] = X; sy[tid] = Y; sz[tid] = Z;
-        if (do_dis) { svx[tid] = rVX[gi]; svy[tid] = rVY[gi]; svz[tid] = rVZ[gi]; }
-        const int np = (do_dis && node) ? p.NPAIR[gi] : 0;
+        if (do_dis) { svx[tid] = vX; svy[tid] = vY; svz[tid] = vZ; }
         __syncthreads();
         uint32_t st = 0;
         bool far = false;
@@ -985,9 +1007,9 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
                 // again next step when that is pair_skin/2 or more from the last search
                 const float dt = P->dt, m = P->mass, rdt = sqrtf(dt);
                 const bool rnd = P->add_rands != 0;
-                const float qx = X + half_kick(vix, p.FCX[gi], fx, rnd ? p.FRX[gi] : 0.f, dt, rdt, m) * dt - p.XBX[gi];
-                const float qy = Y + half_kick(viy, p.FCY[gi], fy, rnd ? p.FRY[gi] : 0.f, dt, rdt, m) * dt - p.XBY[gi];
-                const float qz = Z + half_kick(viz, p.FCZ[gi], fz, rnd ? p.FRZ[gi] : 0.f, dt, rdt, m) * dt - p.XBZ[gi];
+                const float qx = X + half_kick(vix, fcX, fx, rnd ? p.FRX[gi] : 0.f, dt, rdt, m) * dt - xbX;
+                const float qy = Y + half_kick(viy, fcY, fy, rnd ? p.FRY[gi] : 0.f, dt, rdt, m) * dt - xbY;
+                const float qz = Z + half_kick(viz, fcZ, fz, rnd ? p.FRZ[gi] : 0.f, dt, rdt, m) * dt - xbZ;
                 const float h = 0.5f * P->pair_skin;
                 far = !(qx * qx + qy * qy + qz * qz <= h * h);
             }

@@ -2,13 +2,16 @@
 // reference's CellDiv executable (src/GPUbounce.cu main, :289-1512).
 //
 //   celldiv <no_of_cells> <inp.json> <gpuid> [--grid-interval K] [--pair-skin S]
-//           [--mesh-dir DIR] [--no-graph] [--daughter-com-fix]
+//           [--mesh-dir DIR] [--no-graph] [--daughter-com-fix] [--setup-only]
+//           [--sync-output]
 //
 // Same command line, parameter file, input mesh files (C180, C180NN, C180C,
 // C180_pentahexa in the working directory or --mesh-dir) and outputs: the
 // trajectory (binary or ASCII, core.trajFileName), the forces CSV
 // (core.forces_file when core.write_cont_force), and the empty psfil and
-// velocity2.xyz the reference creates.  The random force (add_rands) draws from a
+// velocity2.xyz the reference creates.  Frames are copied off the device at
+// each trajectory step and written by a writer thread while the device steps on
+// (two frame buffers; --sync-output writes in place).  The random force (add_rands) draws from a
 // counter-based Philox stream instead of the reference's cuRAND states.  All
 // simulation work runs on the GPU
 // through the C ABI of libcs3d (include/cs3d.h); this file only parses,
@@ -21,6 +24,8 @@
 // traps) print the system status and exit with 0, as CudaErrorCheck does
 // (:39-45).
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -28,8 +33,10 @@
 #include <cstring>
 #include <ctime>
 #include <fstream>
+#include <mutex>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/cs3d.h"
@@ -566,6 +573,14 @@ bool get_field(cs3d_ctx* ctx, cs3d_sys sys, cs3d_field f, void* host, size_t byt
     return true;
 }
 
+// One output frame copied off the device: positions (and stiffness types), and
+// for the forces CSV the conservative force, velocity, pressure and volume.
+struct Frame {
+    int step = 0, frame = 0, n = 0;
+    bool positions = true;  // false: forces CSV only (step 0, written after frame 0)
+    std::vector<float> X, Y, Z, E, FX, FY, FZ, VX, VY, VZ, P, V;
+};
+
 // write_traj (GPUbounce.cu:2412-2444) / WriteBinaryTraj (:2446-2477) of one frame;
 // with variable stiffness each cell carries its type (0/H: stiffness1, 1/C: other)
 bool write_positions(const Settings& s, Output& out, int step, int frame, int n, const std::vector<float>& X,
@@ -596,43 +611,103 @@ bool write_positions(const Settings& s, Output& out, int step, int frame, int n,
 
 // writeForces (GPUbounce.cu:2567-2598): total conservative force, velocity,
 // position, pressure and volume of every node
-bool write_forces(cs3d_ctx* ctx, cs3d_sys sys, Output& out, int step, int n, const std::vector<float>& X,
-                  const std::vector<float>& Y, const std::vector<float>& Z) {
-    if (out.forces) {
-        std::vector<float> FX((size_t)n * S), FY((size_t)n * S), FZ((size_t)n * S);
-        std::vector<float> VX((size_t)n * S), VY((size_t)n * S), VZ((size_t)n * S), P(n), V(n);
-        if (n > 0 && (!get_field(ctx, sys, CS3D_F_FCX, FX.data(), FX.size() * 4) ||
-                      !get_field(ctx, sys, CS3D_F_FCY, FY.data(), FY.size() * 4) ||
-                      !get_field(ctx, sys, CS3D_F_FCZ, FZ.data(), FZ.size() * 4) ||
-                      !get_field(ctx, sys, CS3D_F_VX, VX.data(), VX.size() * 4) ||
-                      !get_field(ctx, sys, CS3D_F_VY, VY.data(), VY.size() * 4) ||
-                      !get_field(ctx, sys, CS3D_F_VZ, VZ.data(), VZ.size() * 4) ||
-                      !get_field(ctx, sys, CS3D_F_PRESSURE, P.data(), P.size() * 4) ||
-                      !get_field(ctx, sys, CS3D_F_VOLUME, V.data(), V.size() * 4)))
-            return false;
-        for (int c = 0; c < n; ++c)
-            for (int q = 0; q < NN; ++q) {
-                const size_t g = (size_t)c * S + q;
-                const float fm = sqrtf(FX[g] * FX[g] + FY[g] * FY[g] + FZ[g] * FZ[g]);
-                const float vm = sqrtf(VX[g] * VX[g] + VY[g] * VY[g] + VZ[g] * VZ[g]);
-                std::fprintf(out.forces,
-                             "%d,%d,%d,%d,%d,%.8f,%.8f,%.8f,%.8f,%.8f,%.8f,%.8f,%.8f,%.8f,%.8f,%.8f,%.8f,%.8f\n", step,
-                             n, c, q, c * 192 + q, FX[g], FY[g], FZ[g], fm, VX[g], VY[g], VZ[g], vm, X[g], Y[g], Z[g],
-                             P[c], V[c]);
-            }
-    }
+void write_forces(Output& out, const Frame& f) {
+    if (!out.forces) return;
+    for (int c = 0; c < f.n; ++c)
+        for (int q = 0; q < NN; ++q) {
+            const size_t g = (size_t)c * S + q;
+            const float fm = sqrtf(f.FX[g] * f.FX[g] + f.FY[g] * f.FY[g] + f.FZ[g] * f.FZ[g]);
+            const float vm = sqrtf(f.VX[g] * f.VX[g] + f.VY[g] * f.VY[g] + f.VZ[g] * f.VZ[g]);
+            std::fprintf(out.forces,
+                         "%d,%d,%d,%d,%d,%.8f,%.8f,%.8f,%.8f,%.8f,%.8f,%.8f,%.8f,%.8f,%.8f,%.8f,%.8f,%.8f\n", f.step,
+                         f.n, c, q, c * 192 + q, f.FX[g], f.FY[g], f.FZ[g], fm, f.VX[g], f.VY[g], f.VZ[g], vm, f.X[g],
+                         f.Y[g], f.Z[g], f.P[c], f.V[c]);
+        }
+}
+
+bool write_out(const Settings& s, Output& out, const Frame& f) {
+    if (f.positions && !write_positions(s, out, f.step, f.frame, f.n, f.X, f.Y, f.Z, f.E)) return false;
+    write_forces(out, f);
     return true;
 }
 
-bool write_frame(cs3d_ctx* ctx, cs3d_sys sys, const Settings& s, Output& out, int step, int frame, int n) {
-    std::vector<float> X((size_t)n * S), Y((size_t)n * S), Z((size_t)n * S), E(std::max(n, 1));
-    if (n > 0 && (!get_field(ctx, sys, CS3D_F_X, X.data(), X.size() * 4) ||
-                  !get_field(ctx, sys, CS3D_F_Y, Y.data(), Y.size() * 4) ||
-                  !get_field(ctx, sys, CS3D_F_Z, Z.data(), Z.size() * 4) ||
-                  (s.p.use_different_stiffnesses && !get_field(ctx, sys, CS3D_F_STIFFNESS, E.data(), (size_t)n * 4))))
-        return false;
-    return write_positions(s, out, step, frame, n, X, Y, Z, E) && write_forces(ctx, sys, out, step, n, X, Y, Z);
+// Copies one frame's fields to the host (the device work stays queued behind it).
+bool fetch_frame(cs3d_ctx* ctx, cs3d_sys sys, const Settings& s, const Output& out, int step, int frame, int n,
+                 Frame& f) {
+    f.step = step; f.frame = frame; f.n = n; f.positions = true;
+    const size_t nn = (size_t)n * S;
+    auto get = [&](cs3d_field fld, std::vector<float>& v, size_t count) {
+        v.resize(std::max<size_t>(count, 1));
+        return count == 0 || get_field(ctx, sys, fld, v.data(), count * 4);
+    };
+    bool ok = get(CS3D_F_X, f.X, nn) && get(CS3D_F_Y, f.Y, nn) && get(CS3D_F_Z, f.Z, nn);
+    if (ok && s.p.use_different_stiffnesses) ok = get(CS3D_F_STIFFNESS, f.E, n);
+    if (ok && out.forces)
+        ok = get(CS3D_F_FCX, f.FX, nn) && get(CS3D_F_FCY, f.FY, nn) && get(CS3D_F_FCZ, f.FZ, nn) &&
+             get(CS3D_F_VX, f.VX, nn) && get(CS3D_F_VY, f.VY, nn) && get(CS3D_F_VZ, f.VZ, nn) &&
+             get(CS3D_F_PRESSURE, f.P, n) && get(CS3D_F_VOLUME, f.V, n);
+    return ok;
 }
+
+// Output on a writer thread with two frame buffers: the main thread copies frame
+// k + 1 off the device and steps on while frame k is formatted and written; frames
+// reach the files in order.  With sync = true every frame is written in place.
+class FrameWriter {
+  public:
+    FrameWriter(const Settings& s, Output& out, bool sync) : s_(s), out_(out), sync_(sync) {
+        if (!sync_) th_ = std::thread([this] { run(); });
+    }
+    ~FrameWriter() { finish(); }
+    // takes the frame's buffers; blocks while the previous frame is still pending
+    void submit(Frame&& f) {
+        if (sync_) {
+            if (!write_out(s_, out_, f)) failed_ = true;
+            return;
+        }
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [this] { return !has_; });
+        next_ = std::move(f);
+        has_ = true;
+        cv_.notify_all();
+    }
+    // waits until every submitted frame is written; true when all writes succeeded
+    bool finish() {
+        if (!sync_ && th_.joinable()) {
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                stop_ = true;
+            }
+            cv_.notify_all();
+            th_.join();
+        }
+        return !failed_;
+    }
+
+  private:
+    void run() {
+        for (;;) {
+            Frame f;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [this] { return has_ || stop_; });
+                if (!has_) return;
+                f = std::move(next_);
+                has_ = false;
+            }
+            cv_.notify_all();
+            if (!write_out(s_, out_, f)) failed_ = true;
+        }
+    }
+    const Settings& s_;
+    Output& out_;
+    bool sync_;
+    std::thread th_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    Frame next_;
+    bool has_ = false, stop_ = false;
+    std::atomic<bool> failed_{false};
+};
 
 bool get_centres(cs3d_ctx* ctx, cs3d_sys sys, int n, std::vector<float>& cx, std::vector<float>& cy,
                  std::vector<float>& cz) {
@@ -740,7 +815,7 @@ int main(int argc, char** argv) {
     std::printf("celldiv (C180 cell mechanics on AMD Instinct, libcs3d %s)\n", cs3d_version());
     if (argc < 4) {
         std::printf("Usage: celldiv no_of_cells inpFile.json gpuid [--grid-interval K] [--pair-skin S] "
-                    "[--mesh-dir DIR] [--no-graph] [--daughter-com-fix] [--setup-only]\n");
+                    "[--mesh-dir DIR] [--no-graph] [--daughter-com-fix] [--setup-only] [--sync-output]\n");
         return 0;
     }
     const int n_cells = std::atoi(argv[1]);
@@ -748,7 +823,7 @@ int main(int argc, char** argv) {
     int grid_interval = 1;
     float pair_skin = -1.f;
     std::string mesh_dir = ".";
-    bool graph = true, com_fix = false, setup_only = false;
+    bool graph = true, com_fix = false, setup_only = false, sync_output = false;
     for (int a = 4; a < argc; ++a) {
         const std::string o = argv[a];
         if (o == "--grid-interval" && a + 1 < argc) grid_interval = std::atoi(argv[++a]);
@@ -757,6 +832,7 @@ int main(int argc, char** argv) {
         else if (o == "--no-graph") graph = false;
         else if (o == "--daughter-com-fix") com_fix = true;
         else if (o == "--setup-only") setup_only = true;
+        else if (o == "--sync-output") sync_output = true;
         else {
             std::printf("unknown option %s\n", o.c_str());
             return -1;
@@ -910,7 +986,13 @@ int main(int argc, char** argv) {
     };
     int n = n_cells;
     int rcode = 0;
-    if (!write_forces(ctx, sys, out, 0, n, X, Y, Z)) rcode = -1;
+    FrameWriter writer(s, out, sync_output);
+    if (out.forces) {  // the forces of the start-up sequence, step 0
+        Frame f0;
+        if (!fetch_frame(ctx, sys, s, out, 0, frame, n, f0)) rcode = -1;
+        f0.positions = false;
+        if (rcode == 0) writer.submit(std::move(f0));
+    }
     int step = 0;
     while (rcode == 0 && step < total) {
         const int next = next_stop(step);
@@ -977,8 +1059,14 @@ int main(int argc, char** argv) {
         }
         if (step % s.traj_write_int == 0) {
             ++frame;
-            if (!write_frame(ctx, sys, s, out, step, frame, n)) rcode = -1;
+            Frame f;
+            if (!fetch_frame(ctx, sys, s, out, step, frame, n, f)) rcode = -1;
+            else writer.submit(std::move(f));
         }
+    }
+    if (!writer.finish()) {
+        std::printf("ERROR: writing the output files failed\n");
+        if (rcode == 0) rcode = -1;
     }
     if (out.bin) cs3d_traj_close(out.bin, n);
     if (out.ascii) std::fclose(out.ascii);

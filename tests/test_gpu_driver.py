@@ -98,3 +98,34 @@ def test_counting_only_internal_cells(tmp_path):
         {"count_only_internal_cells?": 1, "radius_cutoff": 0.6}), 64)
     assert counts2 == counts
     assert mi2 == [0.0] * 7
+
+
+@pytest.mark.parametrize("binary", [1, 0])
+def test_output_thread_writes_the_same_bytes(tmp_path, binary):
+    """Frames written by the writer thread (the default) and in place
+    (--sync-output) are byte-identical: trajectory (binary or ASCII) and forces
+    CSV, frames in order."""
+    from cellsim3d_amd import configs
+    from cellsim3d_amd import mesh as M
+    v0 = configs.mesh_volume(configs.unit_mesh())
+    d = json.loads(D._strip_comments(D.INP))
+    d["core"].update(division_Vol=float(1.05 * v0), div_time_steps=200, non_div_time_steps=0, trajWriteInt=20,
+                     write_cont_force=1, binaryOutput=binary)
+    outs = []
+    for flag in ([], ["--sync-output"]):
+        wd = tmp_path / ("sync" if flag else "async")
+        wd.mkdir()
+        (wd / "inp.json").write_text(json.dumps(d))
+        M.write_reference_files(str(wd), configs.unit_mesh())
+        print("celldiv 9 cells %s" % (flag or "async"), flush=True)
+        r = subprocess.run([CELLDIV, "9", "inp.json", "0"] + flag, cwd=str(wd), capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stdout[-2000:]
+        outs.append(((wd / "t.xyz").read_bytes(), (wd / "f.csv").read_bytes()))
+    assert outs[0] == outs[1]
+    lines = outs[0][1].decode().splitlines()
+    steps = sorted({int(l.split(",")[0]) for l in lines[1:]})
+    assert steps == list(range(0, 201, 20))
+    if binary:
+        _, frames = D._read_frames(tmp_path / "async" / "t.xyz")
+        assert [f[0] for f in frames] == list(range(0, 201, 20))
