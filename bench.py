@@ -130,7 +130,7 @@ def measure_extras(args, gm, log):
         "autotuned": {"wg_per_cu": wg, "graph_steps": gs},
     }
     log("bench: ensemble batched %.3f s, plain %.3f s (x%.1f)" % (dt_b, dt_p, dt_p / dt_b))
-    reqs = configs.serving_requests(args.serving_requests, 1, 20.0, 8, 4096, 1e3, 1e4)
+    reqs = configs.serving_requests(args.serving_requests, 1, 20.0, 8, 4096, 1e3, 1e5)
     jobs = serving.make_jobs(reqs)
     srv = serving.Server(g, 200e9, max_cells_total=100000, chunk_steps=50)
     wall = srv.run(jobs)
@@ -139,12 +139,13 @@ def measure_extras(args, gm, log):
     lat = np.array([j.finished - j.arrival for j in jobs])
     offered = sum(j.n * 180.0 * j.horizon for j in jobs) / jobs[-1].arrival
     out["serving"] = {
-        "workload": "%d requests, Poisson 20/s, N ~ log-U[8, 4096], horizon ~ log-U[1e3, 1e4] steps, FP32, "
+        "workload": "%d requests, Poisson 20/s, N ~ log-U[8, 4096], horizon ~ log-U[1e3, 1e5] steps, FP32, "
                     "growth off, 200 GB budget" % len(jobs),
         "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
         "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
         "node_updates_per_s": srv.node_updates / wall,
-        "utilisation": round(offered / (srv.node_updates / wall), 3),
+        # offered node-updates/s over served node-updates/s: above 1 the requests queue
+        "offered_load": round(offered / (srv.node_updates / wall), 3),
     }
     log("bench: serving p50 %.3f s p99 %.3f s" % (out["serving"]["p50_latency_s"], out["serving"]["p99_latency_s"]))
     return out

@@ -14,7 +14,7 @@ runs alone or in the mix (tests/test_gpu_serving.py checks this bit for bit).
 
     python -m cellsim3d_amd.serving [--requests 64] [--rate 20] ...
 prints one JSON line: p50/p99 completion latency, aggregate node-updates/s and
-the utilisation of the offered load.
+the offered load (offered over served node-updates/s; above 1 requests queue).
 """
 import argparse
 import json
@@ -191,7 +191,7 @@ def main(argv=None):
         "wall_s": wall,
         "node_updates_per_s": srv.node_updates / wall,
         "offered_node_updates_per_s": offered / span if span > 0 else None,
-        "utilisation": (offered / span) / (srv.node_updates / wall) if span > 0 else None,
+        "offered_load": (offered / span) / (srv.node_updates / wall) if span > 0 else None,
     }
     print(json.dumps(out), flush=True)
     return out
