@@ -55,7 +55,19 @@ __device__ __forceinline__ int vox1(float x, float mn, float DL, int div) {
     return s;
 }
 
-// bin coordinate of one axis (clamping keeps |i - j| <= 1 for nodes within one bin edge)
+// XCD-aware first cell of a workgroup.  The dispatcher hands workgroup v to XCD
+// v mod 8; this maps the 8 XCDs to 8 contiguous runs of workgroup indices, so the
+// cells one XCD works on are neighbours in index and, for tissues placed in
+// order, in space: their partner gathers hit the XCD's own L2.  A bijection on
+// [0, gridDim.x).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block() {
+    const int G = gridDim.x, v = blockIdx.x;
+    const int q = G / kXcds, rem = G % kXcds;
+    const int k = v % kXcds, j = v / kXcds;
+    return k * q + min(k, rem) + j;
+}
+
 // Search bins are periodic: bin coordinate floor((x - origin) / h) modulo the bin
 // count of the axis.  Regions farther apart than the bin grid alias into the same
 // bins, which only adds entries the distance test rejects, so the bin edge never
@@ -225,7 +237,7 @@ __device__ int64_t set_grid_geometry(SysMeta* M, const cs3d_params* P, const flo
 __global__ __launch_bounds__(192) void k_integrate(Pool p, Range r, int32_t flags, int64_t step_override) {
     const int tid = threadIdx.x;
     const bool node = tid < kNodes;
-    for (int slot = r.slot_lo + blockIdx.x; slot < r.slot_hi; slot += gridDim.x) {
+    for (int slot = r.slot_lo + xcd_block(); slot < r.slot_hi; slot += gridDim.x) {
         // the cell's node data are loaded before the system checks (every slot of the
         // range is allocated), so their latency overlaps the metadata reads
         const int64_t gi = (int64_t)slot * kSlots + tid;
@@ -606,7 +618,7 @@ __device__ __forceinline__ int accept_forces(const int32_t* cand, int nc, int32_
 __global__ __launch_bounds__(192) void k_search(Pool p, Range r, int32_t always) {
     __shared__ int32_t lst[kPMax][kSlots];
     const int tid = threadIdx.x;
-    for (int slot = r.slot_lo + blockIdx.x; slot < r.slot_hi; slot += gridDim.x) {
+    for (int slot = r.slot_lo + xcd_block(); slot < r.slot_hi; slot += gridDim.x) {
         const int s = p.slot_sys[slot];
         if (s < 0) continue;
         SysMeta* M = p.meta + s;
@@ -650,7 +662,7 @@ __global__ __launch_bounds__(192) void k_con_dis(Pool p, Range r, int32_t mode, 
     const bool fused = mode == M_STEP;
     const bool forces = mode != M_PAIRS;
     const bool search = mode != M_BONDED;
-    for (int slot = r.slot_lo + blockIdx.x; slot < r.slot_hi; slot += gridDim.x) {
+    for (int slot = r.slot_lo + xcd_block(); slot < r.slot_hi; slot += gridDim.x) {
         // node data first (every slot of the range is allocated): their latency
         // overlaps the metadata reads
         const int64_t gi = (int64_t)slot * kSlots + tid;
@@ -947,7 +959,7 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
     const MeshDev* mesh = p.mesh;
     const bool do_dis = mode == M_STEP || mode == M_DIS_V || mode == M_DIS_VA;
     const bool fused = mode == M_STEP;
-    for (int slot = r.slot_lo + blockIdx.x; slot < r.slot_hi; slot += gridDim.x) {
+    for (int slot = r.slot_lo + xcd_block(); slot < r.slot_hi; slot += gridDim.x) {
         // node data first (every slot of the range is allocated): their latency
         // overlaps the metadata reads
         const int64_t gi = (int64_t)slot * kSlots + tid;
@@ -1445,7 +1457,7 @@ __global__ __launch_bounds__(256) void k_sys_divide(Pool p, Range r, int32_t mod
 __global__ __launch_bounds__(192) void k_com_shift(Pool p, Range r) {
     const int tid = threadIdx.x;
     if (tid >= kNodes) return;
-    for (int slot = r.slot_lo + blockIdx.x; slot < r.slot_hi; slot += gridDim.x) {
+    for (int slot = r.slot_lo + xcd_block(); slot < r.slot_hi; slot += gridDim.x) {
         const int s = p.slot_sys[slot];
         if (s < 0) continue;
         const SysMeta* M = p.meta + s;
@@ -1613,7 +1625,7 @@ __global__ __launch_bounds__(256) void k_sort(Pool p, Range r, int32_t force) {
 // the next step's Integrate): the new bboxes become current.  Then the voxel
 // counters of the current grid are cleared so that k_insert can fill the lists.
 __global__ __launch_bounds__(192) void k_grid_adopt(Pool p, Range r) {
-    for (int slot = r.slot_lo + blockIdx.x; slot < r.slot_hi; slot += gridDim.x) {
+    for (int slot = r.slot_lo + xcd_block(); slot < r.slot_hi; slot += gridDim.x) {
         const int s = p.slot_sys[slot];
         if (s < 0) continue;
         const SysMeta* M = p.meta + s;
@@ -1652,7 +1664,7 @@ __global__ __launch_bounds__(256) void k_vox_clear(Pool p, Range r) {
 // VelocityUpdateA then VelocityUpdateB (propagate.cu:714-736) on V (per-kernel entry point)
 __global__ __launch_bounds__(192) void k_velocity_ab(Pool p, Range r) {
     const int tid = threadIdx.x;
-    for (int slot = r.slot_lo + blockIdx.x; slot < r.slot_hi; slot += gridDim.x) {
+    for (int slot = r.slot_lo + xcd_block(); slot < r.slot_hi; slot += gridDim.x) {
         const int s = p.slot_sys[slot];
         if (s < 0) continue;
         SysMeta* M = p.meta + s;
