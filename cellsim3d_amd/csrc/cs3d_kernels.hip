@@ -1231,7 +1231,8 @@ __device__ __forceinline__ void block_minmax12(float* v, float (*part)[12], floa
     __syncthreads();
     if (tid < 12) {
         float b = part[0][tid];
-        for (int w = 1; w < 4; ++w) b = (tid & 1) ? fmaxf(b, part[w][tid]) : fminf(b, part[w][tid]);
+        const int nw = (int)(blockDim.x >> 6);
+        for (int w = 1; w < nw; ++w) b = (tid & 1) ? fmaxf(b, part[w][tid]) : fminf(b, part[w][tid]);
         out[tid] = b;
     }
     __syncthreads();
@@ -1251,12 +1252,10 @@ __global__ __launch_bounds__(256) void k_sys_divide(Pool p, Range r, int32_t mod
     if (s >= r.sys_hi) return;
     SysMeta* M = p.meta + s;
     const bool step_mode = mode == M_STEP;
-    if (!M->active) {
-        if (tid == 0 && step_mode) { M->rebuild = 0; M->shift_pending = 0; }
-        return;
-    }
-    if (step_mode && !sys_live(M)) {
-        if (tid == 0) { M->rebuild = 0; M->shift_pending = 0; }
+    // a stopped system keeps a pending grid adoption: its bboxes are adopted when it
+    // resumes or when a caller reads the grid
+    if (!M->active || (step_mode && !sys_live(M))) {
+        if (tid == 0 && step_mode) M->shift_pending = 0;
         return;
     }
     const cs3d_params* P = p.prm + s;

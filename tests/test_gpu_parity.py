@@ -283,3 +283,33 @@ def test_correct_com_matches_oracle(oracle_mesh, graph):
              label="correct_com steps=%d" % st1)
     assert sim.n > n, "no division: the test does not cover daughters"
     ctx.close()
+
+
+def test_resume_after_horizon_equals_uninterrupted(oracle_mesh):
+    """A system that stops at its horizon on a grid-rebuild step while others in
+    the batch go on, then resumes, continues bit-identically to a run without the
+    stop (the stopped system keeps its pending grid adoption)."""
+    A, E = _abi(), _engine()
+    pa, Xa, Ya, Za, na, mca = _case("tissue", n=27, seed=3)
+    pb, Xb, Yb, Zb, nb, mcb = _case("tissue", n=40, seed=4, phi=0.3)
+    for p_ in (pa, pb):
+        p_.grid_interval = 10
+    ctx = E.Context(oracle_mesh, capacity_cells=512, voxel_budget=1 << 24)
+    sa = ctx.add_system(pa, Xa, Ya, Za, na, max_cells=mca)
+    sb = ctx.add_system(pb, Xb, Yb, Zb, nb, max_cells=mcb)
+    sa.set_horizon(20)
+    ctx.step(35)  # a stops after step 20 (a rebuild step), b goes on to 35
+    ctx.sync()
+    assert sa.info()[1] == 20 and sb.info()[1] == 35
+    sa.set_horizon(-1)
+    sa.step(30)
+    ctx.sync()
+    ref = E.Context(oracle_mesh, capacity_cells=256, voxel_budget=1 << 22)
+    sr = ref.add_system(pa, Xa, Ya, Za, na, max_cells=mca)
+    sr.step(50)
+    ref.sync()
+    assert sa.info()[1] == 50 and sr.info()[1] == 50
+    for f in (A.F_X, A.F_Y, A.F_Z, A.F_VX, A.F_FCX, A.F_FDX, A.F_BBOX):
+        assert np.array_equal(sa.get(f)[:na * 192], sr.get(f)[:na * 192]), "field %d differs after the resume" % f
+    ctx.close()
+    ref.close()
