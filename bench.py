@@ -202,7 +202,8 @@ def main():
     value = n * 180.0 * args.steps / elapsed
     log("bench: %d steps in %.4f s -> %.4g node-updates/s" % (args.steps, elapsed, value))
 
-    # per-phase timing with HIP events on the engine's stream (eager launches)
+    # per-phase kernel time from start/stop HIP events each launch records on the
+    # engine's stream (eager steps; launch gaps are not counted)
     ph = ctx.step_timed(max(1, args.timed_steps), s) / max(1, args.timed_steps)
     phases = {name: float(ms) for name, ms in zip(engine.PHASE_NAMES, ph)}
     dom = max(("con_dis", "dis_post", "integrate"), key=lambda k: phases[k])

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "cs3d_dev.h"
@@ -21,6 +22,7 @@ void launch_dis(const Pool& p, const Range& r, int mode, int64_t step_override, 
                 hipStream_t st);
 void launch_bonded(const Pool& p, const Range& r, int angles, hipStream_t st);
 void launch_divide(const Pool& p, const Range& r, int mode, hipStream_t st);
+extern thread_local std::vector<std::pair<hipEvent_t, hipEvent_t>>* kernel_timer;
 void launch_com_shift(const Pool& p, const Range& r, hipStream_t st);
 void launch_grid(const Pool& p, const Range& r, int vox_blocks, hipStream_t st);
 void launch_vox_lists(const Pool& p, const Range& r, int vox_blocks, hipStream_t st);
@@ -844,29 +846,45 @@ extern "C" cs3d_status cs3d_set_graph_steps(cs3d_ctx* c, int32_t steps) {
     return CS3D_OK;
 }
 
+namespace {
+// Runs enqueue(q) for q in [0, groups) with every launch timed by its own
+// start/stop events (the launch helper's kernel timer) and adds each group's
+// summed kernel durations to ms[q].
+template <class Enqueue>
+cs3d_status timed_groups(cs3d_ctx* c, int groups, Enqueue enqueue, double* ms) {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
+    std::vector<size_t> mark(groups + 1, 0);
+    kernel_timer = &evs;
+    for (int q = 0; q < groups; ++q) {
+        mark[q] = evs.size();
+        enqueue(q);
+    }
+    mark[groups] = evs.size();
+    kernel_timer = nullptr;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    for (int q = 0; q < groups && e == hipSuccess; ++q)
+        for (size_t i = mark[q]; i < mark[q + 1]; ++i) {
+            float t = 0.f;
+            if (hipEventElapsedTime(&t, evs[i].first, evs[i].second) == hipSuccess) ms[q] += t;
+        }
+    for (auto& pr : evs) {
+        (void)hipEventDestroy(pr.first);
+        (void)hipEventDestroy(pr.second);
+    }
+    return e == hipSuccess ? CS3D_OK : fail(CS3D_HIP_ERROR, hipGetErrorString(e));
+}
+}  // namespace
+
+// Per-phase time of n_steps eager steps: each phase's kernel durations, from
+// events the launches themselves record (no launch gaps).
 extern "C" cs3d_status cs3d_step_timed(cs3d_ctx* c, cs3d_sys s, int32_t n_steps, double* phase_ms) {
     if (!c || !phase_ms) return fail(CS3D_INVALID_ARG, "NULL argument");
     if (s != CS3D_ALL && !valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
     const Range r = range_of(c, s);
     if (r.slot_hi <= r.slot_lo || r.sys_hi <= r.sys_lo) return CS3D_OK;
-    hipEvent_t ev[CS3D_PHASES + 1];
-    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     cs3d_status st = CS3D_OK;
-    for (int k = 0; k < n_steps && st == CS3D_OK; ++k) {
-        (void)hipEventRecord(ev[0], c->stream);
-        for (int q = 0; q < CS3D_PHASES; ++q) {
-            enqueue_step_phase(c, r, q);
-            (void)hipEventRecord(ev[q + 1], c->stream);
-        }
-        hipError_t e = hipEventSynchronize(ev[5]);
-        if (e != hipSuccess) { st = fail(CS3D_HIP_ERROR, hipGetErrorString(e)); break; }
-        for (int q = 0; q < CS3D_PHASES; ++q) {
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, ev[q], ev[q + 1]);
-            phase_ms[q] += ms;
-        }
-    }
-    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (int k = 0; k < n_steps && st == CS3D_OK; ++k)
+        st = timed_groups(c, CS3D_PHASES, [&](int q) { enqueue_step_phase(c, r, q); }, phase_ms);
     return st;
 }
 
@@ -999,19 +1017,10 @@ extern "C" cs3d_status cs3d_time_kernel(cs3d_ctx* c, cs3d_sys s, int32_t which, 
         if (which == 0) launch_bonded(c->pool, r, c->sys[s].prm.constrain_angles ? 1 : 0, c->stream);
         else launch_dis(c->pool, r, M_COM, 0, 0, false, c->stream);
     };
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
     launch();  // warm-up
-    (void)hipEventRecord(e0, c->stream);
-    for (int q = 0; q < reps; ++q) launch();
-    (void)hipEventRecord(e1, c->stream);
-    hipError_t e = hipEventSynchronize(e1);
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (e != hipSuccess) return fail(CS3D_HIP_ERROR, hipGetErrorString(e));
+    double ms = 0.0;
+    const cs3d_status st = timed_groups(c, 1, [&](int) { for (int q = 0; q < reps; ++q) launch(); }, &ms);
+    if (st != CS3D_OK) return st;
     *avg_ms = ms / reps;
     return CS3D_OK;
 }

@@ -22,8 +22,12 @@
 // widened by attraction_range (propagatebound.cu:224-259), so the same range test
 // on the same bbox and grid answers "b in list(v)".  The accepted pairs are
 // stored for the second friction pass.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <math.h>
+
+#include <utility>
+#include <vector>
 
 #include "cs3d_dev.h"
 
@@ -1704,6 +1708,25 @@ __global__ void k_set_flags(Pool p, Range r, int32_t rebuild, int32_t search) {
 // ------------------------------------------------------------------------------
 // launchers (called by cs3d_api.hip)
 // ------------------------------------------------------------------------------
+// Kernel timing for cs3d_step_timed: while kernel_timer is set, every launch
+// carries its own start/stop events (hipExtLaunchKernelGGL), so a phase's time is
+// the sum of its kernels' durations, without the launch gaps.
+thread_local std::vector<std::pair<hipEvent_t, hipEvent_t>>* kernel_timer = nullptr;
+
+template <class F, class... A>
+static void launch(F kernel, dim3 grid, dim3 block, hipStream_t st, A... args) {
+    if (kernel_timer) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+            hipExtLaunchKernelGGL(kernel, grid, block, 0, st, e0, e1, 0, args...);
+            kernel_timer->push_back({e0, e1});
+            return;
+        }
+        if (e0) (void)hipEventDestroy(e0);
+    }
+    hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
+}
+
 // Grid of a cell kernel: one workgroup per cell slot up to 256 CUs x wg_per_cu
 // workgroups; beyond that each workgroup strides over several cells.
 static int grid_cells(const Range& r, int per_cu) {
@@ -1713,72 +1736,72 @@ static int grid_cells(const Range& r, int per_cu) {
 }
 
 void launch_integrate(const Pool& p, const Range& r, int flags, int64_t step_override, hipStream_t st) {
-    hipLaunchKernelGGL(k_integrate, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), 0, st, p, r, flags, step_override);
+    launch(k_integrate, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), st, p, r, flags, step_override);
 }
 void launch_search(const Pool& p, const Range& r, int always, hipStream_t st) {
-    hipLaunchKernelGGL(k_search, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), 0, st, p, r, always);
+    launch(k_search, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), st, p, r, always);
 }
 void launch_con(const Pool& p, const Range& r, int mode, bool use_va, hipStream_t st) {
     const float* vx = use_va ? p.VAX : p.VX;
     const float* vy = use_va ? p.VAY : p.VY;
     const float* vz = use_va ? p.VAZ : p.VZ;
-    hipLaunchKernelGGL(k_con_dis, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), 0, st, p, r, mode, p.X, p.Y, p.Z, vx, vy, vz,
+    launch(k_con_dis, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), st, p, r, mode, p.X, p.Y, p.Z, vx, vy, vz,
                        p.BBOX);
 }
 // angles: 0 = systems without bending, 1 = with, -1 = both launches
 void launch_bonded(const Pool& p, const Range& r, int angles, hipStream_t st) {
-    if (angles <= 0) hipLaunchKernelGGL(k_bonded<false>, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), 0, st, p, r);
-    if (angles != 0) hipLaunchKernelGGL(k_bonded<true>, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), 0, st, p, r);
+    if (angles <= 0) launch(k_bonded<false>, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), st, p, r);
+    if (angles != 0) launch(k_bonded<true>, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), st, p, r);
 }
 void launch_dis(const Pool& p, const Range& r, int mode, int64_t step_override, int clear_bins, bool use_va,
                 hipStream_t st) {
     const float* vx = use_va ? p.VAX : p.VX;
     const float* vy = use_va ? p.VAY : p.VY;
     const float* vz = use_va ? p.VAZ : p.VZ;
-    hipLaunchKernelGGL(k_dis_post, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), 0, st, p, r, mode, step_override, clear_bins,
+    launch(k_dis_post, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), st, p, r, mode, step_override, clear_bins,
                        p.X, p.Y, p.Z, vx, vy, vz);
 }
 void launch_divide(const Pool& p, const Range& r, int mode, hipStream_t st) {
     const int ns = r.sys_hi - r.sys_lo;
-    hipLaunchKernelGGL(k_sys_divide, dim3(ns > 0 ? ns : 1), dim3(256), 0, st, p, r, mode);
+    launch(k_sys_divide, dim3(ns > 0 ? ns : 1), dim3(256), st, p, r, mode);
 }
 void launch_com_shift(const Pool& p, const Range& r, hipStream_t st) {
-    hipLaunchKernelGGL(k_com_shift, dim3(grid_cells(r, p.wg_per_cu)), dim3(kSlots), 0, st, p, r);
+    launch(k_com_shift, dim3(grid_cells(r, p.wg_per_cu)), dim3(kSlots), st, p, r);
 }
 void launch_grid(const Pool& p, const Range& r, int vox_blocks, hipStream_t st) {
     const int n = r.slot_hi - r.slot_lo;
     const int cb = (n + 255) / 256;
     const int bb = (n + kBoxCells - 1) / kBoxCells;
     const int ns = r.sys_hi - r.sys_lo;
-    hipLaunchKernelGGL(k_cellbox, dim3(bb > 0 ? bb : 1), dim3(192), 0, st, p, r);
-    hipLaunchKernelGGL(k_grid_dims, dim3(vox_blocks, ns > 0 ? ns : 1), dim3(256), 0, st, p, r);
-    hipLaunchKernelGGL(k_insert, dim3(cb > 0 ? cb : 1), dim3(256), 0, st, p, r, 0);
-    hipLaunchKernelGGL(k_sort, dim3(vox_blocks, ns > 0 ? ns : 1), dim3(256), 0, st, p, r, 0);
+    launch(k_cellbox, dim3(bb > 0 ? bb : 1), dim3(192), st, p, r);
+    launch(k_grid_dims, dim3(vox_blocks, ns > 0 ? ns : 1), dim3(256), st, p, r);
+    launch(k_insert, dim3(cb > 0 ? cb : 1), dim3(256), st, p, r, 0);
+    launch(k_sort, dim3(vox_blocks, ns > 0 ? ns : 1), dim3(256), st, p, r, 0);
 }
 void launch_vrange(const Pool& p, const Range& r, hipStream_t st) {
     const int n = r.slot_hi - r.slot_lo;
-    hipLaunchKernelGGL(k_vrange, dim3(n > 0 ? (n + 255) / 256 : 1), dim3(256), 0, st, p, r);
+    launch(k_vrange, dim3(n > 0 ? (n + 255) / 256 : 1), dim3(256), st, p, r);
 }
 // voxel lists of the current grid (the fused step keeps only the counts)
 void launch_vox_lists(const Pool& p, const Range& r, int vox_blocks, hipStream_t st) {
     const int n = r.slot_hi - r.slot_lo;
     const int cb = (n + 255) / 256;
     const int ns = r.sys_hi - r.sys_lo;
-    hipLaunchKernelGGL(k_grid_adopt, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), 0, st, p, r);
-    hipLaunchKernelGGL(k_vox_clear, dim3(vox_blocks, ns > 0 ? ns : 1), dim3(256), 0, st, p, r);
-    hipLaunchKernelGGL(k_insert, dim3(cb > 0 ? cb : 1), dim3(256), 0, st, p, r, 1);
-    hipLaunchKernelGGL(k_sort, dim3(vox_blocks, ns > 0 ? ns : 1), dim3(256), 0, st, p, r, 1);
+    launch(k_grid_adopt, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), st, p, r);
+    launch(k_vox_clear, dim3(vox_blocks, ns > 0 ? ns : 1), dim3(256), st, p, r);
+    launch(k_insert, dim3(cb > 0 ? cb : 1), dim3(256), st, p, r, 1);
+    launch(k_sort, dim3(vox_blocks, ns > 0 ? ns : 1), dim3(256), st, p, r, 1);
 }
 void launch_velocity_ab(const Pool& p, const Range& r, hipStream_t st) {
-    hipLaunchKernelGGL(k_velocity_ab, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), 0, st, p, r);
+    launch(k_velocity_ab, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), st, p, r);
 }
 void launch_set_flags(const Pool& p, const Range& r, int rebuild, int search, hipStream_t st) {
     const int ns = r.sys_hi - r.sys_lo;
-    hipLaunchKernelGGL(k_set_flags, dim3((ns + 255) / 256 > 0 ? (ns + 255) / 256 : 1), dim3(256), 0, st, p, r, rebuild,
+    launch(k_set_flags, dim3((ns + 255) / 256 > 0 ? (ns + 255) / 256 : 1), dim3(256), st, p, r, rebuild,
                        search);
 }
 void launch_copy_calibrate(const void* src, void* dst, int64_t n4, hipStream_t st) {
-    hipLaunchKernelGGL(k_copy_calibrate, dim3(2048), dim3(256), 0, st, (const float4*)src, (float4*)dst, n4);
+    launch(k_copy_calibrate, dim3(2048), dim3(256), st, (const float4*)src, (float4*)dst, n4);
 }
 
 }  // namespace cs3d

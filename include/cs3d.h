@@ -268,13 +268,14 @@ cs3d_status cs3d_systems_poll(cs3d_ctx* ctx, int32_t first, int32_t count, int32
 #define CS3D_PH_CON_DIS 2    /* CalculateConForce + first CalculateDisForce + VelocityUpdateA/B */
 #define CS3D_PH_DIS_POST 3   /* second CalculateDisForce + CenterOfMass + volumes + bounding_boxes */
 #define CS3D_PH_DIVIDE 4     /* division phase, end of step, extent + grid geometry on rebuild steps */
-/* Runs n_steps eagerly with HIP events recorded on the context's stream
- * around each phase and adds the elapsed milliseconds of every phase to
- * phase_ms[CS3D_PHASES] (no counterpart in the reference, which has no timers;
- * it replaces profiling by hand). Synchronous. */
+/* Runs n_steps eagerly and adds each phase's kernel time in milliseconds to
+ * phase_ms[CS3D_PHASES]: every launch records start/stop HIP events on the
+ * context's stream (hipExtLaunchKernelGGL), so launch gaps are not counted (no
+ * counterpart in the reference, which has no timers; it replaces profiling by
+ * hand). Synchronous. */
 cs3d_status cs3d_step_timed(cs3d_ctx* ctx, cs3d_sys sys, int32_t n_steps, double* phase_ms);
-/* Average milliseconds of one launch of a kernel on the system's state, timed
- * with HIP events over reps launches on the context's stream: which = 0 the
+/* Average milliseconds of one launch of a kernel on the system's state, from
+ * per-launch start/stop HIP events over reps launches on the context's stream: which = 0 the
  * bonded force (cs3d_k_con_force_bonded), 1 CenterOfMass (cs3d_k_com).  For
  * roofline measurements; no counterpart in the reference. */
 cs3d_status cs3d_time_kernel(cs3d_ctx* ctx, cs3d_sys sys, int32_t which, int32_t reps, double* avg_ms);
