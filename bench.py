@@ -165,6 +165,8 @@ def main():
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run the known-size copy kernel (for HBM counter scaling)")
     ap.add_argument("--no-extras", action="store_true", help="skip the bonded, ensemble and serving measurements")
+    ap.add_argument("--bonded-only", action="store_true",
+                    help="only the standalone bonded-kernel sweep (for profiling it); prints its JSON")
     ap.add_argument("--pair-skin", type=float, default=None,
                     help="contact-candidate skin (performance only: results do not depend on it)")
     ap.add_argument("--ensemble-steps", type=int, default=1000)
@@ -185,6 +187,9 @@ def main():
     tables = mesh.tables(cfg["mesh"])
     rest = configs.rest_positions(cfg["mesh"])
     gm = engine.build_mesh(tables, rest)
+    if args.bonded_only:
+        print(json.dumps(bonded_roofline(gm, log=log)), flush=True)
+        return
     n = cfg["n"]
     log("bench: %d cells, phi %.2f, grid every %d steps" % (n, args.phi, args.grid_interval))
     ctx = engine.Context(gm, capacity_cells=n, voxel_budget=1 << 24)
