@@ -132,7 +132,6 @@ enum : int32_t {
     I_PRESSURE = 2,   // PressureUpdate
     I_BINS = 4,       // insert the nodes into the search bins
     I_CLEAR = 8,      // clear the search-bin counters of the nodes
-    I_DISP = 16,      // flag a candidate search when a node moved pair_skin/2 since the last one
     I_IFSEARCH = 32,  // I_BINS only on systems that search this step
     I_GRIDCOUNT = 64, // systems whose grid was rebuilt at the end of the last step: adopt the new
                       // bboxes and count the voxel occupancy (overflow check)

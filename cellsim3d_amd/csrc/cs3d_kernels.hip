@@ -11,8 +11,8 @@
 // voxel (2.9-wide grid), keeps those whose bbox lies within attraction_range of
 // the node, and tests all 180 nodes of each kept cell (propagate.cu:375-442,
 // 603-655).  Here each node keeps a sorted list of candidate partners: the nodes
-// of other cells within attraction_range + pair_skin, found through search bins
-// of that edge (find_candidates).  The list is rebuilt only on steps where some
+// of other cells within attraction_range + pair_skin, found through periodic
+// search bins of that edge (find_candidates).  The list is rebuilt only on steps where some
 // node has moved pair_skin/2 since the last search, so it always contains every
 // node now within attraction_range.  Every step then applies the reference's own
 // tests to the candidates in (cell, node) order -- the order of the reference's
@@ -296,16 +296,6 @@ __global__ __launch_bounds__(192) void k_integrate(Pool p, Range r, int32_t flag
             p.VAX[gi] = vx; p.VAY[gi] = vy; p.VAZ[gi] = vz;
             if (flags & I_WRITEV) { p.VX[gi] = vx; p.VY[gi] = vy; p.VZ[gi] = vz; }
             p.X[gi] = x; p.Y[gi] = y; p.Z[gi] = z;
-        }
-        if (flags & I_DISP) {  // moved pair_skin/2 since the last candidate search?
-            bool far = false;
-            if (node) {
-                const float h = 0.5f * P->pair_skin;
-                const float dx = x - p.XBX[gi], dy = y - p.XBY[gi], dz = z - p.XBZ[gi];
-                far = !(dx * dx + dy * dy + dz * dz <= h * h);
-            }
-            if (__any(far) && (tid & 63) == __ffsll((unsigned long long)__ballot(far)) - 1)
-                atomicOr(&M->need_search, 1);
         }
         if (node && (flags & I_CLEAR)) p.bins[M->bbase + bin_of(M, x, y, z)] = 0;
         if (node && (flags & I_BINS) && (!(flags & I_IFSEARCH) || M->need_search)) {
