@@ -349,7 +349,7 @@ __device__ int find_candidates(const Pool& p, const SysMeta* M, const cs3d_param
     const float4* ents = reinterpret_cast<const float4*>(p.bins + M->ebase);
     auto consider = [&](const float4 en) {
         const int key = __float_as_int(en.w);
-        if (key / kSlots == c) return;
+        if ((unsigned)(key - c * kSlots) < (unsigned)kSlots) return;  // a node of this cell
         const float dX = X - en.x, dY = Y - en.y, dZ = Z - en.z;
         if (dX * dX + dY * dY + dZ * dZ > rc2) return;
         if (nc < kPMax) lst[nc][tid] = key;
@@ -433,11 +433,6 @@ __device__ __forceinline__ RuleCtx rule_ctx(const SysMeta* M, const cs3d_params*
     R.gx = M->gmin[0]; R.gy = M->gmin[1]; R.gz = M->gmin[2];
     R.xd = M->dims[0]; R.yd = M->dims[1]; R.zd = M->dims[2];
     return R;
-}
-
-// clear the search-bin counter a node incremented (each node clears its own bin)
-__device__ __forceinline__ void clear_own_bin(const Pool& p, const SysMeta* M, float x, float y, float z) {
-    p.bins[M->bbase + bin_of(M, x, y, z)] = 0;
 }
 
 // Shared GetAngleForce pair (propagate.cu:26-60): Fa = F(a-v, b-v), Fb = F(b-v, a-v).
@@ -1006,7 +1001,6 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
             fy = fy - P->gamma_visc * viy;
             fz = fz - P->gamma_visc * viz;
             p.FDX[gi] = fx; p.FDY[gi] = fy; p.FDZ[gi] = fz;
-            if (clear_bins && (mode != M_STEP || M->need_search)) clear_own_bin(p, M, X, Y, Z);
             if (fused) {
                 // the next step's Integrate moves this node to X + v' dt with v' from the
                 // values now in V, F_con and F_dis (bit-identical to k_integrate): search
@@ -1020,8 +1014,20 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
                 far = !(qx * qx + qy * qy + qz * qz <= h * h);
             }
         }
-        if (fused && __any(far) && (tid & 63) == __ffsll((unsigned long long)__ballot(far)) - 1)
+        // one atomic per wave with a far node, skipped once the flag is visibly set
+        if (fused && __any(far) && (tid & 63) == __ffsll((unsigned long long)__ballot(far)) - 1 &&
+            __hip_atomic_load(&M->search_next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
             atomicOr(&M->search_next, 1);
+        if (clear_bins && (mode != M_STEP || M->need_search)) {
+            // the search of this step is done: this cell's share of the system's
+            // search-bin counters (those of the current bin grid, which every
+            // insertion since the last clear used) is cleared in one coalesced run
+            const int64_t nb = min((int64_t)M->bbudget, (int64_t)M->bdims[0] * M->bdims[1] * M->bdims[2]);
+            const int64_t per = (nb + M->n - 1) / M->n;
+            const int64_t lo = (int64_t)c * per, hi = min(nb, lo + per);
+            int32_t* counts = p.bins + M->bbase;
+            for (int64_t b = lo + tid; b < hi; b += blockDim.x) counts[b] = 0;
+        }
         if (fused) {  // this cell's bbox at the end of the step (bounding_boxes, propagatebound.cu:10-141)
             float b[6] = {node ? X : INFINITY, node ? X : -INFINITY, node ? Y : INFINITY,
                           node ? Y : -INFINITY, node ? Z : INFINITY, node ? Z : -INFINITY};
