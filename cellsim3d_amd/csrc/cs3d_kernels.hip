@@ -298,15 +298,36 @@ __global__ __launch_bounds__(192) void k_integrate(Pool p, Range r, int32_t flag
             p.X[gi] = x; p.Y[gi] = y; p.Z[gi] = z;
         }
         if (node && (flags & I_CLEAR)) p.bins[M->bbase + bin_of(M, x, y, z)] = 0;
-        if (node && (flags & I_BINS) && (!(flags & I_IFSEARCH) || M->need_search)) {
-            const int b = bin_of(M, x, y, z);
+        if ((flags & I_BINS) && (!(flags & I_IFSEARCH) || M->need_search)) {
+            // insertion into the search bins: the lanes of a wave that share a bin
+            // take one atomic between them (issued by the group's first lane), and
+            // each lane's slot is the group's base plus its rank in the group
+            const int b = node ? bin_of(M, x, y, z) : -1;
+            const int lane = tid & 63;
+            uint64_t todo = __ballot(node);
+            int leader_of = 0, rank = 0, group = 0;
+            while (todo) {
+                const int leader = __ffsll((unsigned long long)todo) - 1;
+                const int lb = __shfl(b, leader);
+                const uint64_t same = __ballot(b == lb) & todo;
+                if (b == lb) {
+                    leader_of = leader;
+                    rank = __popcll(same & ((1ull << lane) - 1ull));
+                    group = __popcll(same);
+                }
+                todo &= ~same;
+            }
             int32_t* counts = p.bins + M->bbase;
-            const int k = atomicAdd(&counts[b], 1);
-            if (k < kBCap) {
-                float4* ent = reinterpret_cast<float4*>(p.bins + M->ebase);
-                ent[(int64_t)b * kBCap + k] = make_float4(x, y, z, __int_as_float(c * kSlots + tid));
-            } else {
-                flag(M, CS3D_SYS_BIN_OVERFLOW);
+            int k0 = 0;
+            if (node && lane == leader_of) k0 = atomicAdd(&counts[b], group);
+            const int k = __shfl(k0, leader_of) + rank;
+            if (node) {
+                if (k < kBCap) {
+                    float4* ent = reinterpret_cast<float4*>(p.bins + M->ebase);
+                    ent[(int64_t)b * kBCap + k] = make_float4(x, y, z, __int_as_float(c * kSlots + tid));
+                } else {
+                    flag(M, CS3D_SYS_BIN_OVERFLOW);
+                }
             }
         }
         if (tid == 0 && (flags & I_PRESSURE)) {
