@@ -132,11 +132,13 @@ Range range_of(cs3d_ctx* c, cs3d_sys s) {
         r.slot_hi = (int32_t)c->cells.high();
         r.sys_lo = 0;
         r.sys_hi = c->nsys_high;
+        r.one_sys = 0;
     } else {
         r.slot_lo = c->sys[s].base;
         r.slot_hi = c->sys[s].base + c->sys[s].cap;
         r.sys_lo = s;
         r.sys_hi = s + 1;
+        r.one_sys = 1;
     }
     return r;
 }

@@ -111,6 +111,7 @@ struct Pool {
 struct Range {
     int32_t slot_lo, slot_hi;
     int32_t sys_lo, sys_hi;
+    int32_t one_sys;  // 1: the range is exactly the slot range of system sys_lo
 };
 
 // kernel modes

@@ -33,6 +33,12 @@
 
 namespace cs3d {
 
+// The system owning a cell slot.  A launch over one system's own slot range needs
+// no table read: every slot of the range belongs to it.
+__device__ __forceinline__ int sys_of(const Pool& p, const Range& r, int slot) {
+    return r.one_sys ? r.sys_lo : p.slot_sys[slot];
+}
+
 __device__ __forceinline__ bool sys_live(const SysMeta* m) {
     return m->active && (m->horizon < 0 || m->step < m->horizon) && !(m->status & CS3D_SYS_FATAL_MASK);
 }
@@ -253,7 +259,7 @@ __global__ __launch_bounds__(192) void k_integrate(Pool p, Range r, int32_t flag
             fcx = p.FCX[gi]; fcy = p.FCY[gi]; fcz = p.FCZ[gi];
             fdx = p.FDX[gi]; fdy = p.FDY[gi]; fdz = p.FDZ[gi];
         }
-        const int s = p.slot_sys[slot];
+        const int s = sys_of(p, r, slot);
         if (s < 0) continue;
         SysMeta* M = p.meta + s;
         if (!sys_live(M)) continue;
@@ -628,8 +634,12 @@ __device__ __forceinline__ int accept_forces(const int32_t* cand, int nc, int32_
 __global__ __launch_bounds__(192) void k_search(Pool p, Range r, int32_t always) {
     __shared__ int32_t lst[kPMax][kSlots];
     const int tid = threadIdx.x;
+    if (r.one_sys && !always) {  // one system: its flag decides for every workgroup
+        const SysMeta* M0 = p.meta + r.sys_lo;
+        if (!M0->need_search || !sys_live(M0)) return;
+    }
     for (int slot = r.slot_lo + xcd_block(); slot < r.slot_hi; slot += gridDim.x) {
-        const int s = p.slot_sys[slot];
+        const int s = sys_of(p, r, slot);
         if (s < 0) continue;
         SysMeta* M = p.meta + s;
         if (!sys_live(M) || !(always || M->need_search)) continue;
@@ -680,7 +690,7 @@ __global__ __launch_bounds__(192) void k_con_dis(Pool p, Range r, int32_t mode, 
         float vX = 0.f, vY = 0.f, vZ = 0.f;
         if (fused) { vX = rVX[gi]; vY = rVY[gi]; vZ = rVZ[gi]; }
         const int nc_pre = search ? p.NCAND[gi] : 0;
-        const int s = p.slot_sys[slot];
+        const int s = sys_of(p, r, slot);
         if (s < 0) continue;
         SysMeta* M = p.meta + s;
         if (!sys_live(M)) continue;
@@ -876,7 +886,7 @@ __global__ __launch_bounds__(192) void k_bonded(Pool p, Range r) {
             const int64_t gn = (int64_t)nslot * kSlots + tid;
             Xn = p.X[gn]; Yn = p.Y[gn]; Zn = p.Z[gn];
         }
-        const int s = p.slot_sys[slot];
+        const int s = sys_of(p, r, slot);
         bool use = s >= 0;
         SysMeta* M = use ? p.meta + s : nullptr;
         const cs3d_params* P = use ? p.prm + s : nullptr;
@@ -982,7 +992,7 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
             fcX = p.FCX[gi]; fcY = p.FCY[gi]; fcZ = p.FCZ[gi];
             xbX = p.XBX[gi]; xbY = p.XBY[gi]; xbZ = p.XBZ[gi];
         }
-        const int s = p.slot_sys[slot];
+        const int s = sys_of(p, r, slot);
         if (s < 0) continue;
         SysMeta* M = p.meta + s;
         if (!sys_live(M)) continue;
@@ -1478,7 +1488,7 @@ __global__ __launch_bounds__(192) void k_com_shift(Pool p, Range r) {
     const int tid = threadIdx.x;
     if (tid >= kNodes) return;
     for (int slot = r.slot_lo + xcd_block(); slot < r.slot_hi; slot += gridDim.x) {
-        const int s = p.slot_sys[slot];
+        const int s = sys_of(p, r, slot);
         if (s < 0) continue;
         const SysMeta* M = p.meta + s;
         if (!M->shift_pending || slot - M->base >= M->n) continue;
@@ -1510,7 +1520,7 @@ __global__ __launch_bounds__(192) void k_cellbox(Pool p, Range r) {
     for (int q = 0; q < kBoxCells; ++q) {
         const int slot = first + q;
         if (slot >= r.slot_hi) break;
-        const int s = p.slot_sys[slot];
+        const int s = sys_of(p, r, slot);
         const SysMeta* M = s >= 0 ? p.meta + s : nullptr;
         const bool use = M && M->rebuild && (slot - M->base) < M->n;
         if (!use) continue;
@@ -1586,7 +1596,7 @@ __global__ __launch_bounds__(256) void k_grid_dims(Pool p, Range r) {
 __global__ void k_insert(Pool p, Range r, int32_t force) {
     for (int slot = r.slot_lo + blockIdx.x * blockDim.x + threadIdx.x; slot < r.slot_hi;
          slot += gridDim.x * blockDim.x) {
-        const int s = p.slot_sys[slot];
+        const int s = sys_of(p, r, slot);
         if (s < 0) continue;
         SysMeta* M = p.meta + s;
         if (!force && !M->rebuild) continue;
@@ -1646,7 +1656,7 @@ __global__ __launch_bounds__(256) void k_sort(Pool p, Range r, int32_t force) {
 // counters of the current grid are cleared so that k_insert can fill the lists.
 __global__ __launch_bounds__(192) void k_grid_adopt(Pool p, Range r) {
     for (int slot = r.slot_lo + xcd_block(); slot < r.slot_hi; slot += gridDim.x) {
-        const int s = p.slot_sys[slot];
+        const int s = sys_of(p, r, slot);
         if (s < 0) continue;
         const SysMeta* M = p.meta + s;
         if (!M->rebuild || slot - M->base >= M->n) continue;
@@ -1660,7 +1670,7 @@ __global__ __launch_bounds__(192) void k_grid_adopt(Pool p, Range r) {
 __global__ void k_vrange(Pool p, Range r) {
     for (int slot = r.slot_lo + blockIdx.x * blockDim.x + threadIdx.x; slot < r.slot_hi;
          slot += gridDim.x * blockDim.x) {
-        const int s = p.slot_sys[slot];
+        const int s = sys_of(p, r, slot);
         if (s < 0) continue;
         const SysMeta* M = p.meta + s;
         if (slot - M->base >= M->n || (M->status & CS3D_SYS_GRID_TOO_LARGE)) continue;
@@ -1685,7 +1695,7 @@ __global__ __launch_bounds__(256) void k_vox_clear(Pool p, Range r) {
 __global__ __launch_bounds__(192) void k_velocity_ab(Pool p, Range r) {
     const int tid = threadIdx.x;
     for (int slot = r.slot_lo + xcd_block(); slot < r.slot_hi; slot += gridDim.x) {
-        const int s = p.slot_sys[slot];
+        const int s = sys_of(p, r, slot);
         if (s < 0) continue;
         SysMeta* M = p.meta + s;
         if (slot - M->base >= M->n || tid >= kNodes) continue;
