@@ -207,6 +207,14 @@ __device__ __forceinline__ void store_vrange(const Pool& p, int64_t slot, const 
 // of margin, coarser when the extent outgrows the bin budget).  Returns the
 // number of voxel counters to clear.  Called by one thread.
 __device__ int64_t set_grid_geometry(SysMeta* M, const cs3d_params* P, const float* e) {
+    if (!(e[1] >= e[0] && e[3] >= e[2] && e[5] >= e[4])) {
+        // no cells (the extent kept its sentinels): a one-voxel grid, nothing to flag
+        for (int a = 0; a < 3; ++a) {
+            M->gmin[a] = 0.0f; M->dims[a] = 1;
+            M->bmin[a] = 0.0f; M->bdims[a] = 3;
+        }
+        return 1;
+    }
     const float DL = P->grid_dl;
     const int xd = (int)((e[1] - e[0]) / DL + 1.0f);
     const int yd = (int)((e[3] - e[2]) / DL + 1.0f);

@@ -259,7 +259,8 @@ cs3d_status cs3d_set_launch(cs3d_ctx* ctx, int32_t wg_per_cu);
 cs3d_status cs3d_systems_poll(cs3d_ctx* ctx, int32_t first, int32_t count, int32_t* n_cells, int64_t* steps,
                               uint32_t* status);
 
-/* Phases of one step, in launch order (one kernel launch each). */
+/* Phases of one step, in launch order (one kernel launch each; the division phase adds
+ * CorrectCoMMotion for systems with correct_com). */
 #define CS3D_PHASES 5
 #define CS3D_PH_INTEGRATE 0  /* Integrate + ForwardTime + PressureUpdate; adoption of the bboxes and
                                 voxel counts of a grid rebuilt at the end of the previous step;

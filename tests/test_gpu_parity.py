@@ -313,3 +313,50 @@ def test_resume_after_horizon_equals_uninterrupted(oracle_mesh):
         assert np.array_equal(sa.get(f)[:na * 192], sr.get(f)[:na * 192]), "field %d differs after the resume" % f
     ctx.close()
     ref.close()
+
+
+def test_system_without_cells(oracle_mesh):
+    """A system with no cells steps alone and in a batch without any status bit,
+    and leaves the other systems' results unchanged."""
+    A, E = _abi(), _engine()
+    p, X, Y, Z, n, mc = _case("tissue", n=27, seed=3)
+    ctx = E.Context(oracle_mesh, capacity_cells=256, voxel_budget=1 << 22)
+    empty = np.zeros(0, np.float32)
+    s0 = ctx.add_system(p, empty, empty, empty, 0, max_cells=8)
+    s1 = ctx.add_system(p, X, Y, Z, n, max_cells=mc)
+    s0.step(12)
+    ctx.step(25)
+    ctx.sync()
+    n0, st0, status0 = s0.info()
+    assert (n0, st0, status0) == (0, 37, 0)
+    assert s1.info()[1] == 25 and s1.info()[2] & A.SYS_FATAL_MASK == 0
+    ref = E.Context(oracle_mesh, capacity_cells=256, voxel_budget=1 << 22)
+    sr = ref.add_system(p, X, Y, Z, n, max_cells=mc)
+    sr.step(25)
+    ref.sync()
+    for f in (A.F_X, A.F_VX, A.F_FCX):
+        assert np.array_equal(s1.get(f)[:n * 192], sr.get(f)[:n * 192])
+    ctx.close()
+    ref.close()
+
+
+def test_full_size_tissue_matches_oracle(oracle_mesh):
+    """The throughput workload at its full size (2000 cells, 360,000 nodes, volume
+    fraction 0.40), 30 steps with the grid rebuilt every 2 (the candidate lists are
+    re-searched along the way): positions, velocities, forces, centres and the voxel
+    grid against the oracle."""
+    A = _abi()
+    from cellsim3d_amd import configs
+    cfg = configs.single_tissue(2000, 0.40, seed=1, grid_interval=2, steps=10)
+    p, n = cfg["params"], cfg["n"]
+    print("full-size tissue: %d cells" % n, flush=True)
+    sim, ctx, s = _pair(oracle_mesh, p, cfg["X"], cfg["Y"], cfg["Z"], n, n, budget=1 << 14)
+    sim.step(30)
+    s.step(30)
+    ctx.sync()
+    n1, st1, status = s.info()
+    assert n1 == n and st1 == 30 and status & A.SYS_FATAL_MASK == 0
+    _cmp(sim, s, [A.F_X, A.F_Y, A.F_Z, A.F_VX, A.F_VY, A.F_VZ, A.F_FCX, A.F_FCY, A.F_FCZ, A.F_FDX, A.F_FDY,
+                  A.F_FDZ, A.F_CMX], n, label="2000 cells")
+    _cmp_grid(sim, s)
+    ctx.close()
