@@ -553,10 +553,12 @@ __device__ __forceinline__ void pair_friction(const int32_t* lst, int base, int 
 // distance (ConForce skips R^2 >= ar^2, DisForce R^2 > ar^2, propagate.cu:421,641)
 // and the candidate rule per partner cell.  Accepted keys go to rec (for the
 // second friction pass); contact (CON) and friction (DIS) are summed as in
-// propagate.cu:411-440 and 633-653.  Candidates are taken CH at a time: keys,
-// then positions and velocities, are loaded before any is tested.
+// propagate.cu:411-440 and 633-653.  The loop is software-pipelined: while one
+// candidate is tested, the partner data of the next and the key of the one after
+// are already loading, so the key -> position dependence of each candidate does
+// not serialise the loop.
 // Returns the number accepted.
-template <bool CON, bool DIS, int CH>
+template <bool CON, bool DIS>
 __device__ __forceinline__ int accept_forces(const int32_t* cand, int nc, int32_t* rec, int base, float X, float Y,
                                              float Z, const RuleCtx& R, const int32_t* __restrict__ vrange,
                                              const float* __restrict__ rBB, float vix,
@@ -570,65 +572,62 @@ __device__ __forceinline__ int accept_forces(const int32_t* cand, int nc, int32_
     int lastb = -1;
     bool lastok = false;
     int vx = -1, vy = -1, vz = -1;  // the node's voxel (propagate.cu:360-372), found at first use
-    for (int i0 = 0; i0 < nc; i0 += CH) {
-        int key[CH];
-#pragma unroll
-        for (int j = 0; j < CH; ++j) key[j] = i0 + j < nc ? cand[(i0 + j) * kSlots] : 0;
-        float px[CH], py[CH], pz[CH], wx[CH], wy[CH], wz[CH];
-#pragma unroll
-        for (int j = 0; j < CH; ++j) {
-            const bool in = i0 + j < nc;
-            const int64_t gj = key_node(base, key[j]);
-            px[j] = in ? rX[gj] : 0.f;
-            py[j] = in ? rY[gj] : 0.f;
-            pz[j] = in ? rZ[gj] : 0.f;
-            if (DIS) {
-                wx[j] = in ? rVX[gj] : 0.f;
-                wy[j] = in ? rVY[gj] : 0.f;
-                wz[j] = in ? rVZ[gj] : 0.f;
+    int key_n = nc > 0 ? cand[0] : 0;
+    int key_nn = nc > 1 ? cand[kSlots] : 0;
+    float pxn = 0.f, pyn = 0.f, pzn = 0.f, wxn = 0.f, wyn = 0.f, wzn = 0.f;
+    if (nc > 0) {
+        const int64_t g = key_node(base, key_n);
+        pxn = rX[g]; pyn = rY[g]; pzn = rZ[g];
+        if (DIS) { wxn = rVX[g]; wyn = rVY[g]; wzn = rVZ[g]; }
+    }
+    for (int i = 0; i < nc; ++i) {
+        const int key = key_n;
+        const float px = pxn, py = pyn, pz = pzn;
+        const float wx = wxn, wy = wyn, wz = wzn;
+        key_n = key_nn;
+        if (i + 1 < nc) {
+            const int64_t g = key_node(base, key_n);
+            pxn = rX[g]; pyn = rY[g]; pzn = rZ[g];
+            if (DIS) { wxn = rVX[g]; wyn = rVY[g]; wzn = rVZ[g]; }
+        }
+        if (i + 2 < nc) key_nn = cand[(i + 2) * kSlots];
+        const int b = key / kSlots;
+        const float dX = X - px, dY = Y - py, dZ = Z - pz;
+        const float R2 = dX * dX + dY * dY + dZ * dZ;
+        if (R2 > k.ar2) continue;
+        if (b != lastb) {
+            if (vx < 0) {
+                vx = vox1(X, R.gx, R.DL, R.xd);
+                vy = vox1(Y, R.gy, R.DL, R.yd);
+                vz = vox1(Z, R.gz, R.DL, R.zd);
+            }
+            lastb = b;
+            lastok = rule_ok(vrange + (int64_t)(base + b) * 6, rBB + (int64_t)(base + b) * 6, vx, vy, vz, X, Y, Z,
+                             R.ar2);
+        }
+        if (!lastok) continue;
+        rec[np * kSlots] = key;
+        ++np;
+        if (CON && R2 < k.ar2) {
+            const float Rl = sqrtf(R2);
+            if (Rl < k.ar) {
+                cfx += k.ka * (k.ar - Rl) / Rl * dX;
+                cfy += k.ka * (k.ar - Rl) / Rl * dY;
+                cfz += k.ka * (k.ar - Rl) / Rl * dZ;
+            }
+            if (Rl <= k.rr) {
+                cfx += k.kr * (k.rr - Rl) / Rl * dX;
+                cfy += k.kr * (k.rr - Rl) / Rl * dY;
+                cfz += k.kr * (k.rr - Rl) / Rl * dZ;
             }
         }
-#pragma unroll
-        for (int j = 0; j < CH; ++j) {
-            if (i0 + j >= nc) continue;
-            const int b = key[j] / kSlots;
-            const float dX = X - px[j], dY = Y - py[j], dZ = Z - pz[j];
-            const float R2 = dX * dX + dY * dY + dZ * dZ;
-            if (R2 > k.ar2) continue;
-            if (b != lastb) {
-                if (vx < 0) {
-                    vx = vox1(X, R.gx, R.DL, R.xd);
-                    vy = vox1(Y, R.gy, R.DL, R.yd);
-                    vz = vox1(Z, R.gz, R.DL, R.zd);
-                }
-                lastb = b;
-                lastok = rule_ok(vrange + (int64_t)(base + b) * 6, rBB + (int64_t)(base + b) * 6, vx, vy, vz, X, Y, Z,
-                                 R.ar2);
-            }
-            if (!lastok) continue;
-            rec[np * kSlots] = key[j];
-            ++np;
-            if (CON && R2 < k.ar2) {
-                const float Rl = sqrtf(R2);
-                if (Rl < k.ar) {
-                    cfx += k.ka * (k.ar - Rl) / Rl * dX;
-                    cfy += k.ka * (k.ar - Rl) / Rl * dY;
-                    cfz += k.ka * (k.ar - Rl) / Rl * dZ;
-                }
-                if (Rl <= k.rr) {
-                    cfx += k.kr * (k.rr - Rl) / Rl * dX;
-                    cfy += k.kr * (k.rr - Rl) / Rl * dY;
-                    cfz += k.kr * (k.rr - Rl) / Rl * dZ;
-                }
-            }
-            if (DIS) {
-                const float vx = vix - wx[j], vy = viy - wy[j], vz = viz - wz[j];
-                const float d = vx * nx + vy * ny + vz * nz;
-                const float tx = vx - d * nx, ty = vy - d * ny, tz = vz - d * nz;
-                fdx = fdx - k.gext * tx;
-                fdy = fdy - k.gext * ty;
-                fdz = fdz - k.gext * tz;
-            }
+        if (DIS) {
+            const float ux = vix - wx, uy = viy - wy, uz = viz - wz;
+            const float d = ux * nx + uy * ny + uz * nz;
+            const float tx = ux - d * nx, ty = uy - d * ny, tz = uz - d * nz;
+            fdx = fdx - k.gext * tx;
+            fdy = fdy - k.gext * ty;
+            fdz = fdz - k.gext * tz;
         }
     }
     return np;
@@ -675,7 +674,9 @@ __global__ __launch_bounds__(192) void k_search(Pool p, Range r, int32_t always)
 // partners), M_CON / M_INIT (FC, accepted partners), M_BONDED (spring +
 // pressure + angle -> FC), M_PAIRS (accepted partners only).
 // ------------------------------------------------------------------------------
-__global__ __launch_bounds__(192) void k_con_dis(Pool p, Range r, int32_t mode, const float* __restrict__ rX,
+// 4 waves per SIMD: with the pipelined candidate loop the allocator would otherwise
+// take 132 VGPRs (3 waves); at this bound it fits 127 without spills
+__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_con_dis(Pool p, Range r, int32_t mode, const float* __restrict__ rX,
                                                  const float* __restrict__ rY, const float* __restrict__ rZ,
                                                  const float* __restrict__ rVX, const float* __restrict__ rVY,
                                                  const float* __restrict__ rVZ, const float* __restrict__ rBB) {
@@ -799,16 +800,16 @@ __global__ __launch_bounds__(192) void k_con_dis(Pool p, Range r, int32_t mode, 
                 }
                 float nx, ny, nz;
                 node_normal(sx, sy, sz, mesh, tid, nn3[0], nn3[1], nn3[2], nx, ny, nz);
-                np = accept_forces<true, true, 1>(cand, nc, rec, M->base, X, Y, Z, R, p.VRANGE, rBB, vix, viy, viz, nx, ny, nz, k,
+                np = accept_forces<true, true>(cand, nc, rec, M->base, X, Y, Z, R, p.VRANGE, rBB, vix, viy, viz, nx, ny, nz, k,
                                                rX, rY, rZ, rVX, rVY, rVZ, cfx, cfy, cfz, fdx, fdy, fdz);
                 fdx = fdx - P->gamma_visc * vix;  // drag, propagate.cu:658
                 fdy = fdy - P->gamma_visc * viy;
                 fdz = fdz - P->gamma_visc * viz;
             } else if (forces) {
-                np = accept_forces<true, false, 1>(cand, nc, rec, M->base, X, Y, Z, R, p.VRANGE, rBB, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, k,
+                np = accept_forces<true, false>(cand, nc, rec, M->base, X, Y, Z, R, p.VRANGE, rBB, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, k,
                                                 rX, rY, rZ, rVX, rVY, rVZ, cfx, cfy, cfz, fdx, fdy, fdz);
             } else {
-                np = accept_forces<false, false, 1>(cand, nc, rec, M->base, X, Y, Z, R, p.VRANGE, rBB, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
+                np = accept_forces<false, false>(cand, nc, rec, M->base, X, Y, Z, R, p.VRANGE, rBB, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
                                                  k, rX, rY, rZ, rVX, rVY, rVZ, cfx, cfy, cfz, fdx, fdy, fdz);
             }
             p.NPAIR[gi] = np;
