@@ -167,3 +167,50 @@ def test_cell_centres_match_the_device_tree():
         assert np.allclose(got[a], mean, rtol=1e-6, atol=1e-6)
     with pytest.raises(ValueError):
         engine.cell_centres(X[:100], Y, Z, n)
+
+
+C_CLIENT = r"""
+#include <stdio.h>
+#include <string.h>
+#include "cs3d.h"
+
+int main(void) {
+    cs3d_params p;
+    cs3d_ranmar_state rng;
+    float u[20006];
+    float x[192], y[192], z[192], cx, cy, cz;
+    int i;
+    if (cs3d_params_default(&p) != CS3D_OK || cs3d_params_check(&p) != CS3D_OK) return 1;
+    if (cs3d_ranmar_init(&rng, 1802, 9373) != CS3D_OK || cs3d_ranmar(&rng, u, 20006) != CS3D_OK) return 2;
+    memset(x, 0, sizeof x); memset(y, 0, sizeof y); memset(z, 0, sizeof z);
+    for (i = 0; i < CS3D_NODES; ++i) { x[i] = 1.0f; y[i] = (float)i; z[i] = -2.0f; }
+    if (cs3d_cell_centres(x, y, z, 1, &cx, &cy, &cz) != CS3D_OK) return 3;
+    printf("%.1f %.1f %.1f %.1f %s\n", (double)u[20000] * 4096.0 * 4096.0, (double)cx, (double)cy, (double)cz,
+           cs3d_status_str(CS3D_CAPACITY));
+    return 0;
+}
+"""
+
+
+def test_header_compiles_as_c_and_links(tmp_path):
+    """include/cs3d.h is plain C: a C99 client compiles against it, links
+    libcs3d.so and calls host entry points (no device needed)."""
+    import shutil
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "client.c"
+    src.write_text(C_CLIENT)
+    exe = tmp_path / "client"
+    libdir = os.path.dirname(engine.LIB_PATH)
+    import subprocess
+    r = subprocess.run([cc, "-std=c99", "-Wall", "-Werror", "-pedantic", "-I", os.path.join(ROOT, "include"),
+                        str(src), "-L", libdir, "-lcs3d", "-Wl,-rpath," + libdir, "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stdout + out.stderr
+    ranmar, cx, cy, cz, msg = out.stdout.split(maxsplit=4)
+    assert float(ranmar) == 6533892.0  # marsaglia.h:68-74, first of the six values
+    assert (float(cx), float(cy), float(cz)) == (1.0, 89.5, -2.0)
+    assert msg.strip() == "capacity exceeded"
