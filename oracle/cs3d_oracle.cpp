@@ -553,6 +553,9 @@ uint32_t grid_build(int n, const cs3d_params* p, const T* bb, int32_t budget, in
     dims[0] = xd; dims[1] = yd; dims[2] = zd;
     int64_t nv = (int64_t)xd * yd * zd;
     if (nv > budget || xd <= 0 || yd <= 0 || zd <= 0) {
+        // flagged; the grid falls back to one voxel (as the engine's), so later
+        // voxel lookups stay inside the budget
+        dims[0] = dims[1] = dims[2] = 1;
         std::fill(counts, counts + budget, 0);
         return CS3D_SYS_GRID_TOO_LARGE;
     }

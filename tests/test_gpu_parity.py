@@ -49,7 +49,7 @@ def _case(kind="tissue", seed=3, n=27, phi=0.40, growth=False, vcap=64):
     return p, X, Y, Z, n, 4 * n
 
 
-def _pair(oracle_mesh, p, X, Y, Z, n, max_cells, budget=1 << 16, V=None):
+def _pair(oracle_mesh, p, X, Y, Z, n, max_cells, budget=1 << 16, V=None, extra_cells=16):
     E, O, A = _engine(), _oracle(), _abi()
     sim = O.Sim(p, oracle_mesh, X, Y, Z, n, max_cells=max_cells, budget=budget)
     if V is not None:
@@ -58,7 +58,7 @@ def _pair(oracle_mesh, p, X, Y, Z, n, max_cells, budget=1 << 16, V=None):
             full[:len(a)] = a
             sim.set(f, full)
     sim.init()
-    ctx = E.Context(oracle_mesh, capacity_cells=max_cells + 16, voxel_budget=budget * 70)
+    ctx = E.Context(oracle_mesh, capacity_cells=max_cells + extra_cells, voxel_budget=budget * 70)
     sys_ = ctx.add_system(p, X, Y, Z, n, max_cells=max_cells, voxels=budget, V=V)
     return sim, ctx, sys_
 
@@ -359,4 +359,97 @@ def test_full_size_tissue_matches_oracle(oracle_mesh):
     _cmp(sim, s, [A.F_X, A.F_Y, A.F_Z, A.F_VX, A.F_VY, A.F_VZ, A.F_FCX, A.F_FCY, A.F_FCZ, A.F_FDX, A.F_FDY,
                   A.F_FDZ, A.F_CMX], n, label="2000 cells")
     _cmp_grid(sim, s)
+    ctx.close()
+
+
+def _flags(st):
+    A = _abi()
+    return st & A.SYS_FATAL_MASK
+
+
+def test_capacity_flag_matches_oracle(oracle_mesh):
+    """Division past max_cells (GPUbounce.cu:1264-1270): CAPACITY is set, the
+    system stops, and the cell count is the capacity, as in the oracle."""
+    A = _abi()
+    from cellsim3d_amd import configs
+    cfg = configs.plain_small(seed=2)
+    p = cfg["params"]
+    p.division_vol = np.float32(0.5 * cfg["v0"])  # every cell divides at the first growth step
+    n = cfg["n"]
+    sim, ctx, s = _pair(oracle_mesh, p, cfg["X"], cfg["Y"], cfg["Z"], n, n + 3)
+    st_o = sim.step(5)
+    s.step(5)
+    ctx.sync()
+    n1, step1, status = s.info()
+    print("capacity: gpu status 0x%x oracle 0x%x, cells %d / %d" % (status, st_o, n1, sim.n), flush=True)
+    assert status & A.SYS_CAPACITY and st_o & A.SYS_CAPACITY
+    assert n1 == sim.n == n + 3
+    ctx.close()
+
+
+def test_grid_flags_match_oracle(oracle_mesh):
+    """Voxel lists past voxel_capacity (propagatebound.cu:247-259, unchecked in the
+    reference) and a grid larger than the voxel budget are flagged at start-up,
+    as in the oracle."""
+    A, E, O = _abi(), _engine(), _oracle()
+    p, X, Y, Z, n, mc = _case("tissue", n=27, seed=3, vcap=2)
+    sim = O.Sim(p, oracle_mesh, X, Y, Z, n, max_cells=mc, budget=1 << 14)
+    st_o = sim.init()
+    ctx = E.Context(oracle_mesh, capacity_cells=2 * mc + 8, voxel_budget=1 << 20)
+    s = ctx.add_system(p, X, Y, Z, n, max_cells=mc, voxels=1 << 14)
+    status = s.info()[2]
+    print("voxel capacity 2: gpu 0x%x oracle 0x%x" % (status, st_o), flush=True)
+    assert status & A.SYS_VOXEL_OVERFLOW and st_o & A.SYS_VOXEL_OVERFLOW
+    p2, X2, Y2, Z2, n2, mc2 = _case("tissue", n=27, seed=3)
+    sim2 = O.Sim(p2, oracle_mesh, X2, Y2, Z2, n2, max_cells=mc2, budget=8)
+    st_o2 = sim2.init()
+    s2 = ctx.add_system(p2, X2, Y2, Z2, n2, max_cells=mc2, voxels=8)
+    status2 = s2.info()[2]
+    print("voxel budget 8: gpu 0x%x oracle 0x%x" % (status2, st_o2), flush=True)
+    assert status2 & A.SYS_GRID_TOO_LARGE and st_o2 & A.SYS_GRID_TOO_LARGE
+    s2.step(3)  # a flagged system does not step
+    ctx.sync()
+    assert s2.info()[1] == 0
+    ctx.close()
+
+
+def test_nan_flag_matches_oracle(oracle_mesh):
+    """A NaN position (propagate.cu:241-248, a device trap in the reference) sets
+    the NaN bit and stops the system; the other systems of a batch go on."""
+    A = _abi()
+    p, X, Y, Z, n, mc = _case("tissue", n=27, seed=3)
+    sim, ctx, s = _pair(oracle_mesh, p, X, Y, Z, n, mc, extra_cells=64)
+    pb, Xb, Yb, Zb, nb, mcb = _case("tissue", n=8, seed=5)
+    sb = ctx.add_system(pb, Xb, Yb, Zb, nb, max_cells=mcb)
+    x = s.get(A.F_X)
+    x[5 * 192 + 17] = np.nan
+    s.set(A.F_X, x)
+    xo = sim.get(A.F_X)
+    xo[5 * 192 + 17] = np.nan
+    sim.set(A.F_X, xo)
+    st_o = sim.step(2)
+    ctx.step(4)
+    ctx.sync()
+    status = s.info()[2]
+    print("nan: gpu 0x%x oracle 0x%x" % (status, st_o), flush=True)
+    assert status & A.SYS_NAN and st_o & A.SYS_NAN
+    assert s.info()[1] == 0  # the step that finds it is not counted, and no step follows
+    assert sb.info()[1] == 4 and sb.info()[2] & A.SYS_FATAL_MASK == 0
+    ctx.close()
+
+
+def test_search_list_overflow_is_flagged(oracle_mesh):
+    """More contact candidates than a node's list holds, or more nodes than a
+    search bin holds (a skin of 2 puts hundreds of nodes in reach): flagged, and
+    the system stops instead of dropping contacts."""
+    A, E = _abi(), _engine()
+    p, X, Y, Z, n, mc = _case("tissue", n=27, seed=3)
+    p.pair_skin = np.float32(2.0)
+    ctx = E.Context(oracle_mesh, capacity_cells=mc + 8, voxel_budget=1 << 22)
+    s = ctx.add_system(p, X, Y, Z, n, max_cells=mc, voxels=1 << 14)
+    s.step(2)
+    ctx.sync()
+    status = s.info()[2]
+    print("skin 2: gpu 0x%x" % status, flush=True)
+    assert status & (A.SYS_CAND_OVERFLOW | A.SYS_BIN_OVERFLOW)
     ctx.close()
