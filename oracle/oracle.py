@@ -12,7 +12,9 @@ import numpy as np
 from cellsim3d_amd import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# CS3D_ORACLE_LIB points the binding at another build of the oracle (oracle/mutants.py uses it to
+# load copies with planted mistakes without touching the real library).
+LIB_PATH = os.environ.get("CS3D_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libref_ranmar.so")
 
 _lib = None
