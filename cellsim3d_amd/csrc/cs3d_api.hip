@@ -504,6 +504,11 @@ extern "C" cs3d_status cs3d_system_add(cs3d_ctx* c, const cs3d_params* prm, int3
     return CS3D_OK;
 }
 
+extern "C" cs3d_status cs3d_system_startup(cs3d_ctx* c, cs3d_sys s) {
+    if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
+    return run_startup(c, s);
+}
+
 extern "C" cs3d_status cs3d_system_remove(cs3d_ctx* c, cs3d_sys s) {
     if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
     HIPCHK(hipStreamSynchronize(c->stream));

@@ -3,7 +3,7 @@
 //
 //   celldiv <no_of_cells> <inp.json> <gpuid> [--grid-interval K] [--pair-skin S]
 //           [--mesh-dir DIR] [--no-graph] [--daughter-com-fix] [--setup-only]
-//           [--sync-output]
+//           [--sync-output] [--dump-setup PREFIX] [--save-state FILE]
 //
 // Same command line, parameter file, input mesh files (C180, C180NN, C180C,
 // C180_pentahexa in the working directory or --mesh-dir) and outputs: the
@@ -809,20 +809,28 @@ bool write_mit_index(const Settings& s, const std::vector<int>& dividing, const 
     return true;
 }
 
+bool write_bytes(const std::string& path, const void* data, size_t bytes) {
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) return false;
+    const bool ok = std::fwrite(data, 1, bytes, f) == bytes;
+    return std::fclose(f) == 0 && ok;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
     std::printf("celldiv (C180 cell mechanics on AMD Instinct, libcs3d %s)\n", cs3d_version());
     if (argc < 4) {
         std::printf("Usage: celldiv no_of_cells inpFile.json gpuid [--grid-interval K] [--pair-skin S] "
-                    "[--mesh-dir DIR] [--no-graph] [--daughter-com-fix] [--setup-only] [--sync-output]\n");
+                    "[--mesh-dir DIR] [--no-graph] [--daughter-com-fix] [--setup-only] [--sync-output] "
+                    "[--dump-setup PREFIX] [--save-state FILE]\n");
         return 0;
     }
     const int n_cells = std::atoi(argv[1]);
     const int gpuid = std::atoi(argv[3]);
     int grid_interval = 1;
     float pair_skin = -1.f;
-    std::string mesh_dir = ".";
+    std::string mesh_dir = ".", dump_setup, save_state;
     bool graph = true, com_fix = false, setup_only = false, sync_output = false;
     for (int a = 4; a < argc; ++a) {
         const std::string o = argv[a];
@@ -833,6 +841,8 @@ int main(int argc, char** argv) {
         else if (o == "--daughter-com-fix") com_fix = true;
         else if (o == "--setup-only") setup_only = true;
         else if (o == "--sync-output") sync_output = true;
+        else if (o == "--dump-setup" && a + 1 < argc) dump_setup = argv[++a];
+        else if (o == "--save-state" && a + 1 < argc) save_state = argv[++a];
         else {
             std::printf("unknown option %s\n", o.c_str());
             return -1;
@@ -900,6 +910,14 @@ int main(int argc, char** argv) {
     s.p.correct_com = s.correct_com ? 1 : 0;
     if (check_walls(s, n_cells, Z) != 0) return -1;
 
+    // --dump-setup: the parameter block and mesh tables exactly as handed to the engine
+    // (cs3d_params and cs3d_mesh bytes), for tools that re-run the same system
+    if (!dump_setup.empty() && (!write_bytes(dump_setup + ".params", &s.p, sizeof(s.p)) ||
+                                !write_bytes(dump_setup + ".mesh", &mesh, sizeof(mesh)))) {
+        std::printf("ERROR: cannot write %s.params / .mesh\n", dump_setup.c_str());
+        return -1;
+    }
+
     // outputs, GPUbounce.cu:332-335, 768-778, 1007-1048
     if (FILE* f = std::fopen("psfil", "w")) std::fclose(f);
     Output out;
@@ -961,7 +979,18 @@ int main(int argc, char** argv) {
     // the RANMAR stream continues where placement and soft-cell selection left it
     // (one stream, as in the reference)
     cs3d_set(ctx, sys, CS3D_F_RANMAR, &rng, sizeof(rng));
-    if (s.p.use_different_stiffnesses) cs3d_set(ctx, sys, CS3D_F_STIFFNESS, E.data(), (size_t)n_cells * 4);
+    if (s.p.use_different_stiffnesses && n_cells > 0) {
+        // soft cells chosen at set-up are on the device before the initial forces
+        // (GPUbounce.cu:691 uploads d_Youngs_mod before the start-up CalculateConForce at :978-1004)
+        st = cs3d_set(ctx, sys, CS3D_F_STIFFNESS, E.data(), (size_t)n_cells * 4);
+        if (st == CS3D_OK && std::any_of(E.begin(), E.begin() + n_cells, [&](float e) { return e != s.p.stiffness1; }))
+            st = cs3d_system_startup(ctx, sys);
+        if (st != CS3D_OK) {
+            std::printf("ERROR: soft cells at set-up: %s (%s)\n", cs3d_status_str(st), cs3d_last_error());
+            cs3d_ctx_destroy(ctx);
+            return -1;
+        }
+    }
 
     const int total = s.time_steps + s.equi_steps;
     const int I = s.new_cell_count_int;
@@ -1071,6 +1100,11 @@ int main(int argc, char** argv) {
     if (out.bin) cs3d_traj_close(out.bin, n);
     if (out.ascii) std::fclose(out.ascii);
     if (out.forces) std::fclose(out.forces);
+    // --save-state: the system's complete state after the last step (cs3d_system_save)
+    if (rcode == 0 && !save_state.empty() && cs3d_system_save(ctx, sys, save_state.c_str()) != CS3D_OK) {
+        std::printf("ERROR: saving the state: %s\n", cs3d_last_error());
+        rcode = -1;
+    }
     cs3d_ctx_destroy(ctx);
     if (rcode == 0) write_mit_index(s, dividing, totals);
     std::printf("%d cells after %d steps\n", n, step);

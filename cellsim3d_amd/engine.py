@@ -43,6 +43,7 @@ SIGNATURES = {
                         st_t),
     "cs3d_system_add_raw": ([vp, PP, C.c_int32, C.c_int32, C.c_int64, C.POINTER(C.c_int32)], st_t),
     "cs3d_system_remove": ([vp, C.c_int32], st_t),
+    "cs3d_system_startup": ([vp, C.c_int32], st_t),
     "cs3d_system_set_horizon": ([vp, C.c_int32, C.c_int64], st_t),
     "cs3d_system_info": ([vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_uint32)], st_t),
     "cs3d_system_set_counts": ([vp, C.c_int32, C.c_int32, C.c_int64], st_t),
@@ -242,6 +243,10 @@ class System:
     def save(self, path):
         """Checkpoint of the complete state (continues bit-identically after load)."""
         check(lib().cs3d_system_save(self.ctx.h, self.id, str(path).encode()), "cs3d_system_save")
+
+    def startup(self):
+        """Start-up forces again from the current state (after per-cell fields were set)."""
+        check(lib().cs3d_system_startup(self.ctx.h, self.id), "cs3d_system_startup")
 
     def remove(self):
         check(lib().cs3d_system_remove(self.ctx.h, self.id), "cs3d_system_remove")

@@ -221,6 +221,14 @@ cs3d_status cs3d_system_add(cs3d_ctx* ctx, const cs3d_params* p, int32_t n_cells
 cs3d_status cs3d_system_add_raw(cs3d_ctx* ctx, const cs3d_params* p, int32_t n_cells, int32_t max_cells,
                                 int64_t voxels, cs3d_sys* out);
 cs3d_status cs3d_system_remove(cs3d_ctx* ctx, cs3d_sys sys);
+/* Runs the start-up sequence of cs3d_system_add again on the system's current
+ * state: CenterOfMass, bounding boxes, grid, initial CalculateConForce and
+ * CalculateDisForce (GPUbounce.cu:739-763, 978-1004).  For callers that change
+ * per-cell state between cs3d_system_add (or _raw plus cs3d_set) and the first
+ * step: the reference uploads the per-cell stiffness d_Youngs_mod
+ * (GPUbounce.cu:691) before its initial forces, so soft cells chosen at set-up
+ * enter the start-up forces.  Synchronous. */
+cs3d_status cs3d_system_startup(cs3d_ctx* ctx, cs3d_sys sys);
 /* Limits a system to `horizon` total steps (stepping stops there; -1 = none). */
 cs3d_status cs3d_system_set_horizon(cs3d_ctx* ctx, cs3d_sys sys, int64_t horizon);
 cs3d_status cs3d_system_info(cs3d_ctx* ctx, cs3d_sys sys, int32_t* n_cells, int64_t* step, uint32_t* status);

@@ -292,3 +292,26 @@ def test_wall_setup_checks(tmp_path, walls, code):
         d["boxParams"]["useRigidSimulationBox"] = 0
     r, _ = _setup(tmp_path, edit, n=9)
     assert r.returncode == code, r.stdout
+
+
+def test_dump_setup_writes_the_engine_inputs(tmp_path):
+    """--dump-setup writes the parameter block and mesh tables handed to the engine
+    (the bytes of cs3d_params and cs3d_mesh), also with --setup-only."""
+    import ctypes as C
+    from cellsim3d_amd import abi
+    d = json.loads(_strip_comments(INP))
+    _soft({"fractionOfSofterCells": 0.3})(d)
+    (tmp_path / "inp.json").write_text(json.dumps(d))
+    M.write_reference_files(str(tmp_path), configs.unit_mesh())
+    r = subprocess.run([CELLDIV, "9", "inp.json", "0", "--setup-only", "--dump-setup", "s"], cwd=str(tmp_path),
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout
+    pb, mb = (tmp_path / "s.params").read_bytes(), (tmp_path / "s.mesh").read_bytes()
+    assert len(pb) == C.sizeof(abi.Params) and len(mb) == C.sizeof(abi.Mesh)
+    p = abi.Params.from_buffer_copy(pb)
+    assert p.use_different_stiffnesses == 1 and p.use_rigid_box == 1
+    assert p.stiffness2 == np.float32(0.8 * 1000.0) and p.stiffness1 == np.float32(1000.0)
+    assert p.ranmar_ij == 7 and p.ranmar_kl == 700
+    m = abi.Mesh.from_buffer_copy(mb)
+    nn = np.ctypeslib.as_array(m.nn)
+    assert nn.min() >= 0 and nn.max() < 180

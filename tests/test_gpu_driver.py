@@ -129,3 +129,52 @@ def test_output_thread_writes_the_same_bytes(tmp_path, binary):
     if binary:
         _, frames = D._read_frames(tmp_path / "async" / "t.xyz")
         assert [f[0] for f in frames] == list(range(0, 201, 20))
+
+
+def test_soft_cells_at_setup_enter_the_start_up_forces(tmp_path):
+    """Soft cells chosen at set-up are on the device before the start-up forces: the
+    reference uploads d_Youngs_mod (GPUbounce.cu:691) before its initial
+    CalculateConForce (:978-1004).  The driver's state after 4 steps (--save-state)
+    equals, bit for bit, an engine run from the driver's own parameters, mesh and frame 0
+    (--dump-setup) with the stiffness set before the start-up (System.startup).  The same
+    run with the start-up forces taken at stiffness1 ends with other velocities, so the
+    comparison sees the difference."""
+    from cellsim3d_amd import abi, configs, engine
+    from cellsim3d_amd import mesh as M
+    v0 = configs.mesh_volume(configs.unit_mesh())
+    d = json.loads(D._strip_comments(D.INP))
+    d["core"].update(division_Vol=float(2.0 * v0), div_time_steps=4, non_div_time_steps=0, trajWriteInt=1)
+    d["counting"]["cell_count_int"] = 2
+    D._soft({"fractionOfSofterCells": 0.3})(d)
+    (tmp_path / "inp.json").write_text(json.dumps(d))
+    M.write_reference_files(str(tmp_path), configs.unit_mesh())
+    print("celldiv 16 cells, 5 soft, 4 steps", flush=True)
+    r = subprocess.run([CELLDIV, "16", "inp.json", "0", "--dump-setup", "setup", "--save-state", "end.cs3d"],
+                       cwd=str(tmp_path), capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    p = abi.Params.from_buffer_copy((tmp_path / "setup.params").read_bytes())
+    mesh = abi.Mesh.from_buffer_copy((tmp_path / "setup.mesh").read_bytes())
+    _, frames = D._read_frames(tmp_path / "t.xyz")
+    step0, _, n, X, Y, Z, types = frames[0]
+    assert step0 == 0 and n == 16 and types.sum() == 5
+    mc = d["core"]["MaxNoofC180s"]
+    Ecell = np.where(types == 1, p.stiffness2, p.stiffness1).astype(np.float32)
+    ctx = engine.Context(mesh, capacity_cells=3 * mc + 16)
+    drv = ctx.load_system(str(tmp_path / "end.cs3d"))
+    runs = []
+    for startup in (True, False):
+        s = ctx.add_system(p, X.ravel(), Y.ravel(), Z.ravel(), n, max_cells=mc)
+        s.set(abi.F_STIFFNESS, Ecell)
+        if startup:
+            s.startup()
+        s.step(4)
+        runs.append(s)
+    ctx.sync()
+    assert drv.info()[:2] == (n, 4) and runs[0].info()[:2] == (n, 4)
+    for f in (abi.F_X, abi.F_Y, abi.F_Z, abi.F_VX, abi.F_VY, abi.F_VZ, abi.F_FCX, abi.F_FDX, abi.F_STIFFNESS):
+        a, b = drv.get(f), runs[0].get(f)
+        k = n * 192 if f in abi.NODE_FIELDS else n
+        assert np.array_equal(a[:k], b[:k]), "field %d differs from the engine run" % f
+    assert not all(np.array_equal(drv.get(f)[:n * 192], runs[1].get(f)[:n * 192])
+                   for f in (abi.F_VX, abi.F_VY, abi.F_VZ))
+    ctx.close()

@@ -453,3 +453,78 @@ def test_search_list_overflow_is_flagged(oracle_mesh):
     print("skin 2: gpu 0x%x" % status, flush=True)
     assert status & (A.SYS_CAND_OVERFLOW | A.SYS_BIN_OVERFLOW)
     ctx.close()
+
+
+@pytest.mark.parametrize("opt", ["soft_walls", "same_stiffness_no_sphericity"])
+def test_parameter_branches_match_oracle(oracle_mesh, opt):
+    """Branches off the default parameter set, with growth and division, through the
+    fused step (graph path):
+    * soft_walls: soft cells chosen before the start-up forces (d_Youngs_mod is uploaded
+      at GPUbounce.cu:691, before the initial CalculateConForce at :978-1004), the
+      variable-stiffness PressureUpdate (PressureKernels.cu:15-34) switching at
+      phase_count, recalc_r0 keeping soft cells from dividing (volume.cu:155-160),
+      daughters taking stiffness1 (GPUbounce.cu:1257-1262), and the rigid-box walls (propagate.cu:473-510)
+      with nodes inside the threshold distance of all six walls;
+    * same_stiffness_no_sphericity: daughters keep the parent's stiffness and
+      checkSphericity is off (volume.cu:145-153)."""
+    A, E, O = _abi(), _engine(), _oracle()
+    from cellsim3d_amd import configs
+    cfg = configs.plain_small(seed=4)
+    p = cfg["params"]
+    p.division_vol = np.float32(1.02 * cfg["v0"])
+    n, mc = cfg["n"], 64
+    X, Y, Z = cfg["X"].copy(), cfg["Y"].copy(), cfg["Z"].copy()
+    p.use_different_stiffnesses = 1
+    p.stiffness2 = np.float32(0.5 * p.stiffness1)
+    soft = [1, 4, 6]
+    if opt == "soft_walls":
+        p.recalc_r0 = 1
+        p.daught_same_stiffness = 0
+        p.phase_count = 120
+        p.use_rigid_box = 1
+        p.thresh_dist = np.float32(0.1)
+        live = np.zeros((n, 192), bool)
+        live[:, :180] = True
+        for a, arr in enumerate((X, Y, Z)):  # lowest node 0.03 above the wall at 0
+            v = arr.reshape(n, 192)
+            v[live] += np.float32(0.03) - v[live].min()
+            p.box_max[a] = float(v[live].max() + np.float32(0.03))
+    else:
+        p.recalc_r0 = 0
+        p.daught_same_stiffness = 1
+        p.check_sphericity = 0
+        p.phase_count = 10 ** 6
+    Ecell = np.full(mc, p.stiffness1, np.float32)
+    Ecell[soft] = p.stiffness2
+    sim = O.Sim(p, oracle_mesh, X, Y, Z, n, max_cells=mc, budget=1 << 16)
+    sim.set(A.F_STIFFNESS, Ecell)
+    sim.init()
+    ctx = E.Context(oracle_mesh, capacity_cells=mc + 16, voxel_budget=(1 << 16) * 70)
+    s = ctx.add_system(p, X, Y, Z, n, max_cells=mc, voxels=1 << 16)
+    s.set(A.F_STIFFNESS, Ecell[:n])
+    s.startup()
+    _cmp(sim, s, [A.F_FCX, A.F_FCY, A.F_FCZ, A.F_FDX, A.F_FDY, A.F_FDZ], n, label=opt + " start-up")
+    total = 0
+    for k in range(6):
+        sim.step(50)
+        s.step(50)
+        ctx.sync()
+        total += 50
+        n1, st1, status = s.info()
+        print("%s: %d steps, gpu %d cells status 0x%x, oracle %d cells" % (opt, total, n1, status, sim.n), flush=True)
+        assert status & A.SYS_FATAL_MASK == 0
+        assert n1 == sim.n and st1 == sim.stepno
+    nn = sim.n
+    assert nn > n, "no division happened; the test does not exercise the branches"
+    _cmp(sim, s, [A.F_X, A.F_Y, A.F_Z, A.F_VX, A.F_VY, A.F_VZ, A.F_FCX, A.F_FCY, A.F_FCZ, A.F_PRESSURE,
+                  A.F_VOLUME], nn, tol=1e-4, label=opt)
+    e1, e0 = s.get(A.F_STIFFNESS)[:nn], sim.get(A.F_STIFFNESS)[:nn]
+    assert np.array_equal(e1, e0)
+    if opt == "soft_walls":
+        assert np.all(s.get(A.F_PRESSURE)[soft] == 0)  # soft cells lose their pressure after phase_count
+        assert np.all(e1[n:] == p.stiffness1) and np.all(e1[soft] == p.stiffness2)
+        x = s.get(A.F_X).reshape(-1, 192)[:nn, :180]
+        assert x.min() < p.thresh_dist, "no node near the lower wall"
+    else:
+        assert np.sum(e1 == p.stiffness2) > len(soft), "no soft daughter"
+    ctx.close()
