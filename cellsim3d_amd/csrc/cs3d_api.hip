@@ -27,7 +27,11 @@ void launch_com_shift(const Pool& p, const Range& r, hipStream_t st);
 void launch_grid(const Pool& p, const Range& r, int vox_blocks, hipStream_t st);
 void launch_vox_lists(const Pool& p, const Range& r, int vox_blocks, hipStream_t st);
 void launch_vrange(const Pool& p, const Range& r, hipStream_t st);
-void launch_velocity_ab(const Pool& p, const Range& r, hipStream_t st);
+void launch_velocity_ab(const Pool& p, const Range& r, int which, hipStream_t st);
+void launch_cellbox(const Pool& p, const Range& r, hipStream_t st);
+void launch_extent(const Pool& p, const Range& r, float* out, hipStream_t st);
+void launch_pressure_reset(const Pool& p, const Range& r, const int32_t* cells, int count, float value,
+                           hipStream_t st);
 void launch_set_flags(const Pool& p, const Range& r, int rebuild, int search, hipStream_t st);
 void launch_copy_calibrate(const void* src, void* dst, int64_t n4, hipStream_t st);
 }  // namespace cs3d
@@ -96,6 +100,8 @@ struct cs3d_ctx {
     int graph_steps = 10;
     std::map<int, std::pair<hipGraphExec_t, int>> graphs;  // target -> (exec, steps)
     cs3d_mesh mesh_host;  // the tables the context was created with (checkpoint check)
+    void* scratch = nullptr;  // small device buffer of the per-kernel entry points (grown on demand)
+    size_t scratch_bytes = 0;
 };
 
 namespace {
@@ -601,6 +607,35 @@ extern "C" cs3d_status cs3d_get(cs3d_ctx* c, cs3d_sys s, cs3d_field f, void* hos
     return CS3D_OK;
 }
 
+extern "C" cs3d_status cs3d_get_state(cs3d_ctx* c, cs3d_sys s, float* xyz, float* v, float* P, float* CM, float* V,
+                                      int32_t* n_cells, uint32_t* status) {
+    if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
+    SysMeta m;
+    cs3d_status st = read_meta(c, s, m);
+    if (st != CS3D_OK) return st;
+    const size_t nn = (size_t)m.n * kSlots, nc = (size_t)m.n;
+    auto blocks = [&](float* dst, const cs3d_field* f, int k, size_t count) -> cs3d_status {
+        if (!dst || count == 0) return CS3D_OK;
+        for (int a = 0; a < k; ++a) {
+            const cs3d_status e = cs3d_get(c, s, f[a], dst + a * count, count * sizeof(float));
+            if (e != CS3D_OK) return e;
+        }
+        return CS3D_OK;
+    };
+    const cs3d_field fx[3] = {CS3D_F_X, CS3D_F_Y, CS3D_F_Z};
+    const cs3d_field fv[3] = {CS3D_F_VX, CS3D_F_VY, CS3D_F_VZ};
+    const cs3d_field fc[3] = {CS3D_F_CMX, CS3D_F_CMY, CS3D_F_CMZ};
+    const cs3d_field fp[1] = {CS3D_F_PRESSURE};
+    const cs3d_field fo[1] = {CS3D_F_VOLUME};
+    if ((st = blocks(xyz, fx, 3, nn)) != CS3D_OK || (st = blocks(v, fv, 3, nn)) != CS3D_OK ||
+        (st = blocks(P, fp, 1, nc)) != CS3D_OK || (st = blocks(CM, fc, 3, nc)) != CS3D_OK ||
+        (st = blocks(V, fo, 1, nc)) != CS3D_OK)
+        return st;
+    if (n_cells) *n_cells = m.n;
+    if (status) *status = m.status;
+    return CS3D_OK;
+}
+
 extern "C" cs3d_status cs3d_set(cs3d_ctx* c, cs3d_sys s, cs3d_field f, const void* host, size_t bytes) {
     if (!valid_sys(c, s) || !host) return fail(CS3D_INVALID_ARG, "bad set arguments");
     FieldLoc L = field_loc(c, s, f);
@@ -931,6 +966,29 @@ cs3d_status finish(cs3d_ctx* c) {
     HIPCHK(hipStreamSynchronize(c->stream));
     return CS3D_OK;
 }
+
+// The context's scratch buffer with room for `bytes` (the stream is idle when an
+// entry point asks for it: every entry point ends with finish()).
+cs3d_status scratch_of(cs3d_ctx* c, size_t bytes, void** out) {
+    if (c->scratch_bytes < bytes) {
+        if (c->scratch) {
+            c->allocs.erase(std::find(c->allocs.begin(), c->allocs.end(), c->scratch));
+            (void)hipFree(c->scratch);
+            c->scratch = nullptr;
+            c->scratch_bytes = 0;
+        }
+        const size_t want = std::max<size_t>(bytes, 4096);
+        hipError_t e = hipMalloc(&c->scratch, want);
+        if (e != hipSuccess) {
+            c->scratch = nullptr;
+            return fail(CS3D_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+        }
+        c->allocs.push_back(c->scratch);
+        c->scratch_bytes = want;
+    }
+    *out = c->scratch;
+    return CS3D_OK;
+}
 }  // namespace
 
 extern "C" cs3d_status cs3d_k_integrate(cs3d_ctx* c, cs3d_sys s) {
@@ -977,7 +1035,36 @@ extern "C" cs3d_status cs3d_k_dis_force(cs3d_ctx* c, cs3d_sys s, int32_t which) 
 
 extern "C" cs3d_status cs3d_k_velocity_ab(cs3d_ctx* c, cs3d_sys s) {
     if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
-    launch_velocity_ab(c->pool, range_of(c, s), c->stream);
+    launch_velocity_ab(c->pool, range_of(c, s), 3, c->stream);
+    return finish(c);
+}
+
+extern "C" cs3d_status cs3d_k_velocity_a(cs3d_ctx* c, cs3d_sys s) {
+    if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
+    launch_velocity_ab(c->pool, range_of(c, s), 1, c->stream);
+    return finish(c);
+}
+
+extern "C" cs3d_status cs3d_k_velocity_b(cs3d_ctx* c, cs3d_sys s) {
+    if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
+    launch_velocity_ab(c->pool, range_of(c, s), 2, c->stream);
+    return finish(c);
+}
+
+extern "C" cs3d_status cs3d_k_pressure_reset(cs3d_ctx* c, cs3d_sys s, const int32_t* cells, int32_t count,
+                                             float value) {
+    if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
+    if (count < 0 || (count > 0 && !cells)) return fail(CS3D_INVALID_ARG, "bad cell list");
+    if (count == 0) return CS3D_OK;
+    SysMeta m;
+    cs3d_status st = read_meta(c, s, m);
+    if (st != CS3D_OK) return st;
+    for (int32_t i = 0; i < count; ++i)
+        if (cells[i] < 0 || cells[i] >= m.n) return fail(CS3D_INVALID_ARG, "cell index out of range");
+    void* buf = nullptr;
+    if ((st = scratch_of(c, (size_t)count * sizeof(int32_t), &buf)) != CS3D_OK) return st;
+    HIPCHK(hipMemcpyAsync(buf, cells, (size_t)count * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    launch_pressure_reset(c->pool, range_of(c, s), (const int32_t*)buf, count, value, c->stream);
     return finish(c);
 }
 
@@ -990,6 +1077,43 @@ extern "C" cs3d_status cs3d_k_com(cs3d_ctx* c, cs3d_sys s) {
 extern "C" cs3d_status cs3d_k_volumes(cs3d_ctx* c, cs3d_sys s, int64_t step) {
     if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
     launch_dis(c->pool, range_of(c, s), M_VOLUMES, step, 0, false, c->stream);
+    return finish(c);
+}
+
+extern "C" cs3d_status cs3d_k_bboxes(cs3d_ctx* c, cs3d_sys s) {
+    if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
+    SysMeta m0;
+    cs3d_status st = read_meta(c, s, m0);
+    if (st != CS3D_OK) return st;
+    const Range r = range_of(c, s);
+    launch_set_flags(c->pool, r, 1, -1, c->stream);  // the box kernel works on systems flagged for a rebuild
+    launch_cellbox(c->pool, r, c->stream);
+    if ((st = finish(c)) != CS3D_OK) return st;
+    // restore the flag (a grid rebuilt at the end of the last step may still have
+    // its per-cell part pending for the next step), and put the system extent the
+    // box kernel accumulated back to the minmaxpre sentinels
+    SysMeta m;
+    if ((st = read_meta(c, s, m)) != CS3D_OK) return st;
+    m.rebuild = m0.rebuild;
+    for (int a = 0; a < 3; ++a) {
+        float lo = 1.0E8f, hi = -1.0E8f;
+        int32_t il, ih;
+        std::memcpy(&il, &lo, 4);
+        std::memcpy(&ih, &hi, 4);
+        m.ext[2 * a] = il >= 0 ? il : il ^ 0x7FFFFFFF;
+        m.ext[2 * a + 1] = ih >= 0 ? ih : ih ^ 0x7FFFFFFF;
+    }
+    return write_meta(c, s, m);
+}
+
+extern "C" cs3d_status cs3d_k_extent(cs3d_ctx* c, cs3d_sys s, float* out6) {
+    if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
+    if (!out6) return fail(CS3D_INVALID_ARG, "NULL output");
+    void* buf = nullptr;
+    cs3d_status st = scratch_of(c, 6 * sizeof(float), &buf);
+    if (st != CS3D_OK) return st;
+    launch_extent(c->pool, range_of(c, s), (float*)buf, c->stream);
+    HIPCHK(hipMemcpyAsync(out6, buf, 6 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     return finish(c);
 }
 

@@ -1701,7 +1701,8 @@ __global__ __launch_bounds__(256) void k_vox_clear(Pool p, Range r) {
 }
 
 // VelocityUpdateA then VelocityUpdateB (propagate.cu:714-736) on V (per-kernel entry point)
-__global__ __launch_bounds__(192) void k_velocity_ab(Pool p, Range r) {
+// which: 1 = VelocityUpdateA, 2 = VelocityUpdateB, 3 = A then B
+__global__ __launch_bounds__(192) void k_velocity_ab(Pool p, Range r, int32_t which) {
     const int tid = threadIdx.x;
     for (int slot = r.slot_lo + xcd_block(); slot < r.slot_hi; slot += gridDim.x) {
         const int s = sys_of(p, r, slot);
@@ -1716,13 +1717,57 @@ __global__ __launch_bounds__(192) void k_velocity_ab(Pool p, Range r) {
         const float fd[3] = {p.FDX[gi], p.FDY[gi], p.FDZ[gi]};
         const bool rnd = P->add_rands != 0;  // the stored random force (d_fRanList)
         const float fr[3] = {rnd ? p.FRX[gi] : 0.f, rnd ? p.FRY[gi] : 0.f, rnd ? p.FRZ[gi] : 0.f};
+        if (which & 1) {  // VelocityUpdateA (propagate.cu:714-725)
 #pragma unroll
-        for (int a = 0; a < 3; ++a)
-            v[a] = (float)((double)v[a] + 0.5 * (double)(dt * fc[a] + rdt * fr[a]) / (double)m);
+            for (int a = 0; a < 3; ++a)
+                v[a] = (float)((double)v[a] + 0.5 * (double)(dt * fc[a] + rdt * fr[a]) / (double)m);
+        }
+        if (which & 2) {  // VelocityUpdateB (propagate.cu:727-736)
 #pragma unroll
-        for (int a = 0; a < 3; ++a) v[a] = (float)((double)v[a] + 0.5 * (double)dt * (double)fd[a] / (double)m);
+            for (int a = 0; a < 3; ++a) v[a] = (float)((double)v[a] + 0.5 * (double)dt * (double)fd[a] / (double)m);
+        }
         p.VX[gi] = v[0]; p.VY[gi] = v[1]; p.VZ[gi] = v[2];
     }
+}
+
+// minmaxpre + minmaxpost (propagatebound.cu:145-204, 266-323) over the cell boxes
+// in BBOX of system r.sys_lo: {minx, maxx, miny, maxy, minz, maxz} from the
+// sentinels +-1e8 with fminf / fmaxf, into out[6].  One workgroup.
+__global__ __launch_bounds__(256) void k_extent(Pool p, Range r, float* __restrict__ out) {
+    __shared__ float red[6][256];
+    const int tid = threadIdx.x;
+    const SysMeta* M = p.meta + r.sys_lo;
+    float e[6] = {1.0E8f, -1.0E8f, 1.0E8f, -1.0E8f, 1.0E8f, -1.0E8f};
+    for (int c = tid; c < M->n; c += blockDim.x) {
+        const float* b = p.BBOX + (int64_t)(M->base + c) * 6;
+#pragma unroll
+        for (int a = 0; a < 6; a += 2) {
+            e[a] = fminf(e[a], b[a]);
+            e[a + 1] = fmaxf(e[a + 1], b[a + 1]);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 6; ++a) red[a][tid] = e[a];
+    __syncthreads();
+    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+        if (tid < h) {
+#pragma unroll
+            for (int a = 0; a < 6; a += 2) {
+                red[a][tid] = fminf(red[a][tid], red[a][tid + h]);
+                red[a + 1][tid] = fmaxf(red[a + 1][tid], red[a + 1][tid + h]);
+            }
+        }
+        __syncthreads();
+    }
+    if (tid < 6) out[tid] = red[tid][0];
+}
+
+// PressureReset (PressureKernels.cu:45-65): pressure of the listed cells (local
+// indices of system r.sys_lo, checked on the host) set to value
+__global__ __launch_bounds__(256) void k_pressure_reset(Pool p, Range r, const int32_t* __restrict__ cells,
+                                                        int32_t count, float value) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p.P[p.meta[r.sys_lo].base + cells[i]] = value;
 }
 
 // plain streaming copy used to calibrate HBM byte counters
@@ -1828,8 +1873,20 @@ void launch_vox_lists(const Pool& p, const Range& r, int vox_blocks, hipStream_t
     launch(k_insert, dim3(cb > 0 ? cb : 1), dim3(256), st, p, r, 1);
     launch(k_sort, dim3(vox_blocks, ns > 0 ? ns : 1), dim3(256), st, p, r, 1);
 }
-void launch_velocity_ab(const Pool& p, const Range& r, hipStream_t st) {
-    launch(k_velocity_ab, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), st, p, r);
+void launch_velocity_ab(const Pool& p, const Range& r, int which, hipStream_t st) {
+    launch(k_velocity_ab, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), st, p, r, which);
+}
+void launch_cellbox(const Pool& p, const Range& r, hipStream_t st) {
+    const int n = r.slot_hi - r.slot_lo;
+    const int bb = (n + kBoxCells - 1) / kBoxCells;
+    launch(k_cellbox, dim3(bb > 0 ? bb : 1), dim3(192), st, p, r);
+}
+void launch_extent(const Pool& p, const Range& r, float* out, hipStream_t st) {
+    launch(k_extent, dim3(1), dim3(256), st, p, r, out);
+}
+void launch_pressure_reset(const Pool& p, const Range& r, const int32_t* cells, int count, float value,
+                           hipStream_t st) {
+    launch(k_pressure_reset, dim3((count + 255) / 256), dim3(256), st, p, r, cells, count, value);
 }
 void launch_set_flags(const Pool& p, const Range& r, int rebuild, int search, hipStream_t st) {
     const int ns = r.sys_hi - r.sys_lo;

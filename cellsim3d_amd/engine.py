@@ -65,6 +65,12 @@ SIGNATURES = {
     "cs3d_k_con_force_bonded": ([vp, C.c_int32], st_t),
     "cs3d_k_dis_force": ([vp, C.c_int32, C.c_int32], st_t),
     "cs3d_k_velocity_ab": ([vp, C.c_int32], st_t),
+    "cs3d_k_velocity_a": ([vp, C.c_int32], st_t),
+    "cs3d_k_velocity_b": ([vp, C.c_int32], st_t),
+    "cs3d_k_pressure_reset": ([vp, C.c_int32, vp, C.c_int32, C.c_float], st_t),
+    "cs3d_k_bboxes": ([vp, C.c_int32], st_t),
+    "cs3d_k_extent": ([vp, C.c_int32, vp], st_t),
+    "cs3d_get_state": ([vp, C.c_int32, vp, vp, vp, vp, vp, C.POINTER(C.c_int32), C.POINTER(C.c_uint32)], st_t),
     "cs3d_k_com": ([vp, C.c_int32], st_t),
     "cs3d_k_volumes": ([vp, C.c_int32, C.c_int64], st_t),
     "cs3d_k_grid": ([vp, C.c_int32], st_t),
@@ -320,6 +326,39 @@ class System:
 
     def k_velocity_ab(self):
         check(lib().cs3d_k_velocity_ab(self.ctx.h, self.id), "cs3d_k_velocity_ab")
+
+    def k_velocity_a(self):
+        check(lib().cs3d_k_velocity_a(self.ctx.h, self.id), "cs3d_k_velocity_a")
+
+    def k_velocity_b(self):
+        check(lib().cs3d_k_velocity_b(self.ctx.h, self.id), "cs3d_k_velocity_b")
+
+    def k_pressure_reset(self, cells, value):
+        idx = np.ascontiguousarray(cells, dtype=np.int32)
+        check(lib().cs3d_k_pressure_reset(self.ctx.h, self.id, idx.ctypes.data_as(vp), len(idx), float(value)),
+              "cs3d_k_pressure_reset")
+
+    def k_bboxes(self):
+        check(lib().cs3d_k_bboxes(self.ctx.h, self.id), "cs3d_k_bboxes")
+
+    def k_extent(self):
+        """{minx, maxx, miny, maxy, minz, maxz} of the cell boxes (minmaxpre/post)."""
+        out = np.zeros(6, dtype=np.float32)
+        check(lib().cs3d_k_extent(self.ctx.h, self.id, out.ctypes.data_as(vp)), "cs3d_k_extent")
+        return out
+
+    def get_state(self):
+        """Positions, velocities (each (3, n*192)), pressure, centres (3, n), volumes, n and status."""
+        n = self.info()[0]
+        xyz = np.zeros((3, n * 192), np.float32)
+        v = np.zeros((3, n * 192), np.float32)
+        P, V = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        CM = np.zeros((3, n), np.float32)
+        nc, st = C.c_int32(), C.c_uint32()
+        check(lib().cs3d_get_state(self.ctx.h, self.id, xyz.ctypes.data_as(vp), v.ctypes.data_as(vp),
+                                   P.ctypes.data_as(vp), CM.ctypes.data_as(vp), V.ctypes.data_as(vp),
+                                   C.byref(nc), C.byref(st)), "cs3d_get_state")
+        return {"xyz": xyz, "v": v, "P": P, "CM": CM, "V": V, "n": nc.value, "status": st.value}
 
     def k_com(self):
         check(lib().cs3d_k_com(self.ctx.h, self.id), "cs3d_k_com")

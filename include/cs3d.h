@@ -245,6 +245,15 @@ cs3d_status cs3d_system_load(cs3d_ctx* ctx, const char* path, int32_t max_cells,
 cs3d_status cs3d_get(cs3d_ctx* ctx, cs3d_sys sys, cs3d_field f, void* host, size_t bytes);
 cs3d_status cs3d_set(cs3d_ctx* ctx, cs3d_sys sys, cs3d_field f, const void* host, size_t bytes);
 size_t cs3d_field_bytes(cs3d_ctx* ctx, cs3d_sys sys, cs3d_field f);
+/* The state a caller of the reference reads back after a run, in one call:
+ * positions xyz and velocities v as three SoA blocks of n_cells*192 floats
+ * (x block, then y, then z), pressure P[n_cells], centres of mass CM as three
+ * blocks of n_cells floats, volumes V[n_cells], the live cell count and the
+ * sticky status.  Every pointer may be NULL; the buffers are sized for the
+ * system's current cell count (cs3d_system_info).  Replaces the cudaMemcpy
+ * block before the writers (GPUbounce.cu:1007-1048). */
+cs3d_status cs3d_get_state(cs3d_ctx* ctx, cs3d_sys sys, float* xyz, float* v, float* P, float* CM, float* V,
+                           int32_t* n_cells, uint32_t* status);
 
 /* ---- stepping ----------------------------------------------------------- */
 /* Runs n_steps of the reference step loop (GPUbounce.cu:1055-1506) on one
@@ -305,10 +314,29 @@ cs3d_status cs3d_k_con_force_bonded(cs3d_ctx* ctx, cs3d_sys sys);
 cs3d_status cs3d_k_dis_force(cs3d_ctx* ctx, cs3d_sys sys, int32_t which);
 /* VelocityUpdateA then VelocityUpdateB (propagate.cu:714-736) on V. */
 cs3d_status cs3d_k_velocity_ab(cs3d_ctx* ctx, cs3d_sys sys);
+/* VelocityUpdateA alone (propagate.cu:714-725): V += 0.5 (dt F_con + sqrt(dt) F_ran) / m, in double. */
+cs3d_status cs3d_k_velocity_a(cs3d_ctx* ctx, cs3d_sys sys);
+/* VelocityUpdateB alone (propagate.cu:727-736): V += 0.5 dt F_dis / m, in double. */
+cs3d_status cs3d_k_velocity_b(cs3d_ctx* ctx, cs3d_sys sys);
+/* PressureReset (PressureKernels.cu:45-65): the pressure of cells[0 .. count) (local
+ * indices, each below the live cell count; INVALID_ARG otherwise) set to value.
+ * The reference also overwrites its index list with -1; the host list here is
+ * only read. */
+cs3d_status cs3d_k_pressure_reset(cs3d_ctx* ctx, cs3d_sys sys, const int32_t* cells, int32_t count,
+                                  float value);
 /* CenterOfMass (centermass.cu:8-50). */
 cs3d_status cs3d_k_com(cs3d_ctx* ctx, cs3d_sys sys);
 /* volumes (volume.cu:5-163) for step `step`: VOLUME, AREA, DIVFLAG. */
 cs3d_status cs3d_k_volumes(cs3d_ctx* ctx, cs3d_sys sys, int64_t step);
+/* bounding_boxes alone (propagatebound.cu:10-141): the box of each live cell's
+ * nodes {minx, maxx, miny, maxy, minz, maxz} -> CS3D_F_BBOX.  The voxel ranges of
+ * the contact rule follow the boxes only at the next cs3d_k_grid (as makeNNlist
+ * follows bounding_boxes in the reference step). */
+cs3d_status cs3d_k_bboxes(cs3d_ctx* ctx, cs3d_sys sys);
+/* minmaxpre + minmaxpost (propagatebound.cu:145-204, 266-323) over the boxes in
+ * CS3D_F_BBOX: out6 = {minx, maxx, miny, maxy, minz, maxz} from the sentinels
+ * +-1e8 (GPUbounce.cu:1331 copies the same six floats to the host). */
+cs3d_status cs3d_k_extent(cs3d_ctx* ctx, cs3d_sys sys, float* out6);
 /* bounding_boxes + minmaxpre/post + grid dims + makeNNlist
  * (propagatebound.cu:10-323, GPUbounce.cu:1312-1338) with ascending voxel lists. */
 cs3d_status cs3d_k_grid(cs3d_ctx* ctx, cs3d_sys sys);

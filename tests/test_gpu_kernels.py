@@ -97,3 +97,61 @@ def test_integrate_velocity_com_volumes_pressure(oracle_mesh):
     Pn = K.pressure_update(n, p, P, E, 5, np.float32(O.lib().or_rgrowth(C.byref(p), n)))
     _check("pressure", s.get(abi.F_PRESSURE)[:n], Pn)
     ctx.close()
+
+
+def test_split_velocity_boxes_extent_pressure_reset_state(oracle_mesh):
+    """VelocityUpdateA and VelocityUpdateB as entry points of their own
+    (propagate.cu:714-736), bounding_boxes (propagatebound.cu:10-141),
+    minmaxpre/post (:145-204, 266-323), PressureReset (PressureKernels.cu:45-65)
+    and the bulk state getter, on injected state."""
+    import oracle_kernels as K
+    from cellsim3d_amd import abi, engine
+    p, X, Y, Z, V, FC, FD, P, n = _state(seed=6)
+    print("split entry points, %d cells" % n, flush=True)
+    ctx, s = _system(oracle_mesh, p, X, Y, Z, n, V=V)
+    for f, A in zip((abi.F_FCX, abi.F_FCY, abi.F_FCZ, abi.F_FDX, abi.F_FDY, abi.F_FDZ), FC + FD):
+        s.set(f, A)
+    # A alone, against the formula in the reference's precision: float products, double kick
+    s.k_velocity_a()
+    dt, m = np.float32(p.dt), np.float32(p.mass)
+    VA = [(np.float64(V[a]) + 0.5 * np.float64(dt * FC[a]) / np.float64(m)).astype(np.float32) for a in range(3)]
+    for a, f in enumerate((abi.F_VX, abi.F_VY, abi.F_VZ)):
+        got = s.get(f)[:n * S]
+        assert np.array_equal(got, VA[a]), "VelocityUpdateA component %d" % a
+    # then B: A followed by B is the oracle's VelocityUpdateA/B pair
+    s.k_velocity_b()
+    VAB = K.velocity_ab(n, p, V, FC, FD)
+    for a, f in enumerate((abi.F_VX, abi.F_VY, abi.F_VZ)):
+        _check("VelocityUpdateA then B, component %d" % a, s.get(f)[:n * S], VAB[a])
+    # boxes of the current positions, then their extent
+    X1 = X.copy()
+    X1.reshape(n, S)[:, :180] += np.float32(0.125)
+    s.set(abi.F_X, X1)
+    s.k_bboxes()
+    bb_ref = K.bboxes(n, X1, Y, Z)
+    assert np.array_equal(s.get(abi.F_BBOX)[:6 * n], bb_ref)
+    assert np.array_equal(s.k_extent(), K.extent(n, bb_ref))
+    # the grid built after the boxes matches the oracle's grid of the same boxes
+    s.k_grid()
+    _, g_ref, _, _ = K.grid(n, p, bb_ref, budget=1 << 14)
+    assert np.array_equal(s.get(abi.F_GRID)[:6], g_ref[:6])
+    # PressureReset of three cells; the others keep their pressure
+    s.set(abi.F_PRESSURE, P)
+    cells = [0, 5, n - 1]
+    s.k_pressure_reset(cells, 12.5)
+    want = P.copy()
+    want[cells] = np.float32(12.5)
+    assert np.array_equal(s.get(abi.F_PRESSURE)[:n], want)
+    with pytest.raises(engine.EngineError):
+        s.k_pressure_reset([n], 1.0)
+    # bulk state = the fields one by one
+    st = s.get_state()
+    assert st["n"] == n and st["status"] == s.info()[2]
+    for a, f in enumerate((abi.F_X, abi.F_Y, abi.F_Z)):
+        assert np.array_equal(st["xyz"][a], s.get(f)[:n * S])
+    for a, f in enumerate((abi.F_VX, abi.F_VY, abi.F_VZ)):
+        assert np.array_equal(st["v"][a], s.get(f)[:n * S])
+    for a, f in enumerate((abi.F_CMX, abi.F_CMY, abi.F_CMZ)):
+        assert np.array_equal(st["CM"][a], s.get(f)[:n])
+    assert np.array_equal(st["P"], want) and np.array_equal(st["V"], s.get(abi.F_VOLUME)[:n])
+    ctx.close()
