@@ -63,16 +63,33 @@ def cpu_baseline(cfg, mesh_tables, rest, budget_s=20.0):
     sim.step(1)
     one = time.perf_counter() - t0
     k = max(1, min(200, int(budget_s / max(one, 1e-6))))
+    ph0 = sim.phase_seconds()
     t0 = time.perf_counter()
     sim.step(k)
     dt = time.perf_counter() - t0
+    ph = (sim.phase_seconds() - ph0) / k
+    threads = O.get_threads()
+    # one thread, on a short sample (a few seconds) of the same run
+    O.set_threads(1)
+    try:
+        t0 = time.perf_counter()
+        sim.step(1)
+        one1 = time.perf_counter() - t0
+        k1 = max(1, min(50, int(0.25 * budget_s / max(one1, 1e-6))))
+        t0 = time.perf_counter()
+        sim.step(k1)
+        dt1 = time.perf_counter() - t0
+    finally:
+        O.set_threads(threads)
     return {
         "value": n * 180.0 * k / dt,
         "unit": "node-updates/s",
-        "cores": O.get_threads(),
+        "cores": threads,
         "kind": "port",
         "sample": "oracle (C++/OpenMP restatement of the reference kernels), same 2000-cell input, "
                   "%d steps after 2 warm-up steps (%.1f s)" % (k, dt),
+        "ms_per_step_by_part": {name: round(1e3 * v, 3) for name, v in zip(O.Sim.PHASES, ph)},
+        "single_thread": {"value": n * 180.0 * k1 / dt1, "sample": "%d steps (%.1f s)" % (k1, dt1)},
     }
 
 

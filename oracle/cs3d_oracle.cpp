@@ -8,6 +8,7 @@
 #include "cs3d_oracle.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -793,6 +794,11 @@ struct SimBase {
     virtual ~SimBase() {}
     virtual uint32_t init() = 0;
     virtual uint32_t step(int k) = 0;
+    // wall seconds spent per part of the step since construction (CPU baseline breakdown):
+    // 0 Integrate + PressureUpdate, 1 CalculateConForce, 2 both CalculateDisForce,
+    // 3 random force + VelocityUpdateA/B, 4 CenterOfMass + volumes + division,
+    // 5 grid rebuild + CorrectCoMMotion
+    double phase_s[6] = {0, 0, 0, 0, 0, 0};
     virtual size_t get(int f, void* out, size_t bytes) const = 0;
     virtual size_t set(int f, const void* in, size_t bytes) = 0;
     virtual size_t get64(int f, double* out, size_t count) const = 0;
@@ -921,6 +927,13 @@ template <class T> struct Sim : SimBase {
     // one pass of the step loop, GPUbounce.cu:1055-1506
     uint32_t step(int k) override {
         uint32_t st = 0;
+        using clk = std::chrono::steady_clock;
+        auto mark = clk::now();
+        auto lap = [&](int q) {
+            const auto t = clk::now();
+            phase_s[q] += std::chrono::duration<double>(t - mark).count();
+            mark = t;
+        };
         for (int s = 0; s < k; ++s) {
             ++stepno;
             const bool rnd = p.add_rands != 0;  // Integrate uses the previous step's draw (d_fRanList)
@@ -930,8 +943,11 @@ template <class T> struct Sim : SimBase {
             VAX = VX; VAY = VY; VAZ = VZ;
             T rg = rgrowth<T>(&p, n);
             pressure_update<T>(n, &p, P.data(), E.data(), gscale.data(), stepno, rg);
+            lap(0);
             st |= conforce();
+            lap(1);
             st |= disforce();
+            lap(2);
             if (rnd) {  // CalculateRanForce, GPUbounce.cu:1133-1137
                 for (int c = 0; c < n; ++c)
                     for (int i = 0; i < NN; ++i) {
@@ -946,7 +962,9 @@ template <class T> struct Sim : SimBase {
             velocity_ab<T>(n, &p, VX.data(), VY.data(), VZ.data(), FCX.data(), FCY.data(), FCZ.data(), FDX.data(),
                            FDY.data(), FDZ.data(), rnd ? FRX.data() : nullptr, rnd ? FRY.data() : nullptr,
                            rnd ? FRZ.data() : nullptr);
+            lap(3);
             st |= disforce();
+            lap(2);
             com<T>(n, X.data(), Y.data(), Z.data(), CMx.data(), CMy.data(), CMz.data());
             bool divided = false;
             if (stepno <= p.div_time_steps && rg > T(0)) {
@@ -954,6 +972,7 @@ template <class T> struct Sim : SimBase {
                                  E.data(), vol.data(), area.data(), flag.data());
                 st |= divide(divided);
             }
+            lap(4);
             int gi = p.grid_interval > 0 ? p.grid_interval : 1;
             if (divided || stepno % gi == 0) st |= rebuild_grid();
             if (p.correct_com) {
@@ -970,6 +989,7 @@ template <class T> struct Sim : SimBase {
                     }
                 for (int a = 0; a < 3; ++a) sys_cm[a] = (float)c3[a];
             }
+            lap(5);
             status |= st;
             if (st & CS3D_SYS_FATAL_MASK) break;
         }
@@ -1078,6 +1098,9 @@ extern "C" void or_sim_destroy(or_sim* s) {
 }
 extern "C" uint32_t or_sim_init(or_sim* s) { return s->impl->init(); }
 extern "C" uint32_t or_sim_step(or_sim* s, int k) { return s->impl->step(k); }
+extern "C" void or_sim_phase_seconds(const or_sim* s, double* out6) {
+    for (int q = 0; q < 6; ++q) out6[q] = s->impl->phase_s[q];
+}
 extern "C" int or_sim_ncells(const or_sim* s) { return s->impl->n; }
 extern "C" int64_t or_sim_stepno(const or_sim* s) { return s->impl->stepno; }
 extern "C" size_t or_sim_get(const or_sim* s, int f, void* out, size_t bytes) { return s->impl->get(f, out, bytes); }

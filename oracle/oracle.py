@@ -67,6 +67,7 @@ def lib():
         L.or_sim_init.restype = C.c_uint32
         L.or_sim_step.argtypes = [vp, C.c_int]
         L.or_sim_step.restype = C.c_uint32
+        L.or_sim_phase_seconds.argtypes = [vp, np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")]
         L.or_sim_ncells.argtypes = [vp]
         L.or_sim_stepno.argtypes = [vp]
         L.or_sim_stepno.restype = C.c_int64
@@ -159,6 +160,14 @@ class Sim:
 
     def step(self, k=1):
         return lib().or_sim_step(self.h, k)
+
+    PHASES = ("integrate_pressure", "con_force", "dis_force", "velocity", "com_volumes_division", "grid")
+
+    def phase_seconds(self):
+        """Wall seconds per part of the step since the simulation was created (see PHASES)."""
+        out = np.zeros(6, dtype=np.float64)
+        lib().or_sim_phase_seconds(self.h, out)
+        return out
 
     @property
     def n(self):
