@@ -223,6 +223,10 @@ def main():
         raise SystemExit("system status 0x%x after the timed run" % status)
     value = n * 180.0 * args.steps / elapsed
     log("bench: %d steps in %.4f s -> %.4g node-updates/s" % (args.steps, elapsed, value))
+    # contact partners per node at the end of the timed run (pairs within attraction_range
+    # that pass the candidate rule): the per-node work of the friction passes
+    pc = s.get(abi.F_PAIRCOUNT)[:n * 192].reshape(n, 192)[:, :180]
+    contacts = {"partners_per_node_mean": round(float(pc.mean()), 3), "partners_per_node_max": int(pc.max())}
 
     # per-phase kernel time from start/stop HIP events each launch records on the
     # engine's stream (eager steps; launch gaps are not counted)
@@ -285,6 +289,7 @@ def main():
             "grid_interval": args.grid_interval,
         },
         "phases_ms_per_step": {k: round(v, 5) for k, v in phases.items()},
+        "contacts": contacts,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
