@@ -155,3 +155,37 @@ def test_split_velocity_boxes_extent_pressure_reset_state(oracle_mesh):
         assert np.array_equal(st["CM"][a], s.get(f)[:n])
     assert np.array_equal(st["P"], want) and np.array_equal(st["V"], s.get(abi.F_VOLUME)[:n])
     ctx.close()
+
+
+def test_refused_calls_leave_the_system_usable(oracle_mesh):
+    """Invalid calls through the C ABI return a status and a message and change
+    nothing: unknown system, more bytes than a field holds, the read-only system
+    centre, inconsistent cell counts, an exhausted cell pool, a NULL output and a
+    cell index past the live cells.  The system then steps as before."""
+    import ctypes as C
+    from cellsim3d_amd import abi, engine
+    L = engine.lib()
+    vp = C.c_void_p
+    p, X, Y, Z, V, FC, FD, P, n = _state(seed=8)
+    print("refused calls, %d cells" % n, flush=True)
+    ctx, s = _system(oracle_mesh, p, X, Y, Z, n)
+    x0 = s.get(abi.F_X).copy()
+    assert L.cs3d_step(ctx.h, 57, 1) == abi.INVALID_ARG
+    big = np.zeros(n * S * 2, np.float32)
+    assert L.cs3d_get(ctx.h, s.id, abi.F_X, big.ctypes.data_as(vp), big.nbytes) == abi.INVALID_ARG
+    assert L.cs3d_set(ctx.h, s.id, abi.F_X, big.ctypes.data_as(vp), big.nbytes) == abi.INVALID_ARG
+    c3 = np.zeros(3, np.float32)
+    assert L.cs3d_set(ctx.h, s.id, abi.F_SYSCM, c3.ctypes.data_as(vp), 12) == abi.INVALID_ARG
+    assert b"read-only" in L.cs3d_last_error()
+    out = C.c_int32()
+    assert L.cs3d_system_add_raw(ctx.h, C.byref(p), 10, 5, 0, C.byref(out)) == abi.INVALID_ARG
+    assert L.cs3d_system_add_raw(ctx.h, C.byref(p), 1, 100, 0, C.byref(out)) == abi.CAPACITY
+    assert L.cs3d_k_extent(ctx.h, s.id, None) == abi.INVALID_ARG
+    idx = np.array([n], np.int32)
+    assert L.cs3d_k_pressure_reset(ctx.h, s.id, idx.ctypes.data_as(vp), 1, C.c_float(1.0)) == abi.INVALID_ARG
+    assert np.array_equal(s.get(abi.F_X), x0)
+    s.step(3)
+    ctx.sync()
+    n1, st1, status = s.info()
+    assert (n1, st1) == (n, 3) and status & abi.SYS_FATAL_MASK == 0
+    ctx.close()

@@ -214,3 +214,24 @@ def test_header_compiles_as_c_and_links(tmp_path):
     assert float(ranmar) == 6533892.0  # marsaglia.h:68-74, first of the six values
     assert (float(cx), float(cy), float(cz)) == (1.0, 89.5, -2.0)
     assert msg.strip() == "capacity exceeded"
+
+
+def test_context_creation_checks_arguments_and_fails_loudly_without_a_gpu():
+    """cs3d_ctx_create refuses bad arguments before touching a device, and on a
+    machine without a visible HIP device it returns CS3D_NO_DEVICE with a message
+    (no CPU fallback).  Runs where no GPU is visible; skipped elsewhere."""
+    import torch
+    if torch.cuda.device_count() > 0:
+        pytest.skip("a GPU is visible: the no-device path is not reachable")
+    L = engine.lib()
+    m = configs.unit_mesh()
+    mesh = engine.build_mesh(M.tables(m), configs.rest_positions(m))
+    h = C.c_void_p()
+    for cap, vox, bins in ((0, 1 << 20, 1 << 22), ((1 << 30) // 192 + 1, 1 << 20, 1 << 22), (64, 0, 1 << 22),
+                           (64, 1 << 20, 0)):
+        assert L.cs3d_ctx_create(0, C.byref(mesh), cap, vox, bins, C.byref(h)) == abi.INVALID_ARG
+    assert L.cs3d_ctx_create(0, None, 64, 1 << 20, 1 << 22, C.byref(h)) == abi.INVALID_ARG
+    assert L.cs3d_ctx_create(0, C.byref(mesh), 64, 1 << 20, 1 << 22, C.byref(h)) == abi.NO_DEVICE
+    assert b"device" in L.cs3d_last_error()
+    with pytest.raises(engine.EngineError):
+        engine.Context(mesh, capacity_cells=64)
