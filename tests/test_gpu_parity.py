@@ -455,7 +455,7 @@ def test_search_list_overflow_is_flagged(oracle_mesh):
     ctx.close()
 
 
-@pytest.mark.parametrize("opt", ["soft_walls", "same_stiffness_no_sphericity"])
+@pytest.mark.parametrize("opt", ["soft_walls", "same_stiffness_no_sphericity", "basis_z", "basis_oblique"])
 def test_parameter_branches_match_oracle(oracle_mesh, opt):
     """Branches off the default parameter set, with growth and division, through the
     fused step (graph path):
@@ -466,7 +466,10 @@ def test_parameter_branches_match_oracle(oracle_mesh, opt):
       daughters taking stiffness1 (GPUbounce.cu:1257-1262), and the rigid-box walls (propagate.cu:473-510)
       with nodes inside the threshold distance of all six walls;
     * same_stiffness_no_sphericity: daughters keep the parent's stiffness and
-      checkSphericity is off (volume.cu:145-153)."""
+      checkSphericity is off (volume.cu:145-153);
+    * basis_z, basis_oblique: division planes from GetRandomVectorBasis
+      (RandomVector.h:26-78, called at GPUbounce.cu:1230-1231) about the basis
+      (0, 0, 1) and about an oblique one, which take its two branches."""
     A, E, O = _abi(), _engine(), _oracle()
     from cellsim3d_amd import configs
     cfg = configs.plain_small(seed=4)
@@ -489,11 +492,17 @@ def test_parameter_branches_match_oracle(oracle_mesh, opt):
             v = arr.reshape(n, 192)
             v[live] += np.float32(0.03) - v[live].min()
             p.box_max[a] = float(v[live].max() + np.float32(0.03))
-    else:
+    elif opt == "same_stiffness_no_sphericity":
         p.recalc_r0 = 0
         p.daught_same_stiffness = 1
         p.check_sphericity = 0
         p.phase_count = 10 ** 6
+    else:
+        p.recalc_r0 = 0
+        p.daught_same_stiffness = 1
+        p.phase_count = 10 ** 6
+        p.use_div_plane_basis = 1
+        p.div_plane_basis[:] = [0.0, 0.0, 1.0] if opt == "basis_z" else [0.2, 0.6, 0.77]
     Ecell = np.full(mc, p.stiffness1, np.float32)
     Ecell[soft] = p.stiffness2
     sim = O.Sim(p, oracle_mesh, X, Y, Z, n, max_cells=mc, budget=1 << 16)
@@ -525,6 +534,6 @@ def test_parameter_branches_match_oracle(oracle_mesh, opt):
         assert np.all(e1[n:] == p.stiffness1) and np.all(e1[soft] == p.stiffness2)
         x = s.get(A.F_X).reshape(-1, 192)[:nn, :180]
         assert x.min() < p.thresh_dist, "no node near the lower wall"
-    else:
+    elif opt == "same_stiffness_no_sphericity":
         assert np.sum(e1 == p.stiffness2) > len(soft), "no soft daughter"
     ctx.close()
