@@ -455,7 +455,8 @@ def test_search_list_overflow_is_flagged(oracle_mesh):
     ctx.close()
 
 
-@pytest.mark.parametrize("opt", ["soft_walls", "same_stiffness_no_sphericity", "basis_z", "basis_oblique"])
+@pytest.mark.parametrize("opt", ["soft_walls", "same_stiffness_no_sphericity", "basis_z", "basis_oblique",
+                                 "pop_model"])
 def test_parameter_branches_match_oracle(oracle_mesh, opt):
     """Branches off the default parameter set, with growth and division, through the
     fused step (graph path):
@@ -469,7 +470,9 @@ def test_parameter_branches_match_oracle(oracle_mesh, opt):
       checkSphericity is off (volume.cu:145-153);
     * basis_z, basis_oblique: division planes from GetRandomVectorBasis
       (RandomVector.h:26-78, called at GPUbounce.cu:1230-1231) about the basis
-      (0, 0, 1) and about an oblique one, which take its two branches."""
+      (0, 0, 1) and about an oblique one, which take its two branches;
+    * pop_model: the growth rate of the population model, rMax (1 - n / maxPop)
+      (GPUbounce.cu:1077-1088), which stops the growth at maxPop cells."""
     A, E, O = _abi(), _engine(), _oracle()
     from cellsim3d_amd import configs
     cfg = configs.plain_small(seed=4)
@@ -497,6 +500,12 @@ def test_parameter_branches_match_oracle(oracle_mesh, opt):
         p.daught_same_stiffness = 1
         p.check_sphericity = 0
         p.phase_count = 10 ** 6
+    elif opt == "pop_model":
+        p.recalc_r0 = 0
+        p.daught_same_stiffness = 1
+        p.phase_count = 10 ** 6
+        p.do_pop_model = 1
+        p.max_pop = np.float32(12.0)
     else:
         p.recalc_r0 = 0
         p.daught_same_stiffness = 1
