@@ -456,7 +456,7 @@ def test_search_list_overflow_is_flagged(oracle_mesh):
 
 
 @pytest.mark.parametrize("opt", ["soft_walls", "same_stiffness_no_sphericity", "basis_z", "basis_oblique",
-                                 "pop_model"])
+                                 "pop_model", "no_angles", "daughter_com_fix"])
 def test_parameter_branches_match_oracle(oracle_mesh, opt):
     """Branches off the default parameter set, with growth and division, through the
     fused step (graph path):
@@ -472,7 +472,10 @@ def test_parameter_branches_match_oracle(oracle_mesh, opt):
       (RandomVector.h:26-78, called at GPUbounce.cu:1230-1231) about the basis
       (0, 0, 1) and about an oblique one, which take its two branches;
     * pop_model: the growth rate of the population model, rMax (1 - n / maxPop)
-      (GPUbounce.cu:1077-1088), which stops the growth at maxPop cells."""
+      (GPUbounce.cu:1077-1088), which stops the growth at maxPop cells;
+    * no_angles: constrainAngles off, no bending forces (propagate.cu:332-337);
+    * daughter_com_fix: the engine's opt-in fix of the stale daughter centre (the
+      centres of parent and daughter from their new nodes right after division)."""
     A, E, O = _abi(), _engine(), _oracle()
     from cellsim3d_amd import configs
     cfg = configs.plain_small(seed=4)
@@ -500,6 +503,14 @@ def test_parameter_branches_match_oracle(oracle_mesh, opt):
         p.daught_same_stiffness = 1
         p.check_sphericity = 0
         p.phase_count = 10 ** 6
+    elif opt in ("no_angles", "daughter_com_fix"):
+        p.recalc_r0 = 0
+        p.daught_same_stiffness = 1
+        p.phase_count = 10 ** 6
+        if opt == "no_angles":
+            p.constrain_angles = 0
+        else:
+            p.daughter_com_fix = 1
     elif opt == "pop_model":
         p.recalc_r0 = 0
         p.daught_same_stiffness = 1
