@@ -14,7 +14,8 @@
 // of other cells within attraction_range + pair_skin, found through periodic
 // search bins of that edge (find_candidates).  The list is rebuilt only on steps where some
 // node has moved pair_skin/2 since the last search, so it always contains every
-// node now within attraction_range.  Every step then applies the reference's own
+// node now within attraction_range; after divisions only the lists near the divided
+// cells are rebuilt (k_search's partial search).  Every step then applies the reference's own
 // tests to the candidates in (cell, node) order -- the order of the reference's
 // loops: the distance test, voxel-list membership and the bbox test
 // (accept_forces).  Voxel-list membership is evaluated arithmetically: makeNNlist
@@ -247,12 +248,65 @@ __device__ int64_t set_grid_geometry(SysMeta* M, const cs3d_params* P, const flo
     return ok ? nv : (int64_t)M->vbudget;
 }
 
+// 1 = local cell c was divided (parent or daughter) in the last division phase of
+// its system: daughters are the last n_events cells, parents are listed in EVL.
+// Uniform over the workgroup.  n_events <= kMaxPartial (one lane per event).
+__device__ __forceinline__ bool divided_cell(const Pool& p, const SysMeta* M, int c) {
+    const int ne = M->n_events;
+    if (c >= M->n - ne) return true;
+    const int lane = threadIdx.x & 63;
+    return __any(lane < ne && p.EVL[M->base + lane] == c);
+}
+
+// {minx, maxx, miny, maxy, minz, maxz} of the node positions of one cell
+// (workgroup of 192 lanes, nodes on lanes < 180) into out[6] and, through the
+// shared part[3][6], to every lane's return in box[6].  Called by all lanes.
+__device__ __forceinline__ void box_of_cell(float x, float y, float z, bool node, float (*part)[6], float* out,
+                                            float* box = nullptr) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    float v[6] = {node ? x : INFINITY, node ? x : -INFINITY, node ? y : INFINITY,
+                  node ? y : -INFINITY, node ? z : INFINITY, node ? z : -INFINITY};
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+        for (int a = 0; a < 6; a += 2) {
+            v[a] = fminf(v[a], __shfl_xor(v[a], o));
+            v[a + 1] = fmaxf(v[a + 1], __shfl_xor(v[a + 1], o));
+        }
+    }
+    if (lane == 0)
+#pragma unroll
+        for (int a = 0; a < 6; ++a) part[wv][a] = v[a];
+    __syncthreads();
+    float b[6];
+#pragma unroll
+    for (int a = 0; a < 6; a += 2) {
+        b[a] = fminf(fminf(part[0][a], part[1][a]), part[2][a]);
+        b[a + 1] = fmaxf(fmaxf(part[0][a + 1], part[1][a + 1]), part[2][a + 1]);
+    }
+    __syncthreads();  // part is reused
+    if (out && tid < 6) {
+        float val = b[0];
+#pragma unroll
+        for (int a = 1; a < 6; ++a)
+            if (tid == a) val = b[a];
+        out[tid] = val;
+    }
+    if (box)
+#pragma unroll
+        for (int a = 0; a < 6; ++a) box[a] = b[a];
+}
+
 // ------------------------------------------------------------------------------
 // Integrate + ForwardTime (propagate.cu:681-712, 738-763), PressureUpdate
 // (PressureKernels.cu:4-41) and insertion of the new positions into the
 // search bins.
 // ------------------------------------------------------------------------------
-__global__ __launch_bounds__(192) void k_integrate(Pool p, Range r, int32_t flags, int64_t step_override) {
+// 6 waves per SIMD (at most 80 VGPRs): 8 workgroups per CU, one round for 2048 cells
+__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_integrate(Pool p, Range r,
+                                                                                              int32_t flags,
+                                                                                              int64_t step_override) {
+    __shared__ float ibox[3][6];
     const int tid = threadIdx.x;
     const bool node = tid < kNodes;
     for (int slot = r.slot_lo + xcd_block(); slot < r.slot_hi; slot += gridDim.x) {
@@ -315,8 +369,14 @@ __global__ __launch_bounds__(192) void k_integrate(Pool p, Range r, int32_t flag
         if ((flags & I_BINS) && (!(flags & I_IFSEARCH) || M->need_search)) {
             // insertion into the search bins: the lanes of a wave that share a bin
             // take one atomic between them (issued by the group's first lane), and
-            // each lane's slot is the group's base plus its rank in the group
-            const int b = node ? bin_of(M, x, y, z) : -1;
+            // each lane's slot is the group's base plus its rank in the group.  A
+            // partial search (after divisions) works on search positions: the nodes of
+            // cells that did not divide are binned at their last search position
+            float qx = x, qy = y, qz = z;
+            if ((flags & I_IFSEARCH) && M->need_search == 2 && node && !divided_cell(p, M, c)) {
+                qx = p.XBX[gi]; qy = p.XBY[gi]; qz = p.XBZ[gi];
+            }
+            const int b = node ? bin_of(M, qx, qy, qz) : -1;
             const int lane = tid & 63;
             uint64_t todo = __ballot(node);
             int leader_of = 0, rank = 0, group = 0;
@@ -338,7 +398,7 @@ __global__ __launch_bounds__(192) void k_integrate(Pool p, Range r, int32_t flag
             if (node) {
                 if (k < kBCap) {
                     float4* ent = reinterpret_cast<float4*>(p.bins + M->ebase);
-                    ent[(int64_t)b * kBCap + k] = make_float4(x, y, z, __int_as_float(c * kSlots + tid));
+                    ent[(int64_t)b * kBCap + k] = make_float4(qx, qy, qz, __int_as_float(c * kSlots + tid));
                 } else {
                     flag(M, CS3D_SYS_BIN_OVERFLOW);
                 }
@@ -361,6 +421,11 @@ __global__ __launch_bounds__(192) void k_integrate(Pool p, Range r, int32_t flag
                 pr += inc;
             }
             p.P[slot] = pr;
+        }
+        if (kick && (flags & I_IFSEARCH) && M->need_search == 2 && divided_cell(p, M, c)) {
+            // a cell divided at the end of the last step publishes the box of its new
+            // positions: the partial search re-searches the cells within reach of it
+            box_of_cell(x, y, z, node, ibox, p.SBOX + (int64_t)slot * 6);
         }
     }
 }
@@ -640,7 +705,10 @@ __device__ __forceinline__ int accept_forces(const int32_t* cand, int nc, int32_
 // ------------------------------------------------------------------------------
 __global__ __launch_bounds__(192) void k_search(Pool p, Range r, int32_t always) {
     __shared__ int32_t lst[kPMax][kSlots];
+    __shared__ float part[3][6];
+    __shared__ int32_t near;
     const int tid = threadIdx.x;
+    const bool node = tid < kNodes;
     if (r.one_sys && !always) {  // one system: its flag decides for every workgroup
         const SysMeta* M0 = p.meta + r.sys_lo;
         if (!M0->need_search || !sys_live(M0)) return;
@@ -652,17 +720,54 @@ __global__ __launch_bounds__(192) void k_search(Pool p, Range r, int32_t always)
         if (!sys_live(M) || !(always || M->need_search)) continue;
         const int c = slot - M->base;
         if (c >= M->n) continue;
-        if (tid >= kNodes) continue;
         const cs3d_params* P = p.prm + s;
         const int64_t gi = (int64_t)slot * kSlots + tid;
-        const float X = p.X[gi], Y = p.Y[gi], Z = p.Z[gi];
-        uint32_t st = 0;
-        const int nc = find_candidates(p, M, P, c, X, Y, Z, lst, tid, st);
-        int32_t* cstore = p.CAND + (int64_t)slot * kPMax * kSlots + tid;
-        for (int i = 0; i < nc; ++i) cstore[i * kSlots] = lst[i][tid];
-        p.NCAND[gi] = nc;
-        p.XBX[gi] = X; p.XBY[gi] = Y; p.XBZ[gi] = Z;
-        if (st) flag(M, st);
+        float X = p.X[gi], Y = p.Y[gi], Z = p.Z[gi];
+        bool reset = true;
+        if (!always && M->need_search == 2) {
+            // Partial search after divisions (no node has left the skin), in search
+            // positions: a node that did not divide is within pair_skin/2 of its last
+            // search position, and a divided node takes its new position as its search
+            // position.  Two nodes within attraction_range now or later (until the next
+            // full search) are thus within attraction_range + pair_skin in search
+            // positions, which is the search radius.  The nodes were binned at their
+            // search positions (k_integrate).  Lists of cells whose search-position box
+            // is farther than that radius from every divided cell stay valid as they are
+            // (they hold every pair they can meet); the other cells search again around
+            // their search positions, so their lists are as long as after a full search.
+            const bool own = divided_cell(p, M, c);
+            if (!own) {
+                X = p.XBX[gi]; Y = p.XBY[gi]; Z = p.XBZ[gi];
+                float b[6];
+                box_of_cell(X, Y, Z, node, part, nullptr, b);
+                const int ne = M->n_events, n0 = M->n - ne;
+                const float mg = P->attraction_range + P->pair_skin;
+                bool hit = false;
+                if (tid < 2 * ne) {
+                    const int q = tid < ne ? p.EVL[M->base + tid] : n0 + (tid - ne);
+                    const float* o = p.SBOX + (int64_t)(M->base + q) * 6;
+                    hit = b[0] - mg <= o[1] && o[0] <= b[1] + mg && b[2] - mg <= o[3] && o[2] <= b[3] + mg &&
+                          b[4] - mg <= o[5] && o[4] <= b[5] + mg;
+                }
+                if (tid == 0) near = 0;
+                __syncthreads();
+                if (hit) near = 1;
+                __syncthreads();
+                const bool go = near != 0;
+                __syncthreads();  // near is reset by the next cell
+                if (!go) continue;
+            }
+            reset = own;
+        }
+        if (node) {
+            uint32_t st = 0;
+            const int nc = find_candidates(p, M, P, c, X, Y, Z, lst, tid, st);
+            int32_t* cstore = p.CAND + (int64_t)slot * kPMax * kSlots + tid;
+            for (int i = 0; i < nc; ++i) cstore[i * kSlots] = lst[i][tid];
+            p.NCAND[gi] = nc;
+            if (reset) { p.XBX[gi] = X; p.XBY[gi] = Y; p.XBZ[gi] = Z; }
+            if (st) flag(M, st);
+        }
     }
 }
 
@@ -1479,8 +1584,12 @@ __global__ __launch_bounds__(256) void k_sys_divide(Pool p, Range r, int32_t mod
         if (step_mode) {
             M->rebuild = rebuild ? 1 : 0;
             M->step = stepno;
-            // divided cells moved, or the next drift leaves the skin: search next step
-            M->need_search = (events > 0 || M->search_next) ? 1 : 0;
+            // the next drift leaves the skin: search every cell next step; cells divided:
+            // search the cells near them, in search positions (a full search after many
+            // divisions, or when CorrectCoMMotion shifts the nodes, whose search
+            // positions are not shifted)
+            const bool full = M->search_next || events > kMaxPartial || (events > 0 && shift);
+            M->need_search = full ? 1 : (events > 0 ? 2 : 0);
             M->search_next = 0;
         } else if (events > 0) {
             M->need_search = 1;
