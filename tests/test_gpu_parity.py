@@ -557,3 +557,41 @@ def test_parameter_branches_match_oracle(oracle_mesh, opt):
     elif opt == "same_stiffness_no_sphericity":
         assert np.sum(e1 == p.stiffness2) > len(soft), "no soft daughter"
     ctx.close()
+
+
+def test_staggered_divisions_in_a_tissue_match_oracle(oracle_mesh):
+    """A 64-cell tissue whose cells grow at different rates (growth scale
+    U[0.5, 1.5]) divide a few at a time over about 20 steps.  After each
+    such step only the cells near a divided cell search their contact candidates
+    again; cells farther away keep their lists.  Cell counts must agree with the
+    oracle after every 5 steps, and the state bit for bit (1e-4 bar) at the end."""
+    A = _abi()
+    from cellsim3d_amd import configs
+    m, X, Y, Z, info = configs.lattice_tissue(64, 0.40, seed=11)
+    p = configs.base_params(11)
+    p.division_vol = np.float32(1.02 * info["v0"])
+    p.div_time_steps = 10 ** 6
+    p.voxel_capacity = 64
+    n, mc = 64, 256
+    gs = np.ones(mc, np.float32)
+    gs[:n] = np.random.default_rng(5).uniform(0.5, 1.5, n).astype(np.float32)
+    sim, ctx, s = _pair(oracle_mesh, p, X, Y, Z, n, mc)
+    sim.set(A.F_GROWTH_SCALE, gs)
+    s.set(A.F_GROWTH_SCALE, gs[:n])
+    ctx.set_graph(False)
+    counts = []
+    for k in range(20):
+        sim.step(5)
+        s.step(5)
+        ctx.sync()
+        n1, st1, status = s.info()
+        assert status & A.SYS_FATAL_MASK == 0
+        assert n1 == sim.n and st1 == sim.stepno, "after %d steps: gpu %d cells, oracle %d" % (st1, n1, sim.n)
+        counts.append(n1)
+    print("cell counts every 5 steps:", counts, flush=True)
+    jumps = [b - a for a, b in zip([n] + counts, counts)]
+    assert counts[-1] > n and sum(1 for j in jumps if j) >= 3 and max(jumps) < n, "divisions not staggered: %s" % counts
+    nn = sim.n
+    _cmp(sim, s, [A.F_X, A.F_Y, A.F_Z, A.F_VX, A.F_VY, A.F_VZ, A.F_FCX, A.F_FDX, A.F_PRESSURE], nn, tol=1e-4,
+         label="staggered divisions")
+    ctx.close()
