@@ -348,6 +348,8 @@ extern "C" cs3d_status cs3d_ctx_create(int device, const cs3d_mesh* mesh, int64_
     if ((st = dalloc(c, p.XBY, ns)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.XBZ, ns)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.SBOX, capacity_cells * 6)) != CS3D_OK) return cleanup(st);
+    if ((st = dalloc(c, p.RESET, capacity_cells)) != CS3D_OK) return cleanup(st);
+    if ((st = dalloc(c, p.RSL, capacity_cells)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.bins, bin_budget)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.EVL, capacity_cells)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.slot_sys, capacity_cells)) != CS3D_OK) return cleanup(st);

@@ -32,7 +32,7 @@ constexpr int kNodes = CS3D_NODES;   // 180
 constexpr int kPMax = 40;            // contact candidates / partners stored per node
 constexpr int kBCap = CS3D_BIN_CAP;  // nodes per search bin
 constexpr int kMaxSys = 4096;        // systems per context
-constexpr int kMaxPartial = 64;      // divisions in one step beyond which the next search is a full one
+constexpr int kMaxPartial = 64;      // listed cells beyond which the next search is a full one
 
 // Per-system state kept on the device.
 struct SysMeta {
@@ -60,10 +60,11 @@ struct SysMeta {
     float bmin[3];      // bin grid origin
     float bh;           // bin edge
     int32_t bdims[3];   // bins per axis
-    int32_t need_search;  // search contact candidates this step: 0 no, 1 every cell (a node moved
-                          // pair_skin/2, or after too many divisions), 2 only the cells near the
-                          // cells divided in the last step
+    int32_t need_search;  // search contact candidates this step: 0 no, 1 every cell, 2 only the
+                          // cells near the cells listed in RSL (partial search)
     int32_t search_next;  // set during a step when the next step's drift takes a node pair_skin/2 away
+    int32_t n_reset;      // cells listed in RSL for the next partial search (moved past the skin or
+                          // divided); counts past kMaxPartial mean a full search
     float sys_cm[3];      // system centre subtracted at the end of the latest step (correct_com)
     int32_t shift_pending;  // 1 = the step that just ended computed sys_cm and its nodes are still to be
                             // shifted (set by the division phase, consumed by the shift kernel)
@@ -100,8 +101,11 @@ struct Pool {
     int32_t* CAND;
     int32_t* NCAND;
     float *XBX, *XBY, *XBZ;
-    float* SBOX;  // float[cap_cells * 6]: box of a cell divided in the last step, at its positions after
-                  // Integrate (written by k_integrate for the partial search)
+    float* SBOX;     // float[cap_cells * 6]: box of a RESET cell at its positions after Integrate
+                     // (written by k_integrate for the partial search)
+    int8_t* RESET;   // int8[cap_cells]: 1 = the cell takes its current positions as search positions at
+                     // the next partial search (a node left the skin, or the cell divided)
+    int32_t* RSL;    // int32[cap_cells]: per system, the local indices of its RESET cells (n_reset)
     int32_t* EVL;
     int32_t* slot_sys;
     SysMeta* meta;

@@ -248,16 +248,6 @@ __device__ int64_t set_grid_geometry(SysMeta* M, const cs3d_params* P, const flo
     return ok ? nv : (int64_t)M->vbudget;
 }
 
-// 1 = local cell c was divided (parent or daughter) in the last division phase of
-// its system: daughters are the last n_events cells, parents are listed in EVL.
-// Uniform over the workgroup.  n_events <= kMaxPartial (one lane per event).
-__device__ __forceinline__ bool divided_cell(const Pool& p, const SysMeta* M, int c) {
-    const int ne = M->n_events;
-    if (c >= M->n - ne) return true;
-    const int lane = threadIdx.x & 63;
-    return __any(lane < ne && p.EVL[M->base + lane] == c);
-}
-
 // {minx, maxx, miny, maxy, minz, maxz} of the node positions of one cell
 // (workgroup of 192 lanes, nodes on lanes < 180) into out[6] and, through the
 // shared part[3][6], to every lane's return in box[6].  Called by all lanes.
@@ -373,7 +363,7 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             // partial search (after divisions) works on search positions: the nodes of
             // cells that did not divide are binned at their last search position
             float qx = x, qy = y, qz = z;
-            if ((flags & I_IFSEARCH) && M->need_search == 2 && node && !divided_cell(p, M, c)) {
+            if ((flags & I_IFSEARCH) && M->need_search == 2 && node && !p.RESET[slot]) {
                 qx = p.XBX[gi]; qy = p.XBY[gi]; qz = p.XBZ[gi];
             }
             const int b = node ? bin_of(M, qx, qy, qz) : -1;
@@ -422,9 +412,9 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             }
             p.P[slot] = pr;
         }
-        if (kick && (flags & I_IFSEARCH) && M->need_search == 2 && divided_cell(p, M, c)) {
-            // a cell divided at the end of the last step publishes the box of its new
-            // positions: the partial search re-searches the cells within reach of it
+        if (kick && (flags & I_IFSEARCH) && M->need_search == 2 && p.RESET[slot]) {
+            // a cell that divided or left the skin in the last step publishes the box of
+            // its new positions: the partial search re-searches the cells within reach
             box_of_cell(x, y, z, node, ibox, p.SBOX + (int64_t)slot * 6);
         }
     }
@@ -725,26 +715,28 @@ __global__ __launch_bounds__(192) void k_search(Pool p, Range r, int32_t always)
         float X = p.X[gi], Y = p.Y[gi], Z = p.Z[gi];
         bool reset = true;
         if (!always && M->need_search == 2) {
-            // Partial search after divisions (no node has left the skin), in search
-            // positions: a node that did not divide is within pair_skin/2 of its last
-            // search position, and a divided node takes its new position as its search
-            // position.  Two nodes within attraction_range now or later (until the next
-            // full search) are thus within attraction_range + pair_skin in search
-            // positions, which is the search radius.  The nodes were binned at their
-            // search positions (k_integrate).  Lists of cells whose search-position box
-            // is farther than that radius from every divided cell stay valid as they are
-            // (they hold every pair they can meet); the other cells search again around
+            // Partial search, in search positions.  The cells listed in RSL (RESET: a
+            // node of the cell would leave the skin, or the cell divided) take their
+            // current positions as search positions; every other node is within
+            // pair_skin/2 of its search position.  Invariant: every list holds every
+            // node of another cell within attraction_range + pair_skin in search
+            // positions; with the pair_skin/2 bound it holds every node that is, or
+            // comes, within attraction_range before the next search.  The nodes were
+            // binned at their search positions (k_integrate).  A cell whose
+            // search-position box is farther than that radius from every listed cell's
+            // new box keeps its list: the invariant holds for it with the new search
+            // positions.  The listed cells and the cells near them search again around
             // their search positions, so their lists are as long as after a full search.
-            const bool own = divided_cell(p, M, c);
+            const bool own = p.RESET[slot] != 0;
             if (!own) {
                 X = p.XBX[gi]; Y = p.XBY[gi]; Z = p.XBZ[gi];
                 float b[6];
                 box_of_cell(X, Y, Z, node, part, nullptr, b);
-                const int ne = M->n_events, n0 = M->n - ne;
+                const int nr = min(M->n_reset, kMaxPartial);
                 const float mg = P->attraction_range + P->pair_skin;
                 bool hit = false;
-                if (tid < 2 * ne) {
-                    const int q = tid < ne ? p.EVL[M->base + tid] : n0 + (tid - ne);
+                if (tid < nr) {
+                    const int q = p.RSL[M->base + tid];
                     const float* o = p.SBOX + (int64_t)(M->base + q) * 6;
                     hit = b[0] - mg <= o[1] && o[0] <= b[1] + mg && b[2] - mg <= o[3] && o[2] <= b[3] + mg &&
                           b[4] - mg <= o[5] && o[4] <= b[5] + mg;
@@ -813,6 +805,10 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         const cs3d_params* P = p.prm + s;
         sx[tid] = X; sy[tid] = Y; sz[tid] = Z;
         if (fused) { svx[tid] = vX; svy[tid] = vY; svz[tid] = vZ; }
+        if (fused && tid == 0) {  // the search of this step has used the reset list
+            p.RESET[slot] = 0;
+            if (c == 0) M->n_reset = 0;
+        }
         __syncthreads();
         uint32_t st = 0;
         const bool angles = forces && P->constrain_angles;
@@ -1088,6 +1084,7 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
     __shared__ float red[3][128];
     __shared__ float cm[3];
     __shared__ float bpart[3][6];
+    __shared__ int32_t moved;
     const int tid = threadIdx.x;
     const bool node = tid < kNodes;
     const MeshDev* mesh = p.mesh;
@@ -1113,6 +1110,7 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
         const int c = slot - M->base;
         if (c >= M->n) continue;
         const cs3d_params* P = p.prm + s;
+        if (tid == 0) moved = 0;
         sx[tid] = X; sy[tid] = Y; sz[tid] = Z;
         if (do_dis) { svx[tid] = vX; svy[tid] = vY; svz[tid] = vZ; }
         __syncthreads();
@@ -1159,10 +1157,20 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
                 far = !(qx * qx + qy * qy + qz * qz <= h * h);
             }
         }
-        // one atomic per wave with a far node, skipped once the flag is visibly set
-        if (fused && __any(far) && (tid & 63) == __ffsll((unsigned long long)__ballot(far)) - 1 &&
-            __hip_atomic_load(&M->search_next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
-            atomicOr(&M->search_next, 1);
+        // a node of this cell leaves the skin next step: the cell is listed for the next
+        // (partial) search, up to the list size (one atomic per such cell); the flag
+        // atomic is skipped once it is visibly set
+        if (far) moved = 1;
+        __syncthreads();
+        if (fused && tid == 0 && moved) {
+            p.RESET[slot] = 1;
+            if (__hip_atomic_load(&M->search_next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+                atomicOr(&M->search_next, 1);
+            if (__hip_atomic_load(&M->n_reset, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= kMaxPartial) {
+                const int k = atomicAdd(&M->n_reset, 1);
+                if (k < kMaxPartial) p.RSL[M->base + k] = c;
+            }
+        }
         if (clear_bins && (mode != M_STEP || M->need_search)) {
             // the search of this step is done: this cell's share of the system's
             // search-bin counters (those of the current bin grid, which every
@@ -1584,12 +1592,27 @@ __global__ __launch_bounds__(256) void k_sys_divide(Pool p, Range r, int32_t mod
         if (step_mode) {
             M->rebuild = rebuild ? 1 : 0;
             M->step = stepno;
-            // the next drift leaves the skin: search every cell next step; cells divided:
-            // search the cells near them, in search positions (a full search after many
-            // divisions, or when CorrectCoMMotion shifts the nodes, whose search
-            // positions are not shifted)
-            const bool full = M->search_next || events > kMaxPartial || (events > 0 && shift);
-            M->need_search = full ? 1 : (events > 0 ? 2 : 0);
+            // After divisions, the divided cells and the cells listed by k_dis_post (a
+            // node leaves the skin next step) are searched again next step, with the
+            // cells near them (partial search).  Without divisions a node leaving the
+            // skin means a full search: it resets every search position, so the next one
+            // comes many steps later, where partial searches of the cells that cross the
+            // skin one after another would search on almost every step.  A full search
+            // also when the list is too long, or when CorrectCoMMotion shifts the nodes
+            // (their search positions are not shifted).
+            int nr = M->n_reset;
+            bool full = (events == 0 && M->search_next) || nr > kMaxPartial || nr + 2 * events > kMaxPartial ||
+                        (shift && (events > 0 || M->search_next));
+            if (!full)
+                for (int e = 0; e < events; ++e) {
+                    const int pc = p.EVL[base + e], dc = n0 + e;
+                    p.RSL[base + nr++] = pc;
+                    p.RSL[base + nr++] = dc;
+                    p.RESET[base + pc] = 1;
+                    p.RESET[base + dc] = 1;
+                }
+            M->need_search = full ? 1 : (nr > 0 ? 2 : 0);
+            M->n_reset = full ? 0 : nr;
             M->search_next = 0;
         } else if (events > 0) {
             M->need_search = 1;
@@ -1891,7 +1914,10 @@ __global__ void k_set_flags(Pool p, Range r, int32_t rebuild, int32_t search) {
     const int s = r.sys_lo + blockIdx.x * blockDim.x + threadIdx.x;
     if (s < r.sys_hi && p.meta[s].active) {
         if (rebuild >= 0) p.meta[s].rebuild = rebuild;
-        if (search >= 0) p.meta[s].need_search = search;
+        if (search >= 0) {
+            p.meta[s].need_search = search;
+            p.meta[s].n_reset = 0;
+        }
     }
 }
 
