@@ -560,10 +560,10 @@ def test_parameter_branches_match_oracle(oracle_mesh, opt):
 
 
 def test_staggered_divisions_in_a_tissue_match_oracle(oracle_mesh):
-    """A 64-cell tissue whose cells grow at different rates (growth scale
-    U[0.5, 1.5]) divide a few at a time over about 20 steps.  After each
-    such step only the cells near a divided cell search their contact candidates
-    again; cells farther away keep their lists.  Cell counts must agree with the
+    """In a 64-cell tissue (growth scales U[0.5, 1.5]) the cells divide a few at
+    a time over about 20 steps, as their volumes pass the division volume one after
+    another.  After each such step only the cells near a divided cell search their
+    contact candidates again; cells farther away keep their lists.  Cell counts must agree with the
     oracle after every 5 steps, and the state bit for bit (1e-4 bar) at the end."""
     A = _abi()
     from cellsim3d_amd import configs
