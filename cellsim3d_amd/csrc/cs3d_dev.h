@@ -101,8 +101,8 @@ struct Pool {
     int32_t* CAND;
     int32_t* NCAND;
     float *XBX, *XBY, *XBZ;
-    float* SBOX;     // float[cap_cells * 6]: box of a RESET cell at its positions after Integrate
-                     // (written by k_integrate for the partial search)
+    float* SBOX;     // float[cap_cells * 6]: box of each cell's search positions (written by full
+                     // searches, and by k_integrate for RESET cells before a partial search)
     int8_t* RESET;   // int8[cap_cells]: 1 = the cell takes its current positions as search positions at
                      // the next partial search (a node left the skin, or the cell divided)
     int32_t* RSL;    // int32[cap_cells]: per system, the local indices of its RESET cells (n_reset)

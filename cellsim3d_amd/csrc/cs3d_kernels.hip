@@ -722,16 +722,17 @@ __global__ __launch_bounds__(192) void k_search(Pool p, Range r, int32_t always)
             // node of another cell within attraction_range + pair_skin in search
             // positions; with the pair_skin/2 bound it holds every node that is, or
             // comes, within attraction_range before the next search.  The nodes were
-            // binned at their search positions (k_integrate).  A cell whose
-            // search-position box is farther than that radius from every listed cell's
-            // new box keeps its list: the invariant holds for it with the new search
-            // positions.  The listed cells and the cells near them search again around
-            // their search positions, so their lists are as long as after a full search.
+            // binned at their search positions (k_integrate), and SBOX holds the box of
+            // every cell's search positions (the listed cells' new boxes from
+            // k_integrate).  A cell whose box is farther than that radius from every
+            // listed cell's box keeps its list: the invariant holds for it with the new
+            // search positions.  The listed cells and the cells near them search again
+            // around their search positions, so their lists are as long as after a full
+            // search.
             const bool own = p.RESET[slot] != 0;
             if (!own) {
-                X = p.XBX[gi]; Y = p.XBY[gi]; Z = p.XBZ[gi];
-                float b[6];
-                box_of_cell(X, Y, Z, node, part, nullptr, b);
+                const float* ob = p.SBOX + (int64_t)slot * 6;
+                const float b[6] = {ob[0], ob[1], ob[2], ob[3], ob[4], ob[5]};
                 const int nr = min(M->n_reset, kMaxPartial);
                 const float mg = P->attraction_range + P->pair_skin;
                 bool hit = false;
@@ -748,8 +749,13 @@ __global__ __launch_bounds__(192) void k_search(Pool p, Range r, int32_t always)
                 const bool go = near != 0;
                 __syncthreads();  // near is reset by the next cell
                 if (!go) continue;
+                X = p.XBX[gi]; Y = p.XBY[gi]; Z = p.XBZ[gi];
             }
             reset = own;
+        } else {
+            // full search: the current positions become the search positions; their box
+            // is kept for later partial searches
+            box_of_cell(X, Y, Z, node, part, p.SBOX + (int64_t)slot * 6);
         }
         if (node) {
             uint32_t st = 0;
