@@ -223,10 +223,14 @@ def main():
         raise SystemExit("system status 0x%x after the timed run" % status)
     value = n * 180.0 * args.steps / elapsed
     log("bench: %d steps in %.4f s -> %.4g node-updates/s" % (args.steps, elapsed, value))
-    # contact partners per node at the end of the timed run (pairs within attraction_range
-    # that pass the candidate rule): the per-node work of the friction passes
+    # contact partners (pairs within attraction_range that pass the candidate rule: the
+    # per-node work of the friction passes) and candidates (nodes within attraction_range +
+    # pair_skin at the last search: the per-node work of the contact pass) at the end of the
+    # timed run
     pc = s.get(abi.F_PAIRCOUNT)[:n * 192].reshape(n, 192)[:, :180]
-    contacts = {"partners_per_node_mean": round(float(pc.mean()), 3), "partners_per_node_max": int(pc.max())}
+    cc = s.get(abi.F_CANDCOUNT)[:n * 192].reshape(n, 192)[:, :180]
+    contacts = {"partners_per_node_mean": round(float(pc.mean()), 3), "partners_per_node_max": int(pc.max()),
+                "candidates_per_node_mean": round(float(cc.mean()), 3), "candidates_per_node_max": int(cc.max())}
 
     # per-phase kernel time from start/stop HIP events each launch records on the
     # engine's stream (eager steps; launch gaps are not counted)

@@ -31,7 +31,8 @@ SYS_FATAL_MASK = 0x37F
 # cs3d_field
 (F_X, F_Y, F_Z, F_VX, F_VY, F_VZ, F_FCX, F_FCY, F_FCZ, F_FDX, F_FDY, F_FDZ, F_VAX, F_VAY, F_VAZ,
  F_CMX, F_CMY, F_CMZ, F_BBOX, F_PRESSURE, F_STIFFNESS, F_VOLUME, F_AREA, F_DIVFLAG, F_GRID,
- F_VOXCOUNT, F_VOXLIST, F_RANMAR, F_PAIRCOUNT, F_PAIRS, F_GROWTH_SCALE, F_FRX, F_FRY, F_FRZ, F_SYSCM, F__COUNT) = range(36)
+ F_VOXCOUNT, F_VOXLIST, F_RANMAR, F_PAIRCOUNT, F_PAIRS, F_GROWTH_SCALE, F_FRX, F_FRY, F_FRZ, F_SYSCM, F_CANDCOUNT,
+ F__COUNT) = range(37)
 
 NODE_FIELDS = {F_X, F_Y, F_Z, F_VX, F_VY, F_VZ, F_FCX, F_FCY, F_FCZ, F_FDX, F_FDY, F_FDZ, F_VAX, F_VAY, F_VAZ,
                F_FRX, F_FRY, F_FRZ}

@@ -263,6 +263,7 @@ FieldLoc field_loc(cs3d_ctx* c, cs3d_sys s, cs3d_field f) {
         case CS3D_F_VOXCOUNT: L.ptr = p.vox + h.vreg; L.bytes = (size_t)h.vbudget * 4; break;
         case CS3D_F_VOXLIST: L.ptr = p.vox + h.vreg + h.vbudget; L.bytes = (size_t)h.vbudget * h.vcap * 4; break;
         case CS3D_F_PAIRCOUNT: L.ptr = p.NPAIR + nb; L.bytes = nbytes; break;
+        case CS3D_F_CANDCOUNT: L.ptr = p.NCAND + nb; L.bytes = nbytes; break;
         case CS3D_F_PAIRS: L.ptr = p.PAIRS + (int64_t)h.base * kPMax * kSlots; L.bytes = (size_t)h.cap * kPMax * kSlots * 4; break;
         case CS3D_F_GROWTH_SCALE: L.ptr = p.GS + h.base; L.bytes = h.cap * 4; break;
         case CS3D_F_GRID: L.bytes = 32; break;
@@ -645,6 +646,7 @@ extern "C" cs3d_status cs3d_set(cs3d_ctx* c, cs3d_sys s, cs3d_field f, const voi
     if (L.bytes == 0) return fail(CS3D_INVALID_ARG, "unknown field");
     if (bytes > L.bytes) return fail(CS3D_INVALID_ARG, "more bytes than the field holds");
     if (f == CS3D_F_SYSCM) return fail(CS3D_INVALID_ARG, "the system centre is read-only");
+    if (f == CS3D_F_CANDCOUNT) return fail(CS3D_INVALID_ARG, "the candidate counts are read-only");
     if (f == CS3D_F_GRID || f == CS3D_F_RANMAR) {
         SysMeta m;
         cs3d_status st = read_meta(c, s, m);

@@ -277,7 +277,7 @@ class System:
     def _dtype(self, f):
         if f == abi.F_DIVFLAG:
             return np.int8
-        if f in (abi.F_GRID, abi.F_VOXCOUNT, abi.F_VOXLIST, abi.F_PAIRCOUNT, abi.F_PAIRS):
+        if f in (abi.F_GRID, abi.F_VOXCOUNT, abi.F_VOXLIST, abi.F_PAIRCOUNT, abi.F_PAIRS, abi.F_CANDCOUNT):
             return np.int32
         return np.float32
 

@@ -155,6 +155,8 @@ typedef enum cs3d_field {
     CS3D_F_FRX, CS3D_F_FRY, CS3D_F_FRZ,        /* random force of the latest step, d_fRanList (add_rands) */
     CS3D_F_SYSCM,                              /* 3 floats, read-only: the system centre subtracted at the end
                                                   of the latest step (sysCMx/y/z; correct_com), else 0 */
+    CS3D_F_CANDCOUNT,                          /* int32 per node, read-only: contact candidates kept at
+                                                  the last search (engine-specific) */
     CS3D_F__COUNT
 } cs3d_field;
 

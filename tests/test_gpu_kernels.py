@@ -161,7 +161,8 @@ def test_refused_calls_leave_the_system_usable(oracle_mesh):
     """Invalid calls through the C ABI return a status and a message and change
     nothing: unknown system, more bytes than a field holds, the read-only system
     centre, inconsistent cell counts, an exhausted cell pool, a NULL output and a
-    cell index past the live cells.  The system then steps as before."""
+    cell index past the live cells.  The system then steps as before, and its
+    read-only candidate counts bound its partner counts."""
     import ctypes as C
     from cellsim3d_amd import abi, engine
     L = engine.lib()
@@ -188,4 +189,9 @@ def test_refused_calls_leave_the_system_usable(oracle_mesh):
     ctx.sync()
     n1, st1, status = s.info()
     assert (n1, st1) == (n, 3) and status & abi.SYS_FATAL_MASK == 0
+    # the candidate counts are read-only and bound the partner counts node by node
+    cc = s.get(abi.F_CANDCOUNT)[:n * S].reshape(n, S)[:, :180]
+    pc = s.get(abi.F_PAIRCOUNT)[:n * S].reshape(n, S)[:, :180]
+    assert (pc <= cc).all() and cc.max() <= 40 and cc.sum() > 0
+    assert L.cs3d_set(ctx.h, s.id, abi.F_CANDCOUNT, cc.ctypes.data_as(vp), 4) == abi.INVALID_ARG
     ctx.close()
