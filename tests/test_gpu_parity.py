@@ -595,3 +595,44 @@ def test_staggered_divisions_in_a_tissue_match_oracle(oracle_mesh):
     _cmp(sim, s, [A.F_X, A.F_Y, A.F_Z, A.F_VX, A.F_VY, A.F_VZ, A.F_FCX, A.F_FDX, A.F_PRESSURE], nn, tol=1e-4,
          label="staggered divisions")
     ctx.close()
+
+
+def test_growing_colony_matches_oracle(oracle_mesh):
+    """The growing-colony workload at reduced size: one cell with per-cell growth
+    scales U[0.8, 1.2], division at twice the cell volume, voxel lists of 128, the
+    grid rebuilt every 10 steps and after divisions.  It grows to about 17 cells in
+    4000 steps, dividing in parent order first and then a few cells at a time.  Cell
+    counts must agree with the oracle after every 250 steps; positions, velocities,
+    forces, pressures, volumes and centres within 1e-5 at the end, the voxel grid
+    bit for bit."""
+    A = _abi()
+    E, O = _engine(), _oracle()
+    from cellsim3d_amd import configs
+    mc = 64
+    cfg = configs.colony(seed=1, max_cells=mc)
+    p = cfg["params"]
+    p.grid_interval = 10
+    p.voxel_capacity = 128
+    budget = 1 << 14
+    sim = O.Sim(p, oracle_mesh, cfg["X"], cfg["Y"], cfg["Z"], 1, max_cells=mc, budget=budget)
+    sim.set(A.F_GROWTH_SCALE, cfg["growth_scale"])
+    assert sim.init() == 0
+    ctx = E.Context(oracle_mesh, capacity_cells=mc, voxel_budget=budget * (p.voxel_capacity + 1))
+    s = ctx.add_system(p, cfg["X"], cfg["Y"], cfg["Z"], 1, max_cells=mc, voxels=budget)
+    s.set(A.F_GROWTH_SCALE, cfg["growth_scale"])
+    counts = []
+    for k in range(16):
+        assert sim.step(250) & A.SYS_FATAL_MASK == 0
+        s.step(250)
+        ctx.sync()
+        n1, st1, status = s.info()
+        assert status & A.SYS_FATAL_MASK == 0, hex(status)
+        assert n1 == sim.n and st1 == sim.stepno, "after %d steps: gpu %d cells, oracle %d" % (st1, n1, sim.n)
+        counts.append(n1)
+    print("colony cell counts every 250 steps:", counts, flush=True)
+    assert counts[-1] >= 12
+    n = sim.n
+    _cmp(sim, s, [A.F_X, A.F_Y, A.F_Z, A.F_VX, A.F_VY, A.F_VZ, A.F_FCX, A.F_FDX, A.F_PRESSURE, A.F_VOLUME,
+                  A.F_CMX, A.F_CMY, A.F_CMZ], n, label="colony")
+    _cmp_grid(sim, s)
+    ctx.close()
