@@ -28,14 +28,17 @@ from . import mesh as meshmod
 NODE_BYTES = 4 * 192  # one float per slot
 
 
-def build(n_tissues=256, seed=1, nmin=1, nmax=512, steps=5000, grid_interval=10, phi=0.30):
+def build(n_tissues=256, seed=1, nmin=1, nmax=512, steps=5000, grid_interval=10, phi=0.30, room=2):
     """Seeded ensemble: (device mesh, tissues).  Each tissue dict has params,
-    X, Y, Z, n, max_cells."""
+    X, Y, Z, n, max_cells = room * n + 2.  room = 2 lets every cell divide once,
+    enough for 1000 steps; a tissue of a few free cells doubles about every 1000
+    steps, so longer runs need more room (a tissue that reaches max_cells stops
+    with the capacity status)."""
     m, tissues = configs.ensemble(n_tissues=n_tissues, seed=seed, nmin=nmin, nmax=nmax, phi=phi, steps=steps)
     gm = engine.build_mesh(meshmod.tables(m), configs.rest_positions(m))
     for t in tissues:
         t["params"].grid_interval = grid_interval
-        t["max_cells"] = 2 * t["n"] + 2  # room for every cell to divide once
+        t["max_cells"] = room * t["n"] + 2
     return gm, tissues
 
 
@@ -129,7 +132,9 @@ def autotune(gm, tissues, probe_steps=100, wg_options=(4, 8, 16, 32, 64), graph_
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--tissues", type=int, default=256)
-    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--room", type=int, default=2,
+                    help="cell slots per initial cell of each tissue (2: every cell divides once)")
     ap.add_argument("--nmax", type=int, default=512)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--grid-interval", type=int, default=10)
@@ -141,7 +146,7 @@ def main(argv=None):
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
-    gm, tissues = build(args.tissues, args.seed, 1, args.nmax, args.steps, args.grid_interval)
+    gm, tissues = build(args.tissues, args.seed, 1, args.nmax, args.steps, args.grid_interval, room=args.room)
     cells = sum(t["n"] for t in tissues)
     log("ensemble: %d tissues, %d cells, %d steps" % (len(tissues), cells, args.steps))
     wg, gs, table = (16, 10, [])
