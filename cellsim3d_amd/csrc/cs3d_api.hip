@@ -24,6 +24,7 @@ void launch_bonded(const Pool& p, const Range& r, int angles, hipStream_t st);
 void launch_divide(const Pool& p, const Range& r, int mode, hipStream_t st);
 extern thread_local std::vector<std::pair<hipEvent_t, hipEvent_t>>* kernel_timer;
 void launch_com_shift(const Pool& p, const Range& r, hipStream_t st);
+void launch_live(const Pool& p, const Range& r, hipStream_t st);
 void launch_grid(const Pool& p, const Range& r, int vox_blocks, hipStream_t st);
 void launch_vox_lists(const Pool& p, const Range& r, int vox_blocks, hipStream_t st);
 void launch_vrange(const Pool& p, const Range& r, hipStream_t st);
@@ -139,12 +140,14 @@ Range range_of(cs3d_ctx* c, cs3d_sys s) {
         r.sys_lo = 0;
         r.sys_hi = c->nsys_high;
         r.one_sys = 0;
+        r.live = 0;
     } else {
         r.slot_lo = c->sys[s].base;
         r.slot_hi = c->sys[s].base + c->sys[s].cap;
         r.sys_lo = s;
         r.sys_hi = s + 1;
         r.one_sys = 1;
+        r.live = 0;
     }
     return r;
 }
@@ -189,15 +192,22 @@ bool any_correct_com(const cs3d_ctx* c, const Range& r) {
 //              bounding_boxes + next-step search decision
 //   divide     division phase, step bookkeeping, extent and grid geometry
 //              [+ CorrectCoMMotion, a sixth launch, for systems with correct_com]
+//
+// A step of several systems first lists their live cell slots (k_live); the four
+// cell kernels of the step walk that list.  The division phase adds daughters after
+// them, which the next step's list holds.
 void enqueue_step_phase(cs3d_ctx* c, const Range& r, int phase) {
     const Pool& p = c->pool;
+    Range rl = r;
+    rl.live = r.one_sys ? 0 : 1;
     switch (phase) {
         case CS3D_PH_INTEGRATE:
-            launch_integrate(p, r, I_KICK | I_PRESSURE | I_BINS | I_IFSEARCH | I_GRIDCOUNT, -1, c->stream);
+            if (rl.live) launch_live(p, r, c->stream);
+            launch_integrate(p, rl, I_KICK | I_PRESSURE | I_BINS | I_IFSEARCH | I_GRIDCOUNT, -1, c->stream);
             break;
-        case CS3D_PH_SEARCH: launch_search(p, r, 0, c->stream); break;
-        case CS3D_PH_CON_DIS: launch_con(p, r, M_STEP, true, c->stream); break;
-        case CS3D_PH_DIS_POST: launch_dis(p, r, M_STEP, -1, 1, false, c->stream); break;
+        case CS3D_PH_SEARCH: launch_search(p, rl, 0, c->stream); break;
+        case CS3D_PH_CON_DIS: launch_con(p, rl, M_STEP, true, c->stream); break;
+        case CS3D_PH_DIS_POST: launch_dis(p, rl, M_STEP, -1, 1, false, c->stream); break;
         case CS3D_PH_DIVIDE:
             launch_divide(p, r, M_STEP, c->stream);
             if (any_correct_com(c, r)) launch_com_shift(p, r, c->stream);
@@ -351,6 +361,8 @@ extern "C" cs3d_status cs3d_ctx_create(int device, const cs3d_mesh* mesh, int64_
     if ((st = dalloc(c, p.SBOX, capacity_cells * 6)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.RESET, capacity_cells)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.RSL, capacity_cells)) != CS3D_OK) return cleanup(st);
+    if ((st = dalloc(c, p.LIVE, capacity_cells)) != CS3D_OK) return cleanup(st);
+    if ((st = dalloc(c, p.n_live, 1)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.bins, bin_budget)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.EVL, capacity_cells)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.slot_sys, capacity_cells)) != CS3D_OK) return cleanup(st);

@@ -106,6 +106,8 @@ struct Pool {
     int8_t* RESET;   // int8[cap_cells]: 1 = the cell takes its current positions as search positions at
                      // the next partial search (a node left the skin, or the cell divided)
     int32_t* RSL;    // int32[cap_cells]: per system, the local indices of its RESET cells (n_reset)
+    int32_t* LIVE;   // int32[cap_cells]: live cell slots of the stepped systems (k_live)
+    int32_t* n_live; // int32[1]: entries of LIVE
     int32_t* EVL;
     int32_t* slot_sys;
     SysMeta* meta;
@@ -121,6 +123,7 @@ struct Range {
     int32_t slot_lo, slot_hi;
     int32_t sys_lo, sys_hi;
     int32_t one_sys;  // 1: the range is exactly the slot range of system sys_lo
+    int32_t live;     // 1: the cell kernels walk Pool::LIVE (built by k_live) instead of the slots
 };
 
 // kernel modes

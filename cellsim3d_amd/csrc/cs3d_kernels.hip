@@ -293,13 +293,18 @@ __device__ __forceinline__ void box_of_cell(float x, float y, float z, bool node
 // search bins.
 // ------------------------------------------------------------------------------
 // 6 waves per SIMD (at most 80 VGPRs): 8 workgroups per CU, one round for 2048 cells
+// kLive: walk Pool::LIVE (a multi-system step) instead of the range's slots
+template <int kLive>
 __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_integrate(Pool p, Range r,
                                                                                               int32_t flags,
                                                                                               int64_t step_override) {
     __shared__ float ibox[3][6];
     const int tid = threadIdx.x;
     const bool node = tid < kNodes;
-    for (int slot = r.slot_lo + xcd_block(); slot < r.slot_hi; slot += gridDim.x) {
+    // a multi-system step walks the compact list of live slots (k_live)
+    const int nit = kLive ? *p.n_live : r.slot_hi - r.slot_lo;
+    for (int it = xcd_block(); it < nit; it += gridDim.x) {
+        const int slot = kLive ? p.LIVE[it] : r.slot_lo + it;
         // the cell's node data are loaded before the system checks (every slot of the
         // range is allocated), so their latency overlaps the metadata reads
         const int64_t gi = (int64_t)slot * kSlots + tid;
@@ -693,6 +698,8 @@ __device__ __forceinline__ int accept_forces(const int32_t* cand, int nc, int32_
 // moved pair_skin/2 since the last search (or always, outside the fused step),
 // rebuild each node's sorted candidate list and remember the positions.
 // ------------------------------------------------------------------------------
+// kLive: walk Pool::LIVE (a multi-system step) instead of the range's slots
+template <int kLive>
 __global__ __launch_bounds__(192) void k_search(Pool p, Range r, int32_t always) {
     __shared__ int32_t lst[kPMax][kSlots];
     __shared__ float part[3][6];
@@ -703,7 +710,10 @@ __global__ __launch_bounds__(192) void k_search(Pool p, Range r, int32_t always)
         const SysMeta* M0 = p.meta + r.sys_lo;
         if (!M0->need_search || !sys_live(M0)) return;
     }
-    for (int slot = r.slot_lo + xcd_block(); slot < r.slot_hi; slot += gridDim.x) {
+    // a multi-system step walks the compact list of live slots (k_live)
+    const int nit = kLive ? *p.n_live : r.slot_hi - r.slot_lo;
+    for (int it = xcd_block(); it < nit; it += gridDim.x) {
+        const int slot = kLive ? p.LIVE[it] : r.slot_lo + it;
         const int s = sys_of(p, r, slot);
         if (s < 0) continue;
         SysMeta* M = p.meta + s;
@@ -779,6 +789,8 @@ __global__ __launch_bounds__(192) void k_search(Pool p, Range r, int32_t always)
 // ------------------------------------------------------------------------------
 // 4 waves per SIMD: with the pipelined candidate loop the allocator would otherwise
 // take 132 VGPRs (3 waves); at this bound it fits 127 without spills
+// kLive: walk Pool::LIVE (a multi-system step) instead of the range's slots
+template <int kLive>
 __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_con_dis(Pool p, Range r, int32_t mode, const float* __restrict__ rX,
                                                  const float* __restrict__ rY, const float* __restrict__ rZ,
                                                  const float* __restrict__ rVX, const float* __restrict__ rVY,
@@ -794,7 +806,10 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     const bool fused = mode == M_STEP;
     const bool forces = mode != M_PAIRS;
     const bool search = mode != M_BONDED;
-    for (int slot = r.slot_lo + xcd_block(); slot < r.slot_hi; slot += gridDim.x) {
+    // a multi-system step walks the compact list of live slots (k_live)
+    const int nit = kLive ? *p.n_live : r.slot_hi - r.slot_lo;
+    for (int it = xcd_block(); it < nit; it += gridDim.x) {
+        const int slot = kLive ? p.LIVE[it] : r.slot_lo + it;
         // node data first (every slot of the range is allocated): their latency
         // overlaps the metadata reads
         const int64_t gi = (int64_t)slot * kSlots + tid;
@@ -1080,6 +1095,8 @@ __global__ __launch_bounds__(192) void k_bonded(Pool p, Range r) {
 // steps, volumes (volume.cu:5-163).  Modes: M_STEP, M_DIS_V, M_DIS_VA, M_COM,
 // M_VOLUMES.
 // ------------------------------------------------------------------------------
+// kLive: walk Pool::LIVE (a multi-system step) instead of the range's slots
+template <int kLive>
 __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode, int64_t step_override,
                                                   int32_t clear_bins, const float* __restrict__ rX,
                                                   const float* __restrict__ rY, const float* __restrict__ rZ,
@@ -1096,7 +1113,10 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
     const MeshDev* mesh = p.mesh;
     const bool do_dis = mode == M_STEP || mode == M_DIS_V || mode == M_DIS_VA;
     const bool fused = mode == M_STEP;
-    for (int slot = r.slot_lo + xcd_block(); slot < r.slot_hi; slot += gridDim.x) {
+    // a multi-system step walks the compact list of live slots (k_live)
+    const int nit = kLive ? *p.n_live : r.slot_hi - r.slot_lo;
+    for (int it = xcd_block(); it < nit; it += gridDim.x) {
+        const int slot = kLive ? p.LIVE[it] : r.slot_lo + it;
         // node data first (every slot of the range is allocated): their latency
         // overlaps the metadata reads
         const int64_t gi = (int64_t)slot * kSlots + tid;
@@ -1626,6 +1646,33 @@ __global__ __launch_bounds__(256) void k_sys_divide(Pool p, Range r, int32_t mod
     }
 }
 
+// Compact list of the live cell slots of a multi-system range (cells c < n of the
+// systems that step), rebuilt at the start of every step: the step's cell kernels
+// walk it instead of every slot of each system's room.  One workgroup per system:
+// system s writes its slots from the offset given by the live cells of the systems
+// before it; the last workgroup writes the count.
+__device__ __forceinline__ int live_cells(const SysMeta* m) { return sys_live(m) ? m->n : 0; }
+
+__global__ __launch_bounds__(256) void k_live(Pool p, Range r) {
+    __shared__ int32_t red[256];
+    const int tid = threadIdx.x;
+    const int s = r.sys_lo + blockIdx.x;
+    const int hi = s < r.sys_hi ? s : r.sys_hi;
+    int acc = 0;
+    for (int j = r.sys_lo + tid; j < hi; j += 256) acc += live_cells(p.meta + j);
+    red[tid] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) red[tid] += red[tid + o];
+        __syncthreads();
+    }
+    const int off = red[0];
+    const int n = s < r.sys_hi ? live_cells(p.meta + s) : 0;
+    const int base = s < r.sys_hi ? p.meta[s].base : 0;
+    for (int c = tid; c < n; c += 256) p.LIVE[off + c] = base + c;
+    if (blockIdx.x == gridDim.x - 1 && tid == 0) *p.n_live = off + n;
+}
+
 // CorrectCoMMotion (propagate.cu:766-776) for the systems whose division phase
 // just computed the system centre: every node of every cell minus sys_cm.  The
 // reference's kernel also shifts the 12 padding slots of each cell; they are no
@@ -1957,18 +2004,33 @@ static int grid_cells(const Range& r, int per_cu) {
     return n < 1 ? 1 : (n < cap ? n : cap);
 }
 
+void launch_live(const Pool& p, const Range& r, hipStream_t st) {
+    const int nsys = r.sys_hi - r.sys_lo;
+    launch(k_live, dim3(nsys > 0 ? nsys : 1), dim3(256), st, p, r);
+}
+
 void launch_integrate(const Pool& p, const Range& r, int flags, int64_t step_override, hipStream_t st) {
-    launch(k_integrate, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), st, p, r, flags, step_override);
+    if (r.live)
+        launch(k_integrate<1>, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), st, p, r, flags, step_override);
+    else
+        launch(k_integrate<0>, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), st, p, r, flags, step_override);
 }
 void launch_search(const Pool& p, const Range& r, int always, hipStream_t st) {
-    launch(k_search, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), st, p, r, always);
+    if (r.live)
+        launch(k_search<1>, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), st, p, r, always);
+    else
+        launch(k_search<0>, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), st, p, r, always);
 }
 void launch_con(const Pool& p, const Range& r, int mode, bool use_va, hipStream_t st) {
     const float* vx = use_va ? p.VAX : p.VX;
     const float* vy = use_va ? p.VAY : p.VY;
     const float* vz = use_va ? p.VAZ : p.VZ;
-    launch(k_con_dis, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), st, p, r, mode, p.X, p.Y, p.Z, vx, vy, vz,
-                       p.BBOX);
+    if (r.live)
+        launch(k_con_dis<1>, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), st, p, r, mode, p.X, p.Y, p.Z, vx, vy, vz,
+                               p.BBOX);
+    else
+        launch(k_con_dis<0>, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), st, p, r, mode, p.X, p.Y, p.Z, vx, vy, vz,
+                               p.BBOX);
 }
 // angles: 0 = systems without bending, 1 = with, -1 = both launches
 void launch_bonded(const Pool& p, const Range& r, int angles, hipStream_t st) {
@@ -1980,8 +2042,12 @@ void launch_dis(const Pool& p, const Range& r, int mode, int64_t step_override, 
     const float* vx = use_va ? p.VAX : p.VX;
     const float* vy = use_va ? p.VAY : p.VY;
     const float* vz = use_va ? p.VAZ : p.VZ;
-    launch(k_dis_post, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), st, p, r, mode, step_override, clear_bins,
-                       p.X, p.Y, p.Z, vx, vy, vz);
+    if (r.live)
+        launch(k_dis_post<1>, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), st, p, r, mode, step_override, clear_bins,
+                               p.X, p.Y, p.Z, vx, vy, vz);
+    else
+        launch(k_dis_post<0>, dim3(grid_cells(r, p.wg_per_cu)), dim3(192), st, p, r, mode, step_override, clear_bins,
+                               p.X, p.Y, p.Z, vx, vy, vz);
 }
 void launch_divide(const Pool& p, const Range& r, int mode, hipStream_t st) {
     const int ns = r.sys_hi - r.sys_lo;
