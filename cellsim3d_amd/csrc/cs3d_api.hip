@@ -700,11 +700,13 @@ extern "C" cs3d_status cs3d_set(cs3d_ctx* c, cs3d_sys s, cs3d_field f, const voi
 // cs3d_params, i32 n, i32 saved_cap, i64 step, i64 horizon, u32 status,
 // cs3d_ranmar_state, grid geometry {f32 gmin[3], i32 dims[3], f32 bmin[3], f32 bh,
 // i32 bdims[3]}, f32 system centre[3], u32 mesh checksum, then for cells 0..n-1: 15 node arrays
-// (x, y, z, v, F_con, F_dis, F_ran; 192 floats per cell each), 6 cell arrays (CM, P,
-// stiffness, growth scale) and the contact-rule bbox and its voxel range (6 each).
+// (x, y, z, v, F_con, F_dis, F_ran; 192 floats per cell each), 8 cell arrays (CM, P,
+// stiffness, growth scale, volume, area) and the contact-rule bbox and its voxel range
+// (6 each).  Volume and area are kept so that a restarted system reports the last
+// computed values, as an uninterrupted run does on steps without a volume pass.
 namespace {
 const char kCkptMagic[8] = {'C', 'S', '3', 'D', 'C', 'K', 'P', '1'};
-constexpr uint32_t kCkptVersion = 3;
+constexpr uint32_t kCkptVersion = 4;
 
 uint32_t mesh_checksum(const cs3d_ctx* c) {
     // FNV-1a over the mesh tables the context was built with
@@ -763,7 +765,7 @@ extern "C" cs3d_status cs3d_system_save(cs3d_ctx* c, cs3d_sys s, const char* pat
         }
         ok = std::fwrite(buf.data(), 4, (size_t)n * kSlots, f) == (size_t)n * kSlots;
     }
-    const float* cell[] = {p.CMX, p.CMY, p.CMZ, p.P, p.E, p.GS};
+    const float* cell[] = {p.CMX, p.CMY, p.CMZ, p.P, p.E, p.GS, p.VOL, p.AREA};
     for (const float* a : cell) {
         if (!ok) break;
         if (hipMemcpy(buf.data(), a + base, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess) {
@@ -826,7 +828,7 @@ extern "C" cs3d_status cs3d_system_load(cs3d_ctx* c, const char* path, int32_t m
         ok = std::fread(buf.data(), 4, (size_t)n * kSlots, f) == (size_t)n * kSlots &&
              hipMemcpy(a + base * kSlots, buf.data(), (size_t)n * kSlots * 4, hipMemcpyHostToDevice) == hipSuccess;
     }
-    float* cell[] = {p.CMX, p.CMY, p.CMZ, p.P, p.E, p.GS};
+    float* cell[] = {p.CMX, p.CMY, p.CMZ, p.P, p.E, p.GS, p.VOL, p.AREA};
     for (float* a : cell) {
         if (!ok) break;
         ok = std::fread(buf.data(), 4, (size_t)n, f) == (size_t)n &&

@@ -1626,8 +1626,11 @@ __global__ __launch_bounds__(256) void k_sys_divide(Pool p, Range r, int32_t mod
             // skin one after another would search on almost every step.  A full search
             // also when the list is too long, or when CorrectCoMMotion shifts the nodes
             // (their search positions are not shifted).
+            // RSL holds one entry per cell slot of the system (M->cap entries at
+            // RSL[base]): a list that would not fit there means a full search
             int nr = M->n_reset;
-            bool full = (events == 0 && M->search_next) || nr > kMaxPartial || nr + 2 * events > kMaxPartial ||
+            const int lim = min(kMaxPartial, M->cap);
+            bool full = (events == 0 && M->search_next) || nr > lim || nr + 2 * events > lim ||
                         (shift && (events > 0 || M->search_next));
             if (!full)
                 for (int e = 0; e < events; ++e) {

@@ -16,7 +16,8 @@ vertices, 180 triangles); the cell nodes are the triangle centroids projected
 onto the unit sphere (the dual polyhedron), which gives 180 nodes of degree 3,
 270 bonds, 12 pentagons and 80 hexagons.  Nodes are numbered by recursive
 median bisection so that every run of 16 consecutive indices is a compact
-patch of the surface (the engine prunes contact work per 16-node patch).
+patch of the surface: neighbouring lanes of a wavefront then hold nearby
+nodes, whose contact candidates lie in the same few search bins.
 """
 import os
 
@@ -24,7 +25,7 @@ import numpy as np
 
 from . import abi
 
-PATCH = 16  # nodes per contact patch in the engine (csrc kernels use the same value)
+PATCH = 16  # nodes per compact patch of the numbering (a quarter of a wavefront)
 
 
 def _icosahedron():

@@ -55,7 +55,7 @@ typedef enum cs3d_status {
                                                 writes past localNNs[10] there (propagate.cu:222,398-403) */
 #define CS3D_SYS_CAND_OVERFLOW    (1u << 8)  /* a node has more contact candidates (nodes of other cells within
                                                 attraction_range + pair_skin) than its list holds (40) */
-#define CS3D_SYS_BIN_OVERFLOW     (1u << 9)  /* more nodes in one contact-search bin than it holds (16) */
+#define CS3D_SYS_BIN_OVERFLOW     (1u << 9)  /* more nodes in one contact-search bin than it holds (CS3D_BIN_CAP) */
 #define CS3D_SYS_FATAL_MASK       (0x37Fu)   /* every bit except the NN_OVER_10 diagnostic */
 
 /* Mesh topology and rest state shared by all cells.

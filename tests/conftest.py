@@ -39,3 +39,16 @@ def relerr(a, b):
     b = np.asarray(b, dtype=np.float64)
     den = max(np.max(np.abs(b)), 1e-30)
     return float(np.max(np.abs(a - b)) / den)
+
+
+def elemerr(a, b, floor_frac=1e-6):
+    """max over elements of |a-b| / max(|b|, floor_frac * max|b|): the relative
+    error of each element, with values far below the field's scale measured
+    against that floor instead of themselves."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if a.size == 0:
+        return 0.0
+    scale = max(float(np.max(np.abs(b))), 1e-30)
+    den = np.maximum(np.abs(b), floor_frac * scale)
+    return float(np.max(np.abs(a - b) / den))

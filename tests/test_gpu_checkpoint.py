@@ -1,20 +1,38 @@
-"""Checkpoint / restart: a system saved mid-run and loaded into a fresh context
-continues bit-identically to the one that kept running (growth and division
-included), and a mismatched mesh is refused."""
+"""Checkpoint / restart.  A system saved mid-run and loaded into a fresh
+context continues bit-identically to the one that kept running (growth and
+division included, and with growth off, where volumes are no longer
+recomputed and must come from the file), and it ends where the CPU oracle
+run straight through ends.  A mismatched mesh is refused."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
+FIELDS = ("F_X", "F_Y", "F_Z", "F_VX", "F_VY", "F_VZ", "F_FCX", "F_FDZ", "F_PRESSURE", "F_CMX", "F_VOLUME", "F_AREA")
 
-def test_restart_continues_bit_identically(oracle_mesh, tmp_path):
-    from cellsim3d_amd import abi, configs, engine
-    cfg = configs.plain_small(seed=3)
+
+def _case(growth, seed=3):
+    from cellsim3d_amd import configs
+    cfg = configs.plain_small(seed=seed)
     p = cfg["params"]
-    p.division_vol = np.float32(1.05 * cfg["v0"])  # divisions within the run
     p.grid_interval = 7
+    if growth:
+        p.division_vol = np.float32(1.05 * cfg["v0"])  # divisions within the run
+    else:
+        # volumes are computed while growing (GPUbounce.cu:1168); growth ends at step 100
+        p.div_time_steps = 100
+        p.non_div_time_steps = 1000
+    return cfg, p
+
+
+@pytest.mark.parametrize("growth", [True, False])
+def test_restart_continues_bit_identically(oracle_mesh, tmp_path, growth):
+    from cellsim3d_amd import abi, engine
+    from oracle import oracle as O
+    cfg, p = _case(growth)
     mc = 64
-    print("checkpoint test: 8 cells, 150 + 150 steps", flush=True)
+    fields = [getattr(abi, f) for f in FIELDS]
+    print("checkpoint test: 8 cells, 150 + 150 steps, growth %s" % growth, flush=True)
     ctx = engine.Context(oracle_mesh, capacity_cells=mc + 8, voxel_budget=1 << 22)
     s = ctx.add_system(p, cfg["X"], cfg["Y"], cfg["Z"], cfg["n"], max_cells=mc, voxels=1 << 14)
     s.step(150)
@@ -23,11 +41,11 @@ def test_restart_continues_bit_identically(oracle_mesh, tmp_path):
     s.save(path)
     s.step(150)
     ctx.sync()
-    ref = {f: s.get(f) for f in (abi.F_X, abi.F_Y, abi.F_Z, abi.F_VX, abi.F_FCX, abi.F_FDZ, abi.F_PRESSURE,
-                                   abi.F_CMX)}
+    ref = {f: s.get(f) for f in fields}
     n_ref, step_ref, st_ref = s.info()
     ctx.close()
-    assert n_ref > cfg["n"], "the run was meant to include divisions"
+    if growth:
+        assert n_ref > cfg["n"], "the run was meant to include divisions"
     ctx2 = engine.Context(oracle_mesh, capacity_cells=mc + 8, voxel_budget=1 << 22)
     t = ctx2.load_system(path, voxels=1 << 14)
     assert t.info()[1] == 150
@@ -37,6 +55,15 @@ def test_restart_continues_bit_identically(oracle_mesh, tmp_path):
     assert (n2, step2, st2) == (n_ref, step_ref, st_ref)
     for f, a in ref.items():
         assert np.array_equal(t.get(f), a), "field %d differs after restart" % f
+    # the restarted run ends where the oracle, run straight through, ends
+    sim = O.Sim(p, oracle_mesh, cfg["X"], cfg["Y"], cfg["Z"], cfg["n"], max_cells=mc, budget=1 << 14)
+    sim.init()
+    sim.step(300)
+    assert sim.n == n2 and sim.stepno == 300
+    for f in fields:
+        a, b = t.get(f), sim.get(f)
+        k = n2 * 192 if f in abi.NODE_FIELDS else n2
+        assert np.array_equal(a[:k], b[:k]), "field %d differs from the oracle after restart" % f
     ctx2.close()
 
 

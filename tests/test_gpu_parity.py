@@ -2,17 +2,19 @@
 same seeded, injected state.
 
 Bar: neighbour grids, candidate lists, division events and cell counts are
-bit-exact; forces, velocities, positions, volumes and centres of mass agree
-within 1e-5 relative (scale-relative: max|gpu - oracle| / max|oracle|).  The
-kernels follow the reference's operation order without contraction, so most
-fields are in fact bit-identical; the tests print the bit-exact fraction.
+bit-exact.  The kernels follow the oracle's operation order (no contraction,
+correctly rounded float division and square root, the reference's double
+promotions), so positions, velocities, forces, pressures, volumes and
+centres of mass are compared bit for bit as well; where a test states a
+tolerance instead, it is per element: |gpu - oracle| <= tol * max(|oracle|,
+1e-6 * max|oracle|) with tol = 1e-5, the north-star force bar.
 """
 import ctypes as C
 
 import numpy as np
 import pytest
 
-from conftest import relerr
+from conftest import elemerr, relerr
 
 pytestmark = pytest.mark.gpu
 
@@ -63,7 +65,9 @@ def _pair(oracle_mesh, p, X, Y, Z, n, max_cells, budget=1 << 16, V=None, extra_c
     return sim, ctx, sys_
 
 
-def _cmp(sim, sys_, fields, n, tol=TOL, label=""):
+def _cmp(sim, sys_, fields, n, tol=None, label=""):
+    """Compares fields of the first n cells: bit for bit (tol None), or per
+    element within tol."""
     A = _abi()
     out = {}
     for f in fields:
@@ -79,11 +83,16 @@ def _cmp(sim, sys_, fields, n, tol=TOL, label=""):
             a = a[:6 * n]
             b = b[:6 * n]
         err = relerr(a, b)
+        eerr = elemerr(a, b)
         exact = float(np.mean(a == b))
         out[f] = (err, exact)
-        print("%s field %d: rel err %.3g, bit-exact fraction %.4f" % (label, f, err, exact))
+        print("%s field %d: scale-relative err %.3g, per-element err %.3g, bit-exact fraction %.6f" %
+              (label, f, err, eerr, exact))
         assert np.all(np.isfinite(a)), "non-finite values in field %d" % f
-        assert err <= tol, "field %d rel err %g > %g" % (f, err, tol)
+        if tol is None:
+            assert np.array_equal(a, b), "field %d is not bit-identical (per-element err %g)" % (f, eerr)
+        else:
+            assert eerr <= tol, "field %d per-element err %g > %g" % (f, eerr, tol)
     return out
 
 
@@ -109,7 +118,7 @@ def test_startup_matches_oracle(oracle_mesh):
     sim, ctx, s = _pair(oracle_mesh, p, X, Y, Z, n, mc)
     nv = _cmp_grid(sim, s)
     assert nv > 1
-    _cmp(sim, s, [A.F_CMX, A.F_CMY, A.F_CMZ, A.F_BBOX], n, tol=0.0, label="startup")
+    _cmp(sim, s, [A.F_CMX, A.F_CMY, A.F_CMZ, A.F_BBOX], n, label="startup")
     _cmp(sim, s, [A.F_FCX, A.F_FCY, A.F_FCZ, A.F_FDX, A.F_FDY, A.F_FDZ], n, label="startup")
     ctx.close()
 
@@ -172,8 +181,7 @@ def test_division_matches_oracle(oracle_mesh):
         assert n1 == sim.n, "cell count differs after %d steps: gpu %d oracle %d" % (total, n1, sim.n)
     assert sim.n > n, "no division happened; test is not exercising division"
     nn = sim.n
-    _cmp(sim, s, [A.F_X, A.F_Y, A.F_Z, A.F_VX, A.F_VY, A.F_VZ, A.F_PRESSURE, A.F_VOLUME], nn, tol=1e-4,
-         label="division")
+    _cmp(sim, s, [A.F_X, A.F_Y, A.F_Z, A.F_VX, A.F_VY, A.F_VZ, A.F_PRESSURE, A.F_VOLUME], nn, label="division")
     r1, r0 = s.get(A.F_RANMAR), sim.get(A.F_RANMAR)
     assert r1.i97 == r0.i97 and r1.j97 == r0.j97
     assert np.array_equal(np.ctypeslib.as_array(r1.u), np.ctypeslib.as_array(r0.u))
@@ -546,7 +554,7 @@ def test_parameter_branches_match_oracle(oracle_mesh, opt):
     nn = sim.n
     assert nn > n, "no division happened; the test does not exercise the branches"
     _cmp(sim, s, [A.F_X, A.F_Y, A.F_Z, A.F_VX, A.F_VY, A.F_VZ, A.F_FCX, A.F_FCY, A.F_FCZ, A.F_PRESSURE,
-                  A.F_VOLUME], nn, tol=1e-4, label=opt)
+                  A.F_VOLUME], nn, label=opt)
     e1, e0 = s.get(A.F_STIFFNESS)[:nn], sim.get(A.F_STIFFNESS)[:nn]
     assert np.array_equal(e1, e0)
     if opt == "soft_walls":
@@ -564,7 +572,7 @@ def test_staggered_divisions_in_a_tissue_match_oracle(oracle_mesh):
     a time over about 20 steps, as their volumes pass the division volume one after
     another.  After each such step only the cells near a divided cell search their
     contact candidates again; cells farther away keep their lists.  Cell counts must agree with the
-    oracle after every 5 steps, and the state bit for bit (1e-4 bar) at the end."""
+    oracle after every 5 steps, and the state bit for bit at the end."""
     A = _abi()
     from cellsim3d_amd import configs
     m, X, Y, Z, info = configs.lattice_tissue(64, 0.40, seed=11)
@@ -592,7 +600,7 @@ def test_staggered_divisions_in_a_tissue_match_oracle(oracle_mesh):
     jumps = [b - a for a, b in zip([n] + counts, counts)]
     assert counts[-1] > n and sum(1 for j in jumps if j) >= 3 and max(jumps) < n, "divisions not staggered: %s" % counts
     nn = sim.n
-    _cmp(sim, s, [A.F_X, A.F_Y, A.F_Z, A.F_VX, A.F_VY, A.F_VZ, A.F_FCX, A.F_FDX, A.F_PRESSURE], nn, tol=1e-4,
+    _cmp(sim, s, [A.F_X, A.F_Y, A.F_Z, A.F_VX, A.F_VY, A.F_VZ, A.F_FCX, A.F_FDX, A.F_PRESSURE], nn,
          label="staggered divisions")
     ctx.close()
 
@@ -600,15 +608,15 @@ def test_staggered_divisions_in_a_tissue_match_oracle(oracle_mesh):
 def test_growing_colony_matches_oracle(oracle_mesh):
     """The growing-colony workload at reduced size: one cell with per-cell growth
     scales U[0.8, 1.2], division at twice the cell volume, voxel lists of 128, the
-    grid rebuilt every 10 steps and after divisions.  It grows to about 17 cells in
-    4000 steps, dividing in parent order first and then a few cells at a time.  Cell
-    counts must agree with the oracle after every 250 steps; positions, velocities,
-    forces, pressures, volumes and centres within 1e-5 at the end, the voxel grid
-    bit for bit."""
+    grid rebuilt every 10 steps and after divisions.  It grows past 100 cells in
+    about 11,000 steps, dividing in parent order first and then a few cells at a
+    time, with partial searches after most divisions.  Cell counts must agree with
+    the oracle after every 250 steps; at the end positions, velocities, forces,
+    pressures, volumes, centres and the voxel grid are compared bit for bit."""
     A = _abi()
     E, O = _engine(), _oracle()
     from cellsim3d_amd import configs
-    mc = 64
+    mc = 256
     cfg = configs.colony(seed=1, max_cells=mc)
     p = cfg["params"]
     p.grid_interval = 10
@@ -621,7 +629,7 @@ def test_growing_colony_matches_oracle(oracle_mesh):
     s = ctx.add_system(p, cfg["X"], cfg["Y"], cfg["Z"], 1, max_cells=mc, voxels=budget)
     s.set(A.F_GROWTH_SCALE, cfg["growth_scale"])
     counts = []
-    for k in range(16):
+    for k in range(56):
         assert sim.step(250) & A.SYS_FATAL_MASK == 0
         s.step(250)
         ctx.sync()
@@ -629,10 +637,13 @@ def test_growing_colony_matches_oracle(oracle_mesh):
         assert status & A.SYS_FATAL_MASK == 0, hex(status)
         assert n1 == sim.n and st1 == sim.stepno, "after %d steps: gpu %d cells, oracle %d" % (st1, n1, sim.n)
         counts.append(n1)
+        print("colony: %d steps, %d cells" % (st1, n1), flush=True)
+        if n1 >= 100:
+            break
     print("colony cell counts every 250 steps:", counts, flush=True)
-    assert counts[-1] >= 12
+    assert counts[-1] >= 100
     n = sim.n
-    _cmp(sim, s, [A.F_X, A.F_Y, A.F_Z, A.F_VX, A.F_VY, A.F_VZ, A.F_FCX, A.F_FDX, A.F_PRESSURE, A.F_VOLUME,
-                  A.F_CMX, A.F_CMY, A.F_CMZ], n, label="colony")
+    _cmp(sim, s, [A.F_X, A.F_Y, A.F_Z, A.F_VX, A.F_VY, A.F_VZ, A.F_FCX, A.F_FCY, A.F_FCZ, A.F_FDX, A.F_FDY,
+                  A.F_FDZ, A.F_PRESSURE, A.F_VOLUME, A.F_AREA, A.F_CMX, A.F_CMY, A.F_CMZ], n, label="colony")
     _cmp_grid(sim, s)
     ctx.close()
