@@ -9,17 +9,24 @@ rebuilt every 10 steps, division off.  A step is one full pass of the step loop
 dissipative force, centre of mass, grid rebuild on its schedule) over all
 cells; the state is device-resident before the timed region starts.
 
-Prints ONE JSON line.  `roofline` covers the dominant kernel (the fused
-contact/friction kernel), timed live with HIP events on the engine's stream;
+Prints ONE JSON line.  The captured step graph is instantiated and uploaded
+before the warm-up, so the timed region holds graph replays only.  `roofline`
+covers the dominant kernel (the fused contact/friction kernel), timed live with
+HIP events on the engine's stream; its HBM fraction is reported next to its
+FP32 FLOP fraction (the kernel is limited by vector-ALU issue, not bytes).
 `cpu_baseline` is the CPU oracle (a C++/OpenMP restatement of the reference
 kernels) timed on this host on a bounded sample of the same workload.
 Further objects in the same line:
+  fp64_error       the FP32 engine against the oracle's FP64 mode on the same
+                   2000-cell input (forces, centres of mass, volumes)
   bonded_roofline  the standalone bonded-force kernel on 20,000 cells (3.6 M
                    nodes), springs + pressure (HBM-bound, 24 B per node) and
                    with bending (VALU-bound), HIP-event timed
-  ensemble         256 independent tissues batched in one context against the
-                   same tissues run one after another (plain path, sampled)
-  serving          a Poisson request stream through the continuous-batching loop
+  ensemble         256 independent tissues for 5000 steps with growth and
+                   division, batched in one context, against a size-stratified
+                   sample of the same tissues run one after another (plain path)
+  serving          256 requests of a Poisson stream through the
+                   continuous-batching loop, admitted against a 200 GB budget
 """
 import argparse
 import json
@@ -37,6 +44,16 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 #   dis_post:  reads x, v (24 B), writes F_dis (12 B)
 ALG_BYTES = {"integrate": 72.0, "con_dis": 48.0, "dis_post": 36.0}
 BONDED_BYTES = 24.0  # x in, F_con out, per node
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector peak
+
+
+def con_dis_flops(pc, pin):
+    """FP32 operations per node of k_con_dis: springs + pressure (75), bending
+    (12 angle terms of about 65), the contact pass over the node's candidates
+    (10 per candidate, 20 more per pair inside attraction_range), the first
+    friction pass over the accepted pairs (10 per candidate, 17 per pair, 15 for
+    bond damping, normal and drag) and the two velocity half-kicks (20)."""
+    return 75.0 + 780.0 + (10.0 * pc + 20.0 * pin) + (10.0 * pc + 17.0 * pin + 15.0) + 20.0
 
 
 def log(msg):
@@ -50,6 +67,46 @@ def load_traffic():
         with open(path) as f:
             return json.load(f)
     return None
+
+
+def fp64_error(cfg, gm, steps=10):
+    """Relative error of the FP32 engine against the oracle's FP64 mode, both
+    started from the same 2000-cell input and run for `steps` steps with growth
+    on (volumes every step) and division out of reach.  Per field: the largest
+    per-element error |a-b| / max(|b|, 1e-6 max|b|) and the scale-relative error
+    max|a-b| / max|b|.  Part of the CPU leg: the oracle is the checker here."""
+    import numpy as np
+    from cellsim3d_amd import abi, engine
+    from oracle import oracle as O
+    p = cfg["params"].copy()
+    p.growth_rate = 0.002
+    p.division_vol = 1.0e30
+    n = cfg["n"]
+    sim = O.Sim(p, gm, cfg["X"], cfg["Y"], cfg["Z"], n, max_cells=n, budget=1 << 16, precision=1)
+    sim.init()
+    sim.step(steps)
+    out = {"workload": "the 2000-cell input, %d steps, growth on (volumes every step), no division" % steps}
+    with engine.Context(gm, capacity_cells=n, voxel_budget=1 << 24) as ctx:
+        s = ctx.add_system(p, cfg["X"], cfg["Y"], cfg["Z"], n, max_cells=n, voxels=1 << 18)
+        s.step(steps)
+        ctx.sync()
+        groups = {"force_con": (abi.F_FCX, abi.F_FCY, abi.F_FCZ), "force_dis": (abi.F_FDX, abi.F_FDY, abi.F_FDZ),
+                  "position": (abi.F_X, abi.F_Y, abi.F_Z), "centre_of_mass": (abi.F_CMX, abi.F_CMY, abi.F_CMZ),
+                  "volume": (abi.F_VOLUME,)}
+        for name, fields in groups.items():
+            el, sc = 0.0, 0.0
+            for f in fields:
+                k = n * 192 if f in abi.NODE_FIELDS else n
+                a = s.get(f)[:k].astype(np.float64)
+                b = sim.get64(f)[:k]
+                if f in abi.NODE_FIELDS:  # padding slots are not nodes
+                    a = a.reshape(n, 192)[:, :180].ravel()
+                    b = b.reshape(n, 192)[:, :180].ravel()
+                m = max(float(np.max(np.abs(b))), 1e-30)
+                el = max(el, float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-6 * m))))
+                sc = max(sc, float(np.max(np.abs(a - b)) / m))
+            out[name] = {"per_element": el, "scale_relative": sc}
+    return out
 
 
 def cpu_baseline(cfg, mesh_tables, rest, budget_s=20.0):
@@ -129,42 +186,16 @@ def measure_extras(args, gm, log):
     out = {"bonded_roofline": bonded_roofline(gm, log=log)}
     g, tissues = ensemble.build(256, 1, 1, 512, args.ensemble_steps, 10)
     wg, gs, _ = ensemble.autotune(g, tissues, probe_steps=50)
-    dt_b, ctx, _ = ensemble.run_batched(g, tissues, args.ensemble_steps, wg_per_cu=wg, graph_steps=gs, warmup=gs)
-    ctx.close()
-    sample = min(100, args.ensemble_steps)
-    dt_p = ensemble.run_plain(g, tissues, args.ensemble_steps, sample_steps=sample)
-    nu = ensemble.node_updates(tissues, args.ensemble_steps)
-    cells = sum(t["n"] for t in tissues)
-    out["ensemble"] = {
-        "workload": "256 tissues, N ~ log-U[1, 512] (%d cells), stiffness, growth rate and adhesion x U[0.7, 1.3], "
-                    "%d steps, FP32" % (cells, args.ensemble_steps),
-        "batched_sims_per_hour": round(256 / dt_b * 3600.0, 1),
-        "batched_node_updates_per_s": nu / dt_b,
-        "plain_sims_per_hour": round(256 / dt_p * 3600.0, 1),
-        "plain_node_updates_per_s": nu / dt_p,
-        "plain_sample": "%d timed steps per tissue, scaled to %d" % (sample, args.ensemble_steps),
-        "batched_over_plain": round(dt_p / dt_b, 2),
-        "autotuned": {"wg_per_cu": wg, "graph_steps": gs},
-    }
-    log("bench: ensemble batched %.3f s, plain %.3f s (x%.1f)" % (dt_b, dt_p, dt_p / dt_b))
+    out["ensemble"] = ensemble.measure(g, tissues, args.ensemble_steps, wg, gs, args.plain_every, log)
+    log("bench: ensemble x%.2f over the plain loop" % out["ensemble"]["batched_over_plain"])
     reqs = configs.serving_requests(args.serving_requests, 1, 20.0, 8, 4096, 1e3, 1e5)
     jobs = serving.make_jobs(reqs)
-    srv = serving.Server(g, 200e9, max_cells_total=100000, chunk_steps=50)
+    srv = serving.Server(g, 200e9, chunk_steps=50)
     wall = srv.run(jobs)
     srv.close()
-    import numpy as np
-    lat = np.array([j.finished - j.arrival for j in jobs])
-    offered = sum(j.n * 180.0 * j.horizon for j in jobs) / jobs[-1].arrival
-    out["serving"] = {
-        "workload": "%d requests, Poisson 20/s, N ~ log-U[8, 4096], horizon ~ log-U[1e3, 1e5] steps, FP32, "
-                    "growth off, 200 GB budget" % len(jobs),
-        "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
-        "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
-        "node_updates_per_s": srv.node_updates / wall,
-        # offered node-updates/s over served node-updates/s: above 1 the requests queue
-        "offered_load": round(offered / (srv.node_updates / wall), 3),
-    }
-    log("bench: serving p50 %.3f s p99 %.3f s" % (out["serving"]["p50_latency_s"], out["serving"]["p99_latency_s"]))
+    out["serving"] = serving.summary(jobs, srv, wall)
+    log("bench: serving p50 %.3f s p99 %.3f s rho %.2f" % (out["serving"]["p50_latency_s"],
+                                                         out["serving"]["p99_latency_s"], out["serving"]["rho"]))
     return out
 
 
@@ -186,8 +217,10 @@ def main():
                     help="only the standalone bonded-kernel sweep (for profiling it); prints its JSON")
     ap.add_argument("--pair-skin", type=float, default=None,
                     help="contact-candidate skin (performance only: results do not depend on it)")
-    ap.add_argument("--ensemble-steps", type=int, default=1000)
-    ap.add_argument("--serving-requests", type=int, default=32)
+    ap.add_argument("--ensemble-steps", type=int, default=5000)
+    ap.add_argument("--plain-every", type=int, default=8,
+                    help="plain-path sample of the ensemble: every k-th tissue by size, full length")
+    ap.add_argument("--serving-requests", type=int, default=256)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -211,9 +244,10 @@ def main():
     log("bench: %d cells, phi %.2f, grid every %d steps" % (n, args.phi, args.grid_interval))
     ctx = engine.Context(gm, capacity_cells=n, voxel_budget=1 << 24)
     s = ctx.add_system(cfg["params"], cfg["X"], cfg["Y"], cfg["Z"], n, max_cells=n, voxels=1 << 18)
+    ctx.prepare_graph(s)  # capture, instantiate and upload before any timed step
     ctx.step(args.warmup, s)
     ctx.sync()
-    log("bench: warm-up done")
+    log("bench: graph prepared, warm-up done")
     t0 = time.perf_counter()
     ctx.step(args.steps, s)
     ctx.sync()
@@ -255,6 +289,15 @@ def main():
         "avg_launch_ms": round(dom_ms, 5),
         "alg_bytes_per_launch": alg,
     }
+    if dom == "con_dis":
+        # the kernel's limit is vector-ALU issue: its FLOP fraction next to the HBM one
+        fl = con_dis_flops(contacts["candidates_per_node_mean"], contacts["partners_per_node_mean"])
+        tf = fl * n * 180 / (dom_ms * 1e-3) / 1e12
+        roofline["limiter"] = "valu"
+        roofline["flops_per_node"] = round(fl, 1)
+        roofline["achieved_tflops"] = round(tf, 3)
+        roofline["valu_frac"] = round(tf / FP32_PEAK_TFLOPS, 5)
+        roofline["valu_peak_tflops"] = FP32_PEAK_TFLOPS
     log("bench: phases ms/step %s" % json.dumps({k: round(v, 4) for k, v in phases.items()}))
 
     calib = None
@@ -268,9 +311,12 @@ def main():
     if not args.no_extras:
         extras = measure_extras(args, gm, log)
     cpu = None
+    fp64 = None
     if not args.no_cpu:
         log("bench: CPU oracle baseline (~%.0f s)" % args.cpu_seconds)
         cpu = cpu_baseline(cfg, tables, rest, args.cpu_seconds)
+        log("bench: FP64 oracle run for the FP32 error")
+        fp64 = fp64_error(cfg, gm)
     ms_per_step = elapsed / args.steps * 1e3
     out = {
         "metric": "node-updates/s at 2k cells",
@@ -297,6 +343,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if fp64:
+        out["fp64_error"] = fp64
     out.update(extras)
     if calib:
         out["copy_calibration"] = calib

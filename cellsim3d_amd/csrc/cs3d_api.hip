@@ -25,6 +25,7 @@ void launch_divide(const Pool& p, const Range& r, int mode, hipStream_t st);
 extern thread_local std::vector<std::pair<hipEvent_t, hipEvent_t>>* kernel_timer;
 void launch_com_shift(const Pool& p, const Range& r, hipStream_t st);
 void launch_live(const Pool& p, const Range& r, hipStream_t st);
+void launch_set_nsys(const Pool& p, int n, hipStream_t st);
 void launch_grid(const Pool& p, const Range& r, int vox_blocks, hipStream_t st);
 void launch_vox_lists(const Pool& p, const Range& r, int vox_blocks, hipStream_t st);
 void launch_vrange(const Pool& p, const Range& r, hipStream_t st);
@@ -99,7 +100,12 @@ struct cs3d_ctx {
     int nsys_high = 0;
     bool use_graph = true;
     int graph_steps = 10;
-    std::map<int, std::pair<hipGraphExec_t, int>> graphs;  // target -> (exec, steps)
+    struct GraphRec {
+        hipGraphExec_t exec;
+        int steps;
+        bool shift;  // the captured steps include CorrectCoMMotion
+    };
+    std::map<int, GraphRec> graphs;  // target -> captured steps
     cs3d_mesh mesh_host;  // the tables the context was created with (checkpoint check)
     void* scratch = nullptr;  // small device buffer of the per-kernel entry points (grown on demand)
     size_t scratch_bytes = 0;
@@ -119,10 +125,13 @@ template <class T> cs3d_status dalloc(cs3d_ctx* c, T*& p, int64_t count) {
     return CS3D_OK;
 }
 
+// Drops the captured steps of one target (-2: all).  The all-systems capture
+// reads the systems in use from the device, so adding or removing a system
+// leaves it valid.
 void drop_graphs(cs3d_ctx* c, int target_only = -2) {
     for (auto it = c->graphs.begin(); it != c->graphs.end();) {
-        if (target_only == -2 || it->first == target_only || it->first == CS3D_ALL) {
-            (void)hipGraphExecDestroy(it->second.first);
+        if (target_only == -2 || it->first == target_only) {
+            (void)hipGraphExecDestroy(it->second.exec);
             it = c->graphs.erase(it);
         } else {
             ++it;
@@ -135,10 +144,12 @@ bool valid_sys(cs3d_ctx* c, cs3d_sys s) { return c && s >= 0 && s < (int)c->sys.
 Range range_of(cs3d_ctx* c, cs3d_sys s) {
     Range r;
     if (s == CS3D_ALL) {
+        // the whole pool and every system id: the kernels read the systems in use
+        // from the device, so a capture of this range survives adds and removes
         r.slot_lo = 0;
-        r.slot_hi = (int32_t)c->cells.high();
+        r.slot_hi = (int32_t)c->cap_cells;
         r.sys_lo = 0;
-        r.sys_hi = c->nsys_high;
+        r.sys_hi = kMaxSys;
         r.one_sys = 0;
         r.live = 0;
     } else {
@@ -366,6 +377,7 @@ extern "C" cs3d_status cs3d_ctx_create(int device, const cs3d_mesh* mesh, int64_
     if ((st = dalloc(c, p.bins, bin_budget)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.EVL, capacity_cells)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.slot_sys, capacity_cells)) != CS3D_OK) return cleanup(st);
+    if ((st = dalloc(c, p.nsys, 1)) != CS3D_OK) return cleanup(st);
     if ((st = dalloc(c, p.meta, kMaxSys)) != CS3D_OK) return cleanup(st);
     cs3d_params* prm = nullptr;
     if ((st = dalloc(c, prm, kMaxSys)) != CS3D_OK) return cleanup(st);
@@ -491,9 +503,12 @@ extern "C" cs3d_status cs3d_system_add_raw(cs3d_ctx* c, const cs3d_params* prm, 
         m.ext[2 * a + 1] = ih >= 0 ? ih : ih ^ 0x7FFFFFFF;
     }
     cs3d_ranmar_init(&m.rng, prm->ranmar_ij, prm->ranmar_kl);
+    launch_set_nsys(c->pool, c->nsys_high, c->stream);
     st = write_meta(c, sid, m);
     if (st != CS3D_OK) return st;
-    drop_graphs(c, CS3D_ALL);
+    // a captured all-systems step without CorrectCoMMotion cannot serve this system
+    auto g = c->graphs.find(CS3D_ALL);
+    if (prm->correct_com && g != c->graphs.end() && !g->second.shift) drop_graphs(c, CS3D_ALL);
     *out = sid;
     return CS3D_OK;
 }
@@ -549,6 +564,8 @@ extern "C" cs3d_status cs3d_system_remove(cs3d_ctx* c, cs3d_sys s) {
     for (int i = 0; i < (int)c->sys.size(); ++i)
         if (c->sys[i].used) hi = i + 1;
     c->nsys_high = hi;
+    launch_set_nsys(c->pool, hi, c->stream);
+    HIPCHK(hipGetLastError());
     return CS3D_OK;
 }
 
@@ -697,7 +714,7 @@ extern "C" cs3d_status cs3d_set(cs3d_ctx* c, cs3d_sys s, cs3d_field f, const voi
 
 // ---- checkpoint / restart -------------------------------------------------------
 // File layout (little-endian): "CS3DCKP1", u32 version, u32 sizeof(cs3d_params),
-// cs3d_params, i32 n, i32 saved_cap, i64 step, i64 horizon, u32 status,
+// cs3d_params, i32 n, i32 saved_cap, i64 step, i64 horizon, i64 cell steps, u32 status,
 // cs3d_ranmar_state, grid geometry {f32 gmin[3], i32 dims[3], f32 bmin[3], f32 bh,
 // i32 bdims[3]}, f32 system centre[3], u32 mesh checksum, then for cells 0..n-1: 15 node arrays
 // (x, y, z, v, F_con, F_dis, F_ran; 192 floats per cell each), 8 cell arrays (CM, P,
@@ -746,7 +763,7 @@ extern "C" cs3d_status cs3d_system_save(cs3d_ctx* c, cs3d_sys s, const char* pat
     if (!f) return fail(CS3D_INVALID_ARG, std::string("cannot open ") + path);
     bool ok = std::fwrite(kCkptMagic, 1, 8, f) == 8 && put(f, kCkptVersion) && put(f, (uint32_t)sizeof(cs3d_params)) &&
               put(f, h.prm) && put(f, (int32_t)n) && put(f, (int32_t)h.cap) && put(f, m.step) && put(f, m.horizon) &&
-              put(f, m.status) && put(f, m.rng);
+              put(f, m.cell_steps) && put(f, m.status) && put(f, m.rng);
     CkptGrid g;
     std::memcpy(g.gmin, m.gmin, sizeof(g.gmin));
     std::memcpy(g.dims, m.dims, sizeof(g.dims));
@@ -795,14 +812,14 @@ extern "C" cs3d_status cs3d_system_load(cs3d_ctx* c, const char* path, int32_t m
     uint32_t ver = 0, psize = 0, mcheck = 0;
     cs3d_params prm;
     int32_t n = 0, scap = 0;
-    int64_t step = 0, horizon = -1;
+    int64_t step = 0, horizon = -1, cell_steps = 0;
     uint32_t status = 0;
     cs3d_ranmar_state rng;
     CkptGrid g;
     float sys_cm[3];
     bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kCkptMagic, 8) == 0 && get(f, ver) &&
               ver == kCkptVersion && get(f, psize) && psize == sizeof(cs3d_params) && get(f, prm) && get(f, n) &&
-              get(f, scap) && get(f, step) && get(f, horizon) && get(f, status) && get(f, rng) && get(f, g) &&
+              get(f, scap) && get(f, step) && get(f, horizon) && get(f, cell_steps) && get(f, status) && get(f, rng) && get(f, g) &&
               get(f, sys_cm) && get(f, mcheck);
     if (!ok) {
         std::fclose(f);
@@ -848,6 +865,7 @@ extern "C" cs3d_status cs3d_system_load(cs3d_ctx* c, const char* path, int32_t m
     if (ok) {
         m.step = step;
         m.horizon = horizon;
+        m.cell_steps = cell_steps;
         m.status = status;
         m.rng = rng;
         std::memcpy(m.gmin, g.gmin, sizeof(g.gmin));
@@ -900,6 +918,115 @@ extern "C" cs3d_status cs3d_systems_poll(cs3d_ctx* c, int32_t first, int32_t cou
     return CS3D_OK;
 }
 
+extern "C" cs3d_status cs3d_systems_work(cs3d_ctx* c, int32_t first, int32_t count, int64_t* cell_steps) {
+    if (!c || !cell_steps || first < 0 || count < 0 || first + count > kMaxSys)
+        return fail(CS3D_INVALID_ARG, "bad work-query range");
+    if (count == 0) return CS3D_OK;
+    std::vector<SysMeta> m(count);
+    HIPCHK(hipMemcpyAsync(m.data(), c->pool.meta + first, sizeof(SysMeta) * count, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int32_t i = 0; i < count; ++i) cell_steps[i] = c->sys[first + i].used ? m[i].cell_steps : -1;
+    return CS3D_OK;
+}
+
+// Moves a system to a larger slot range (with voxel and search-bin regions sized
+// for it) and frees the old one.  Cells keep their local indices, so contact keys,
+// event order and every result are unchanged; the next step searches all
+// contact candidates again (which does not change any result either).
+extern "C" cs3d_status cs3d_system_grow(cs3d_ctx* c, cs3d_sys s, int32_t max_cells, int64_t voxels) {
+    if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
+    SysHost& h = c->sys[s];
+    if (max_cells <= h.cap) return fail(CS3D_INVALID_ARG, "the new room must exceed the current one");
+    SysMeta m;
+    cs3d_status st = read_meta(c, s, m);
+    if (st != CS3D_OK) return st;
+    int64_t base = 0, vreg = 0, breg = 0;
+    if (!c->cells.alloc(max_cells, base)) return fail(CS3D_CAPACITY, "cell pool exhausted");
+    int64_t vb = voxels > 0 ? voxels : std::max<int64_t>(4096, 32 * (int64_t)max_cells);
+    vb = std::min<int64_t>(std::max<int64_t>(vb, h.vbudget), 1ll << 30);
+    if (!c->voxels.alloc(vb * (1 + h.vcap), vreg)) {
+        c->cells.release(base);
+        return fail(CS3D_CAPACITY, "voxel pool exhausted");
+    }
+    int64_t bb = std::max<int64_t>(16384, 640 * (int64_t)max_cells);
+    bb = std::min<int64_t>(bb, c->bin_total / (1 + 4 * kBCap)) / 4 * 4;
+    while (bb >= h.bbudget && !c->bins.alloc(bb * (1 + 4 * kBCap) + 4, breg)) bb /= 2;
+    if (bb < h.bbudget) {  // the current bin grid must fit the new region
+        c->cells.release(base);
+        c->voxels.release(vreg);
+        return fail(CS3D_CAPACITY, "search-bin pool exhausted");
+    }
+    HIPCHK(hipSetDevice(c->device));
+    const Pool& p = c->pool;
+    hipStream_t q = c->stream;
+    const int64_t n = m.n, ob = h.base, ns = (int64_t)max_cells * kSlots;
+    // the new range as cs3d_system_add leaves it, then the live cells copied over
+    float* node[] = {p.X,   p.Y,   p.Z,   p.VX,  p.VY,  p.VZ,  p.VAX, p.VAY, p.VAZ, p.FCX, p.FCY,
+                     p.FCZ, p.FDX, p.FDY, p.FDZ, p.FRX, p.FRY, p.FRZ, p.XBX, p.XBY, p.XBZ};
+    for (float* a : node) {
+        HIPCHK(hipMemsetAsync(a + base * kSlots, 0, ns * 4, q));
+        HIPCHK(hipMemcpyAsync(a + base * kSlots, a + ob * kSlots, n * kSlots * 4, hipMemcpyDeviceToDevice, q));
+    }
+    int32_t* nodei[] = {p.NPAIR, p.NCAND};
+    for (int32_t* a : nodei) {
+        HIPCHK(hipMemsetAsync(a + base * kSlots, 0, ns * 4, q));
+        HIPCHK(hipMemcpyAsync(a + base * kSlots, a + ob * kSlots, n * kSlots * 4, hipMemcpyDeviceToDevice, q));
+    }
+    int32_t* lists[] = {p.PAIRS, p.CAND};
+    for (int32_t* a : lists)
+        HIPCHK(hipMemcpyAsync(a + base * kPMax * kSlots, a + ob * kPMax * kSlots, n * kPMax * kSlots * 4,
+                              hipMemcpyDeviceToDevice, q));
+    std::vector<float> pv(max_cells, h.prm.min_pressure), ev(max_cells, h.prm.stiffness1), gv(max_cells, 1.0f);
+    HIPCHK(hipMemcpyAsync(p.P + base, pv.data(), max_cells * 4, hipMemcpyHostToDevice, q));
+    HIPCHK(hipMemcpyAsync(p.E + base, ev.data(), max_cells * 4, hipMemcpyHostToDevice, q));
+    HIPCHK(hipMemcpyAsync(p.GS + base, gv.data(), max_cells * 4, hipMemcpyHostToDevice, q));
+    float* cellz[] = {p.CMX, p.CMY, p.CMZ, p.VOL, p.AREA};
+    for (float* a : cellz) HIPCHK(hipMemsetAsync(a + base, 0, (size_t)max_cells * 4, q));
+    float* cells[] = {p.CMX, p.CMY, p.CMZ, p.P, p.E, p.VOL, p.AREA, p.GS};
+    for (float* a : cells) HIPCHK(hipMemcpyAsync(a + base, a + ob, n * 4, hipMemcpyDeviceToDevice, q));
+    HIPCHK(hipMemsetAsync(p.FLAG + base, 0, max_cells, q));
+    HIPCHK(hipMemcpyAsync(p.FLAG + base, p.FLAG + ob, n, hipMemcpyDeviceToDevice, q));
+    HIPCHK(hipMemsetAsync(p.RESET + base, 0, max_cells, q));
+    float* boxes[] = {p.BBOX, p.BBOXN, p.SBOX};
+    for (float* a : boxes) {
+        HIPCHK(hipMemsetAsync(a + base * 6, 0, (size_t)max_cells * 24, q));
+        HIPCHK(hipMemcpyAsync(a + base * 6, a + ob * 6, n * 24, hipMemcpyDeviceToDevice, q));
+    }
+    HIPCHK(hipMemcpyAsync(p.VRANGE + base * 6, p.VRANGE + ob * 6, n * 24, hipMemcpyDeviceToDevice, q));
+    // voxel counts of the current grid (the voxel index depends on the grid dims only),
+    // the rest of the voxel region and the bin counters cleared
+    HIPCHK(hipMemsetAsync(p.vox + vreg, 0, (size_t)vb * (1 + h.vcap) * 4, q));
+    HIPCHK(hipMemcpyAsync(p.vox + vreg, p.vox + h.vreg, (size_t)h.vbudget * 4, hipMemcpyDeviceToDevice, q));
+    HIPCHK(hipMemsetAsync(p.bins + breg, 0, (size_t)bb * 4, q));
+    std::vector<int32_t> owner(max_cells, s), none(h.cap, -1);
+    HIPCHK(hipMemcpyAsync(p.slot_sys + base, owner.data(), max_cells * 4, hipMemcpyHostToDevice, q));
+    HIPCHK(hipMemcpyAsync(p.slot_sys + ob, none.data(), h.cap * 4, hipMemcpyHostToDevice, q));
+    m.base = (int32_t)base;
+    m.cap = max_cells;
+    m.vbase = vreg;
+    m.lbase = vreg + vb;
+    m.vbudget = (int32_t)vb;
+    m.bbudget = (int32_t)bb;
+    m.bbase = breg;
+    m.ebase = (breg + bb + 3) / 4 * 4;
+    m.need_search = 1;
+    m.n_reset = 0;
+    m.search_next = 0;
+    st = write_meta(c, s, m);  // synchronises the stream (the host vectors above are still alive)
+    if (st != CS3D_OK) return st;
+    c->cells.release(h.base);
+    c->voxels.release(h.vreg);
+    c->bins.release(h.breg);
+    h.base = (int32_t)base;
+    h.cap = max_cells;
+    h.vreg = vreg;
+    h.vbudget = (int32_t)vb;
+    h.breg = breg;
+    h.bbudget = (int32_t)bb;
+    drop_graphs(c, s);
+    return CS3D_OK;
+}
+
 extern "C" cs3d_status cs3d_set_graph_steps(cs3d_ctx* c, int32_t steps) {
     if (!c || steps < 1 || steps > 1000) return fail(CS3D_INVALID_ARG, "graph steps must be in [1, 1000]");
     c->graph_steps = steps;
@@ -949,29 +1076,58 @@ extern "C" cs3d_status cs3d_step_timed(cs3d_ctx* c, cs3d_sys s, int32_t n_steps,
     return st;
 }
 
+namespace {
+// The captured graph_steps steps of a target, captured, instantiated and
+// uploaded on first use.
+cs3d_status graph_of(cs3d_ctx* c, cs3d_sys s, const cs3d_ctx::GraphRec** out) {
+    auto it = c->graphs.find(s);
+    if (it == c->graphs.end()) {
+        const Range r = range_of(c, s);
+        const int vb = vox_blocks_of(c, s);
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < c->graph_steps; ++k) enqueue_step(c, r, vb);
+        HIPCHK(hipStreamEndCapture(c->stream, &g));
+        hipGraphExec_t ex = nullptr;
+        hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(CS3D_HIP_ERROR, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        e = hipGraphUpload(ex, c->stream);
+        if (e != hipSuccess) {
+            (void)hipGraphExecDestroy(ex);
+            return fail(CS3D_HIP_ERROR, std::string("hipGraphUpload: ") + hipGetErrorString(e));
+        }
+        it = c->graphs.emplace(s, cs3d_ctx::GraphRec{ex, c->graph_steps, any_correct_com(c, r)}).first;
+    }
+    *out = &it->second;
+    return CS3D_OK;
+}
+}  // namespace
+
+extern "C" cs3d_status cs3d_graph_prepare(cs3d_ctx* c, cs3d_sys s) {
+    if (!c) return fail(CS3D_INVALID_ARG, "NULL context");
+    if (s != CS3D_ALL && !valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
+    if (!c->use_graph) return CS3D_OK;
+    const cs3d_ctx::GraphRec* g = nullptr;
+    cs3d_status st = graph_of(c, s, &g);
+    if (st != CS3D_OK) return st;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CS3D_OK;
+}
+
 extern "C" cs3d_status cs3d_step(cs3d_ctx* c, cs3d_sys s, int32_t n_steps) {
     if (!c) return fail(CS3D_INVALID_ARG, "NULL context");
     if (s != CS3D_ALL && !valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
     if (n_steps <= 0) return CS3D_OK;
+    if (s == CS3D_ALL && c->nsys_high == 0) return CS3D_OK;
     const Range r = range_of(c, s);
-    if (r.slot_hi <= r.slot_lo || r.sys_hi <= r.sys_lo) return CS3D_OK;
     const int vb = vox_blocks_of(c, s);
     int done = 0;
     if (c->use_graph && n_steps >= c->graph_steps) {
-        auto it = c->graphs.find(s);
-        if (it == c->graphs.end()) {
-            hipGraph_t g = nullptr;
-            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            for (int k = 0; k < c->graph_steps; ++k) enqueue_step(c, r, vb);
-            HIPCHK(hipStreamEndCapture(c->stream, &g));
-            hipGraphExec_t ex = nullptr;
-            hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (e != hipSuccess) return fail(CS3D_HIP_ERROR, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-            it = c->graphs.emplace(s, std::make_pair(ex, c->graph_steps)).first;
-        }
-        const int per = it->second.second;
-        for (; done + per <= n_steps; done += per) HIPCHK(hipGraphLaunch(it->second.first, c->stream));
+        const cs3d_ctx::GraphRec* g = nullptr;
+        cs3d_status st = graph_of(c, s, &g);
+        if (st != CS3D_OK) return st;
+        for (; done + g->steps <= n_steps; done += g->steps) HIPCHK(hipGraphLaunch(g->exec, c->stream));
     }
     for (; done < n_steps; ++done) enqueue_step(c, r, vb);
     HIPCHK(hipGetLastError());

@@ -68,6 +68,7 @@ struct SysMeta {
     float sys_cm[3];      // system centre subtracted at the end of the latest step (correct_com)
     int32_t shift_pending;  // 1 = the step that just ended computed sys_cm and its nodes are still to be
                             // shifted (set by the division phase, consumed by the shift kernel)
+    int64_t cell_steps;   // sum over the completed steps of the cells stepped (node-updates / 180)
     cs3d_ranmar_state rng;
 };
 
@@ -110,6 +111,8 @@ struct Pool {
     int32_t* n_live; // int32[1]: entries of LIVE
     int32_t* EVL;
     int32_t* slot_sys;
+    int32_t* nsys;   // int32[1]: system ids in use are below this (kernels over all systems read it, so a
+                     // captured all-systems step stays valid when systems are added or removed)
     SysMeta* meta;
     const cs3d_params* prm;
     int32_t* vox;

@@ -48,6 +48,20 @@ __device__ __forceinline__ void flag(SysMeta* m, uint32_t bits) {
     if (bits) atomicOr(&m->status, bits);
 }
 
+// Loads and stores at a uniform base pointer plus a 32-bit element index: the
+// address is an SGPR pair plus one VGPR byte offset (the saddr form of the global
+// memory instructions), so per-lane addressing costs one 32-bit register that all
+// arrays of a cell share instead of a 64-bit pointer per array.  The byte offset
+// must stay below 4 GiB: a cell's node offset (< 768 B), a list entry (< 31 KB),
+// or a node key of a system (a context holds at most 2^30 / 192 cells, so every
+// key * 4 < 2^32).
+template <class T> __device__ __forceinline__ T ldu(const T* base, uint32_t i) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (uint32_t)sizeof(T));
+}
+template <class T> __device__ __forceinline__ void stu(T* base, uint32_t i, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (uint32_t)sizeof(T)) = v;
+}
+
 // rGrowth of the population model, GPUbounce.cu:1077-1088
 __device__ __forceinline__ float rgrowth(const cs3d_params* P, int n) {
     if (P->do_pop_model == 1) {
@@ -64,6 +78,37 @@ __device__ __forceinline__ int vox1(float x, float mn, float DL, int div) {
     if (s < 0) s = 0;
     if (s >= div) s = div - 1;
     return s;
+}
+
+// Read-only views of a system's parameters and metadata in the constant address
+// space: a kernel reads them with scalar loads into scalar registers (the values
+// are the same for every lane of a workgroup).  Only for data no kernel writes
+// while the reader runs; the scalar cache is only read through.
+typedef __attribute__((address_space(4))) const cs3d_params* CParams;
+typedef __attribute__((address_space(4))) const SysMeta* CMeta;
+__device__ __forceinline__ CParams cparams(const cs3d_params* q) { return (CParams)q; }
+__device__ __forceinline__ CMeta cmeta(const SysMeta* q) { return (CMeta)q; }
+template <class T> __device__ __forceinline__ T cld(const T* q) {
+    return *(__attribute__((address_space(4))) const T*)q;
+}
+
+// The cell slot of loop iteration `it` of a cell kernel, made wave-uniform with
+// readfirstlane: the per-cell base pointers derived from it stay in scalar
+// registers instead of becoming per-lane 64-bit induction variables.
+template <int kLive>
+__device__ __forceinline__ int cell_slot(const Pool& p, const Range& r, int it) {
+    return __builtin_amdgcn_readfirstlane(kLive ? p.LIVE[it] : r.slot_lo + it);
+}
+
+// The lane's node index as a value the compiler cannot see through (an empty asm
+// statement per loop iteration): otherwise it combines "array + lane" into one
+// loop-invariant 64-bit pointer per array and lane, which costs two VGPRs per
+// array for the whole kernel.  With it, every access is the cell's uniform base
+// plus this one 32-bit offset.
+__device__ __forceinline__ uint32_t lane_index() {
+    uint32_t t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
 }
 
 // XCD-aware first cell of a workgroup.  The dispatcher hands workgroup v to XCD
@@ -304,17 +349,18 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     // a multi-system step walks the compact list of live slots (k_live)
     const int nit = kLive ? *p.n_live : r.slot_hi - r.slot_lo;
     for (int it = xcd_block(); it < nit; it += gridDim.x) {
-        const int slot = kLive ? p.LIVE[it] : r.slot_lo + it;
+        const int slot = cell_slot<kLive>(p, r, it);
         // the cell's node data are loaded before the system checks (every slot of the
         // range is allocated), so their latency overlaps the metadata reads
-        const int64_t gi = (int64_t)slot * kSlots + tid;
-        float x = p.X[gi], y = p.Y[gi], z = p.Z[gi];
+        const size_t cb = (size_t)slot * kSlots;  // the cell's first node (uniform)
+        const uint32_t ln = lane_index();
+        float x = ldu(p.X + cb, ln), y = ldu(p.Y + cb, ln), z = ldu(p.Z + cb, ln);
         const bool kick = flags & I_KICK;
         float vx = 0.f, vy = 0.f, vz = 0.f, fcx = 0.f, fcy = 0.f, fcz = 0.f, fdx = 0.f, fdy = 0.f, fdz = 0.f;
         if (kick) {
-            vx = p.VX[gi]; vy = p.VY[gi]; vz = p.VZ[gi];
-            fcx = p.FCX[gi]; fcy = p.FCY[gi]; fcz = p.FCZ[gi];
-            fdx = p.FDX[gi]; fdy = p.FDY[gi]; fdz = p.FDZ[gi];
+            vx = ldu(p.VX + cb, ln); vy = ldu(p.VY + cb, ln); vz = ldu(p.VZ + cb, ln);
+            fcx = ldu(p.FCX + cb, ln); fcy = ldu(p.FCY + cb, ln); fcz = ldu(p.FCZ + cb, ln);
+            fdx = ldu(p.FDX + cb, ln); fdy = ldu(p.FDY + cb, ln); fdz = ldu(p.FDZ + cb, ln);
         }
         const int s = sys_of(p, r, slot);
         if (s < 0) continue;
@@ -350,15 +396,15 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             const float dt = P->dt, m = P->mass;
             const float rdt = sqrtf(dt);
             const bool rnd = P->add_rands != 0;  // the previous step's random force (d_fRanList)
-            vx = half_kick(vx, fcx, fdx, rnd ? p.FRX[gi] : 0.f, dt, rdt, m);
-            vy = half_kick(vy, fcy, fdy, rnd ? p.FRY[gi] : 0.f, dt, rdt, m);
-            vz = half_kick(vz, fcz, fdz, rnd ? p.FRZ[gi] : 0.f, dt, rdt, m);
+            vx = half_kick(vx, fcx, fdx, rnd ? ldu(p.FRX + cb, ln) : 0.f, dt, rdt, m);
+            vy = half_kick(vy, fcy, fdy, rnd ? ldu(p.FRY + cb, ln) : 0.f, dt, rdt, m);
+            vz = half_kick(vz, fcz, fdz, rnd ? ldu(p.FRZ + cb, ln) : 0.f, dt, rdt, m);
             x = x + vx * dt;
             y = y + vy * dt;
             z = z + vz * dt;
-            p.VAX[gi] = vx; p.VAY[gi] = vy; p.VAZ[gi] = vz;
-            if (flags & I_WRITEV) { p.VX[gi] = vx; p.VY[gi] = vy; p.VZ[gi] = vz; }
-            p.X[gi] = x; p.Y[gi] = y; p.Z[gi] = z;
+            stu(p.VAX + cb, ln, vx); stu(p.VAY + cb, ln, vy); stu(p.VAZ + cb, ln, vz);
+            if (flags & I_WRITEV) { stu(p.VX + cb, ln, vx); stu(p.VY + cb, ln, vy); stu(p.VZ + cb, ln, vz); }
+            stu(p.X + cb, ln, x); stu(p.Y + cb, ln, y); stu(p.Z + cb, ln, z);
         }
         if (node && (flags & I_CLEAR)) p.bins[M->bbase + bin_of(M, x, y, z)] = 0;
         if ((flags & I_BINS) && (!(flags & I_IFSEARCH) || M->need_search)) {
@@ -369,7 +415,7 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             // cells that did not divide are binned at their last search position
             float qx = x, qy = y, qz = z;
             if ((flags & I_IFSEARCH) && M->need_search == 2 && node && !p.RESET[slot]) {
-                qx = p.XBX[gi]; qy = p.XBY[gi]; qz = p.XBZ[gi];
+                qx = ldu(p.XBX + cb, ln); qy = ldu(p.XBY + cb, ln); qz = ldu(p.XBZ + cb, ln);
             }
             const int b = node ? bin_of(M, qx, qy, qz) : -1;
             const int lane = tid & 63;
@@ -502,13 +548,15 @@ struct RuleCtx {
     int xd, yd, zd;
 };
 
-__device__ __forceinline__ bool rule_ok(const int32_t* __restrict__ vr, const float* __restrict__ bb, int vx, int vy,
-                                        int vz, float X, float Y, float Z, float ar2) {
+// sVR, sBB: voxel ranges and boxes from the system's first cell (uniform); b: the
+// partner cell's local index
+__device__ __forceinline__ bool rule_ok(const int32_t* __restrict__ sVR, const float* __restrict__ sBB, uint32_t b,
+                                        int vx, int vy, int vz, float X, float Y, float Z, float ar2) {
     // all twelve words are loaded before any test (8-byte aligned pairs)
-    const int2 r01 = reinterpret_cast<const int2*>(vr)[0], r23 = reinterpret_cast<const int2*>(vr)[1],
-               r45 = reinterpret_cast<const int2*>(vr)[2];
-    const float2 c01 = reinterpret_cast<const float2*>(bb)[0], c23 = reinterpret_cast<const float2*>(bb)[1],
-                 c45 = reinterpret_cast<const float2*>(bb)[2];
+    const int2* vr = reinterpret_cast<const int2*>(sVR);
+    const float2* bb = reinterpret_cast<const float2*>(sBB);
+    const int2 r01 = ldu(vr, 3 * b), r23 = ldu(vr, 3 * b + 1), r45 = ldu(vr, 3 * b + 2);
+    const float2 c01 = ldu(bb, 3 * b), c23 = ldu(bb, 3 * b + 1), c45 = ldu(bb, 3 * b + 2);
     const bool inlist = (r01.x <= vx) & (vx <= r01.y) & (r23.x <= vy) & (vy <= r23.y) & (r45.x <= vz) & (vz <= r45.y);
     const float b0 = c01.x, b1 = c01.y, b2 = c23.x, b3 = c23.y, b4 = c45.x, b5 = c45.y;
     float qx = (float)(X - b1 > 0.0f) * (X - b1);
@@ -520,7 +568,8 @@ __device__ __forceinline__ bool rule_ok(const int32_t* __restrict__ vr, const fl
     return inlist && !(qx * qx + qy * qy + qz * qz > ar2);
 }
 
-__device__ __forceinline__ RuleCtx rule_ctx(const SysMeta* M, const cs3d_params* P) {
+template <class MP, class PP>
+__device__ __forceinline__ RuleCtx rule_ctx(MP M, PP P) {
     RuleCtx R;
     const float ar = P->attraction_range;
     R.ar2 = ar * ar;
@@ -568,32 +617,31 @@ struct PairConsts {
     float ar, ar2, rr, ka, kr, gext;
 };
 
-// Partner key -> global node index (keys are local cell * 192 + node).
-__device__ __forceinline__ int64_t key_node(int base, int key) {
-    const int b = key / kSlots;
-    return (int64_t)(base + b) * kSlots + (key - b * kSlots);
-}
+// Partner keys are local cell * 192 + node: the index of the node in a system's
+// arrays, read through the system's first slot (a uniform pointer).
 
 constexpr int kChunk = 4;  // partners whose loads are issued together
 
 // Friction (propagate.cu:633-653) over the node's stored partners, in stored
 // (reference) order.  The keys, then the partner velocities, of kChunk partners
 // are loaded before any of them is used.
-__device__ __forceinline__ void pair_friction(const int32_t* lst, int base, int np, float vix, float viy, float viz,
+// lst: the cell's list (uniform), entries at i * 192 + tid; sVX/sVY/sVZ: the
+// velocity arrays from the system's first slot (uniform).
+__device__ __forceinline__ void pair_friction(const int32_t* lst, int tid, int np, float vix, float viy, float viz,
                                               float nx, float ny, float nz, float gext,
-                                              const float* __restrict__ rVX, const float* __restrict__ rVY,
-                                              const float* __restrict__ rVZ, float& fdx, float& fdy, float& fdz) {
+                                              const float* __restrict__ sVX, const float* __restrict__ sVY,
+                                              const float* __restrict__ sVZ, float& fdx, float& fdy, float& fdz) {
     for (int i0 = 0; i0 < np; i0 += kChunk) {
-        int64_t g[kChunk];
+        uint32_t g[kChunk];
 #pragma unroll
-        for (int j = 0; j < kChunk; ++j) g[j] = i0 + j < np ? key_node(base, lst[(i0 + j) * kSlots]) : 0;
+        for (int j = 0; j < kChunk; ++j) g[j] = i0 + j < np ? (uint32_t)ldu(lst, (uint32_t)((i0 + j) * kSlots + tid)) : 0u;
         float wx[kChunk], wy[kChunk], wz[kChunk];
 #pragma unroll
         for (int j = 0; j < kChunk; ++j) {
             const bool in = i0 + j < np;
-            wx[j] = in ? rVX[g[j]] : 0.f;
-            wy[j] = in ? rVY[g[j]] : 0.f;
-            wz[j] = in ? rVZ[g[j]] : 0.f;
+            wx[j] = in ? ldu(sVX, g[j]) : 0.f;
+            wy[j] = in ? ldu(sVY, g[j]) : 0.f;
+            wz[j] = in ? ldu(sVZ, g[j]) : 0.f;
         }
 #pragma unroll
         for (int j = 0; j < kChunk; ++j) {
@@ -618,27 +666,30 @@ __device__ __forceinline__ void pair_friction(const int32_t* lst, int base, int 
 // are already loading, so the key -> position dependence of each candidate does
 // not serialise the loop.
 // Returns the number accepted.
+// cand, rec: the cell's lists (uniform), entries at i * 192 + tid; sX .. sVZ,
+// sVR, sBB: node arrays, voxel ranges and boxes from the system's first slot
+// (uniform).
 template <bool CON, bool DIS>
-__device__ __forceinline__ int accept_forces(const int32_t* cand, int nc, int32_t* rec, int base, float X, float Y,
-                                             float Z, const RuleCtx& R, const int32_t* __restrict__ vrange,
-                                             const float* __restrict__ rBB, float vix,
+__device__ __forceinline__ int accept_forces(const int32_t* cand, int tid, int nc, int32_t* rec, float X, float Y,
+                                             float Z, const RuleCtx& R, const int32_t* __restrict__ sVR,
+                                             const float* __restrict__ sBB, float vix,
                                              float viy, float viz, float nx, float ny, float nz, const PairConsts& k,
-                                             const float* __restrict__ rX, const float* __restrict__ rY,
-                                             const float* __restrict__ rZ, const float* __restrict__ rVX,
-                                             const float* __restrict__ rVY, const float* __restrict__ rVZ,
+                                             const float* __restrict__ sX, const float* __restrict__ sY,
+                                             const float* __restrict__ sZ, const float* __restrict__ sVX,
+                                             const float* __restrict__ sVY, const float* __restrict__ sVZ,
                                              float& cfx, float& cfy, float& cfz, float& fdx, float& fdy,
                                              float& fdz) {
     int np = 0;
     int lastb = -1;
     bool lastok = false;
     int vx = -1, vy = -1, vz = -1;  // the node's voxel (propagate.cu:360-372), found at first use
-    int key_n = nc > 0 ? cand[0] : 0;
-    int key_nn = nc > 1 ? cand[kSlots] : 0;
+    int key_n = nc > 0 ? ldu(cand, (uint32_t)tid) : 0;
+    int key_nn = nc > 1 ? ldu(cand, (uint32_t)(kSlots + tid)) : 0;
     float pxn = 0.f, pyn = 0.f, pzn = 0.f, wxn = 0.f, wyn = 0.f, wzn = 0.f;
     if (nc > 0) {
-        const int64_t g = key_node(base, key_n);
-        pxn = rX[g]; pyn = rY[g]; pzn = rZ[g];
-        if (DIS) { wxn = rVX[g]; wyn = rVY[g]; wzn = rVZ[g]; }
+        const uint32_t g = (uint32_t)key_n;
+        pxn = ldu(sX, g); pyn = ldu(sY, g); pzn = ldu(sZ, g);
+        if (DIS) { wxn = ldu(sVX, g); wyn = ldu(sVY, g); wzn = ldu(sVZ, g); }
     }
     for (int i = 0; i < nc; ++i) {
         const int key = key_n;
@@ -646,11 +697,11 @@ __device__ __forceinline__ int accept_forces(const int32_t* cand, int nc, int32_
         const float wx = wxn, wy = wyn, wz = wzn;
         key_n = key_nn;
         if (i + 1 < nc) {
-            const int64_t g = key_node(base, key_n);
-            pxn = rX[g]; pyn = rY[g]; pzn = rZ[g];
-            if (DIS) { wxn = rVX[g]; wyn = rVY[g]; wzn = rVZ[g]; }
+            const uint32_t g = (uint32_t)key_n;
+            pxn = ldu(sX, g); pyn = ldu(sY, g); pzn = ldu(sZ, g);
+            if (DIS) { wxn = ldu(sVX, g); wyn = ldu(sVY, g); wzn = ldu(sVZ, g); }
         }
-        if (i + 2 < nc) key_nn = cand[(i + 2) * kSlots];
+        if (i + 2 < nc) key_nn = ldu(cand, (uint32_t)((i + 2) * kSlots + tid));
         const int b = key / kSlots;
         const float dX = X - px, dY = Y - py, dZ = Z - pz;
         const float R2 = dX * dX + dY * dY + dZ * dZ;
@@ -662,11 +713,10 @@ __device__ __forceinline__ int accept_forces(const int32_t* cand, int nc, int32_
                 vz = vox1(Z, R.gz, R.DL, R.zd);
             }
             lastb = b;
-            lastok = rule_ok(vrange + (int64_t)(base + b) * 6, rBB + (int64_t)(base + b) * 6, vx, vy, vz, X, Y, Z,
-                             R.ar2);
+            lastok = rule_ok(sVR, sBB, (uint32_t)b, vx, vy, vz, X, Y, Z, R.ar2);
         }
         if (!lastok) continue;
-        rec[np * kSlots] = key;
+        stu(rec, (uint32_t)(np * kSlots + tid), key);
         ++np;
         if (CON && R2 < k.ar2) {
             const float Rl = sqrtf(R2);
@@ -713,7 +763,7 @@ __global__ __launch_bounds__(192) void k_search(Pool p, Range r, int32_t always)
     // a multi-system step walks the compact list of live slots (k_live)
     const int nit = kLive ? *p.n_live : r.slot_hi - r.slot_lo;
     for (int it = xcd_block(); it < nit; it += gridDim.x) {
-        const int slot = kLive ? p.LIVE[it] : r.slot_lo + it;
+        const int slot = cell_slot<kLive>(p, r, it);
         const int s = sys_of(p, r, slot);
         if (s < 0) continue;
         SysMeta* M = p.meta + s;
@@ -721,8 +771,9 @@ __global__ __launch_bounds__(192) void k_search(Pool p, Range r, int32_t always)
         const int c = slot - M->base;
         if (c >= M->n) continue;
         const cs3d_params* P = p.prm + s;
-        const int64_t gi = (int64_t)slot * kSlots + tid;
-        float X = p.X[gi], Y = p.Y[gi], Z = p.Z[gi];
+        const size_t cb = (size_t)slot * kSlots;  // the cell's first node (uniform)
+        const uint32_t ln = lane_index();
+        float X = ldu(p.X + cb, ln), Y = ldu(p.Y + cb, ln), Z = ldu(p.Z + cb, ln);
         bool reset = true;
         if (!always && M->need_search == 2) {
             // Partial search, in search positions.  The cells listed in RSL (RESET: a
@@ -759,7 +810,7 @@ __global__ __launch_bounds__(192) void k_search(Pool p, Range r, int32_t always)
                 const bool go = near != 0;
                 __syncthreads();  // near is reset by the next cell
                 if (!go) continue;
-                X = p.XBX[gi]; Y = p.XBY[gi]; Z = p.XBZ[gi];
+                X = ldu(p.XBX + cb, ln); Y = ldu(p.XBY + cb, ln); Z = ldu(p.XBZ + cb, ln);
             }
             reset = own;
         } else {
@@ -770,10 +821,10 @@ __global__ __launch_bounds__(192) void k_search(Pool p, Range r, int32_t always)
         if (node) {
             uint32_t st = 0;
             const int nc = find_candidates(p, M, P, c, X, Y, Z, lst, tid, st);
-            int32_t* cstore = p.CAND + (int64_t)slot * kPMax * kSlots + tid;
-            for (int i = 0; i < nc; ++i) cstore[i * kSlots] = lst[i][tid];
-            p.NCAND[gi] = nc;
-            if (reset) { p.XBX[gi] = X; p.XBY[gi] = Y; p.XBZ[gi] = Z; }
+            int32_t* cstore = p.CAND + (size_t)slot * kPMax * kSlots;
+            for (int i = 0; i < nc; ++i) stu(cstore, (uint32_t)(i * kSlots) + ln, lst[i][tid]);
+            stu(p.NCAND + cb, ln, nc);
+            if (reset) { stu(p.XBX + cb, ln, X); stu(p.XBY + cb, ln, Y); stu(p.XBZ + cb, ln, Z); }
             if (st) flag(M, st);
         }
     }
@@ -809,21 +860,26 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     // a multi-system step walks the compact list of live slots (k_live)
     const int nit = kLive ? *p.n_live : r.slot_hi - r.slot_lo;
     for (int it = xcd_block(); it < nit; it += gridDim.x) {
-        const int slot = kLive ? p.LIVE[it] : r.slot_lo + it;
+        const int slot = cell_slot<kLive>(p, r, it);
         // node data first (every slot of the range is allocated): their latency
         // overlaps the metadata reads
-        const int64_t gi = (int64_t)slot * kSlots + tid;
-        const float X = rX[gi], Y = rY[gi], Z = rZ[gi];
+        const size_t cb = (size_t)slot * kSlots;  // the cell's first node (uniform)
+        const uint32_t ln = lane_index();
+        const float X = ldu(rX + cb, ln), Y = ldu(rY + cb, ln), Z = ldu(rZ + cb, ln);
         float vX = 0.f, vY = 0.f, vZ = 0.f;
-        if (fused) { vX = rVX[gi]; vY = rVY[gi]; vZ = rVZ[gi]; }
-        const int nc_pre = search ? p.NCAND[gi] : 0;
-        const int s = sys_of(p, r, slot);
+        if (fused) { vX = ldu(rVX + cb, ln); vY = ldu(rVY + cb, ln); vZ = ldu(rVZ + cb, ln); }
+        const int nc_pre = search ? ldu(p.NCAND + cb, ln) : 0;
+        const int s = __builtin_amdgcn_readfirstlane(sys_of(p, r, slot));
         if (s < 0) continue;
         SysMeta* M = p.meta + s;
         if (!sys_live(M)) continue;
-        const int c = slot - M->base;
-        if (c >= M->n) continue;
-        const cs3d_params* P = p.prm + s;
+        // parameters and the fields of the metadata that this kernel does not write,
+        // through scalar loads
+        const CMeta MC = cmeta(M);
+        const int sbase = MC->base;
+        const int c = slot - sbase;
+        if (c >= MC->n) continue;
+        const CParams P = cparams(p.prm + s);
         sx[tid] = X; sy[tid] = Y; sz[tid] = Z;
         if (fused) { svx[tid] = vX; svy[tid] = vY; svz[tid] = vZ; }
         if (fused && tid == 0) {  // the search of this step has used the reset list
@@ -853,7 +909,7 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         if (forces) {
             if (node && (isnan(X) || isnan(Y) || isnan(Z))) st |= CS3D_SYS_NAN;  // propagate.cu:241-248
             // ---- springs, propagate.cu:298-314
-            const float stiffness = p.E[slot];
+            const float stiffness = cld(p.E + slot);
             if (node) {
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
@@ -867,9 +923,9 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                     FZ += stiffness * (R - R0) / R0 * dZ / R;
                 }
                 // ---- pressure, propagate.cu:318-330 (centre of mass from the last CenterOfMass)
-                const float rx = X - p.CMX[slot], ry = Y - p.CMY[slot], rz = Z - p.CMZ[slot];
+                const float rx = X - cld(p.CMX + slot), ry = Y - cld(p.CMY + slot), rz = Z - cld(p.CMZ + slot);
                 const float inv = 1.0f / sqrtf(rx * rx + ry * ry + rz * rz);
-                const float g = 3.0f * p.P[slot];
+                const float g = 3.0f * cld(p.P + slot);
                 FX += g * (inv * rx);
                 FY += g * (inv * ry);
                 FZ += g * (inv * rz);
@@ -897,8 +953,9 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         }
         if (search && node) {
             const int nc = nc_pre;
-            const int32_t* cand = p.CAND + (int64_t)slot * kPMax * kSlots + tid;
-            const RuleCtx R = rule_ctx(M, P);
+            const int32_t* cand = p.CAND + (size_t)slot * kPMax * kSlots;  // the cell's lists (uniform)
+            const size_t sb = (size_t)sbase * kSlots;                      // the system's first node
+            const RuleCtx R = rule_ctx(MC, P);
             PairConsts k;
             k.ar = P->attraction_range;
             k.ar2 = k.ar * k.ar;
@@ -908,7 +965,9 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
             k.gext = P->viscotic_damping;
             float cfx = 0.f, cfy = 0.f, cfz = 0.f;
             float fdx = 0.f, fdy = 0.f, fdz = 0.f;
-            int32_t* rec = p.PAIRS + (int64_t)slot * kPMax * kSlots + tid;
+            int32_t* rec = p.PAIRS + (size_t)slot * kPMax * kSlots;
+            const int32_t* sVR = p.VRANGE + (size_t)sbase * 6;
+            const float* sBB = rBB + (size_t)sbase * 6;
             int np;
             if (fused) {
                 const float vix = svx[tid], viy = svy[tid], viz = svz[tid];
@@ -922,19 +981,22 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                 }
                 float nx, ny, nz;
                 node_normal(sx, sy, sz, mesh, tid, nn3[0], nn3[1], nn3[2], nx, ny, nz);
-                np = accept_forces<true, true>(cand, nc, rec, M->base, X, Y, Z, R, p.VRANGE, rBB, vix, viy, viz, nx, ny, nz, k,
-                                               rX, rY, rZ, rVX, rVY, rVZ, cfx, cfy, cfz, fdx, fdy, fdz);
+                np = accept_forces<true, true>(cand, (int)ln, nc, rec, X, Y, Z, R, sVR, sBB, vix, viy, viz, nx, ny, nz, k,
+                                               rX + sb, rY + sb, rZ + sb, rVX + sb, rVY + sb, rVZ + sb, cfx, cfy,
+                                               cfz, fdx, fdy, fdz);
                 fdx = fdx - P->gamma_visc * vix;  // drag, propagate.cu:658
                 fdy = fdy - P->gamma_visc * viy;
                 fdz = fdz - P->gamma_visc * viz;
             } else if (forces) {
-                np = accept_forces<true, false>(cand, nc, rec, M->base, X, Y, Z, R, p.VRANGE, rBB, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, k,
-                                                rX, rY, rZ, rVX, rVY, rVZ, cfx, cfy, cfz, fdx, fdy, fdz);
+                np = accept_forces<true, false>(cand, (int)ln, nc, rec, X, Y, Z, R, sVR, sBB, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, k,
+                                                rX + sb, rY + sb, rZ + sb, rVX + sb, rVY + sb, rVZ + sb, cfx, cfy,
+                                                cfz, fdx, fdy, fdz);
             } else {
-                np = accept_forces<false, false>(cand, nc, rec, M->base, X, Y, Z, R, p.VRANGE, rBB, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
-                                                 k, rX, rY, rZ, rVX, rVY, rVZ, cfx, cfy, cfz, fdx, fdy, fdz);
+                np = accept_forces<false, false>(cand, (int)ln, nc, rec, X, Y, Z, R, sVR, sBB, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
+                                                 k, rX + sb, rY + sb, rZ + sb, rVX + sb, rVY + sb, rVZ + sb, cfx, cfy,
+                                                 cfz, fdx, fdy, fdz);
             }
-            p.NPAIR[gi] = np;
+            stu(p.NPAIR + cb, ln, np);
             if (forces) {
                 FX += cfx; FY += cfy; FZ += cfz;
                 if (P->use_rigid_box) {  // propagate.cu:473-510 (lower wall at 0)
@@ -956,10 +1018,10 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                 float frx = 0.f, fry = 0.f, frz = 0.f;
                 if (P->add_rands) {  // CalculateRanForce for this step (propagate.cu:667-676)
                     float zr[3];
-                    noise3(P->noise_seed, (uint32_t)(c * kSlots + tid), M->step + 1, zr);
+                    noise3(P->noise_seed, (uint32_t)(c * kSlots + tid), MC->step + 1, zr);
                     const float sc = P->rand_scale_factor;
                     frx = sc * zr[0]; fry = sc * zr[1]; frz = sc * zr[2];
-                    p.FRX[gi] = frx; p.FRY[gi] = fry; p.FRZ[gi] = frz;
+                    stu(p.FRX + cb, ln, frx); stu(p.FRY + cb, ln, fry); stu(p.FRZ + cb, ln, frz);
                 }
                 const float vix = svx[tid], viy = svy[tid], viz = svz[tid];
                 // VelocityUpdateA (propagate.cu:721-723) then VelocityUpdateB (:732-734)
@@ -969,10 +1031,10 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                 vx = (float)((double)vx + 0.5 * (double)dt * (double)fdx / (double)m);
                 vy = (float)((double)vy + 0.5 * (double)dt * (double)fdy / (double)m);
                 vz = (float)((double)vz + 0.5 * (double)dt * (double)fdz / (double)m);
-                p.VX[gi] = vx; p.VY[gi] = vy; p.VZ[gi] = vz;
+                stu(p.VX + cb, ln, vx); stu(p.VY + cb, ln, vy); stu(p.VZ + cb, ln, vz);
             }
         }
-        if (node && forces) { p.FCX[gi] = FX; p.FCY[gi] = FY; p.FCZ[gi] = FZ; }
+        if (node && forces) { stu(p.FCX + cb, ln, FX); stu(p.FCY + cb, ln, FY); stu(p.FCZ + cb, ln, FZ); }
         if (st) flag(M, st);
         __syncthreads();
     }
@@ -1116,18 +1178,19 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
     // a multi-system step walks the compact list of live slots (k_live)
     const int nit = kLive ? *p.n_live : r.slot_hi - r.slot_lo;
     for (int it = xcd_block(); it < nit; it += gridDim.x) {
-        const int slot = kLive ? p.LIVE[it] : r.slot_lo + it;
+        const int slot = cell_slot<kLive>(p, r, it);
         // node data first (every slot of the range is allocated): their latency
         // overlaps the metadata reads
-        const int64_t gi = (int64_t)slot * kSlots + tid;
-        const float X = rX[gi], Y = rY[gi], Z = rZ[gi];
+        const size_t cb = (size_t)slot * kSlots;  // the cell's first node (uniform)
+        const uint32_t ln = lane_index();
+        const float X = ldu(rX + cb, ln), Y = ldu(rY + cb, ln), Z = ldu(rZ + cb, ln);
         float vX = 0.f, vY = 0.f, vZ = 0.f;
-        if (do_dis) { vX = rVX[gi]; vY = rVY[gi]; vZ = rVZ[gi]; }
-        const int np = (do_dis && node) ? p.NPAIR[gi] : 0;
+        if (do_dis) { vX = ldu(rVX + cb, ln); vY = ldu(rVY + cb, ln); vZ = ldu(rVZ + cb, ln); }
+        const int np = (do_dis && node) ? ldu(p.NPAIR + cb, ln) : 0;
         float fcX = 0.f, fcY = 0.f, fcZ = 0.f, xbX = 0.f, xbY = 0.f, xbZ = 0.f;
         if (fused && do_dis && node) {
-            fcX = p.FCX[gi]; fcY = p.FCY[gi]; fcZ = p.FCZ[gi];
-            xbX = p.XBX[gi]; xbY = p.XBY[gi]; xbZ = p.XBZ[gi];
+            fcX = ldu(p.FCX + cb, ln); fcY = ldu(p.FCY + cb, ln); fcZ = ldu(p.FCZ + cb, ln);
+            xbX = ldu(p.XBX + cb, ln); xbY = ldu(p.XBY + cb, ln); xbZ = ldu(p.XBZ + cb, ln);
         }
         const int s = sys_of(p, r, slot);
         if (s < 0) continue;
@@ -1164,21 +1227,22 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
             k.ka = 0.f;
             k.kr = 0.f;
             k.gext = P->viscotic_damping;
-            const int32_t* lst = p.PAIRS + (int64_t)slot * kPMax * kSlots + tid;
-            pair_friction(lst, M->base, np, vix, viy, viz, nx, ny, nz, k.gext, rVX, rVY, rVZ, fx, fy, fz);
+            const int32_t* lst = p.PAIRS + (size_t)slot * kPMax * kSlots;
+            const size_t sb = (size_t)M->base * kSlots;  // the system's first node
+            pair_friction(lst, (int)ln, np, vix, viy, viz, nx, ny, nz, k.gext, rVX + sb, rVY + sb, rVZ + sb, fx, fy, fz);
             fx = fx - P->gamma_visc * vix;  // drag, propagate.cu:658
             fy = fy - P->gamma_visc * viy;
             fz = fz - P->gamma_visc * viz;
-            p.FDX[gi] = fx; p.FDY[gi] = fy; p.FDZ[gi] = fz;
+            stu(p.FDX + cb, ln, fx); stu(p.FDY + cb, ln, fy); stu(p.FDZ + cb, ln, fz);
             if (fused) {
                 // the next step's Integrate moves this node to X + v' dt with v' from the
                 // values now in V, F_con and F_dis (bit-identical to k_integrate): search
                 // again next step when that is pair_skin/2 or more from the last search
                 const float dt = P->dt, m = P->mass, rdt = sqrtf(dt);
                 const bool rnd = P->add_rands != 0;
-                const float qx = X + half_kick(vix, fcX, fx, rnd ? p.FRX[gi] : 0.f, dt, rdt, m) * dt - xbX;
-                const float qy = Y + half_kick(viy, fcY, fy, rnd ? p.FRY[gi] : 0.f, dt, rdt, m) * dt - xbY;
-                const float qz = Z + half_kick(viz, fcZ, fz, rnd ? p.FRZ[gi] : 0.f, dt, rdt, m) * dt - xbZ;
+                const float qx = X + half_kick(vix, fcX, fx, rnd ? ldu(p.FRX + cb, ln) : 0.f, dt, rdt, m) * dt - xbX;
+                const float qy = Y + half_kick(viy, fcY, fy, rnd ? ldu(p.FRY + cb, ln) : 0.f, dt, rdt, m) * dt - xbY;
+                const float qz = Z + half_kick(viz, fcZ, fz, rnd ? ldu(p.FRZ + cb, ln) : 0.f, dt, rdt, m) * dt - xbZ;
                 const float h = 0.5f * P->pair_skin;
                 far = !(qx * qx + qy * qy + qz * qz <= h * h);
             }
@@ -1417,7 +1481,9 @@ __device__ __forceinline__ void block_minmax12(float* v, float (*part)[12], floa
     __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void k_sys_divide(Pool p, Range r, int32_t mode) {
+// Division phase of system s by one 256-lane workgroup (every lane calls it; the
+// early returns depend on the system only, so the workgroup stays together).
+__device__ void divide_system(const Pool& p, int s, int32_t mode) {
     __shared__ int32_t scan[256];
     __shared__ int32_t nev, ncur, stop;
     __shared__ float nrm[3];
@@ -1427,8 +1493,6 @@ __global__ __launch_bounds__(256) void k_sys_divide(Pool p, Range r, int32_t mod
     __shared__ float mres[12];
     __shared__ int64_t nclear;
     const int tid = threadIdx.x;
-    const int s = r.sys_lo + blockIdx.x;
-    if (s >= r.sys_hi) return;
     SysMeta* M = p.meta + s;
     const bool step_mode = mode == M_STEP;
     // a stopped system keeps a pending grid adoption: its bboxes are adopted when it
@@ -1618,6 +1682,7 @@ __global__ __launch_bounds__(256) void k_sys_divide(Pool p, Range r, int32_t mod
         if (step_mode) {
             M->rebuild = rebuild ? 1 : 0;
             M->step = stepno;
+            M->cell_steps += n0;  // the n0 cells that were stepped
             // After divisions, the divided cells and the cells listed by k_dis_post (a
             // node leaves the skin next step) are searched again next step, with the
             // cells near them (partial search).  Without divisions a node leaving the
@@ -1647,33 +1712,74 @@ __global__ __launch_bounds__(256) void k_sys_divide(Pool p, Range r, int32_t mod
             M->need_search = 1;
         }
     }
+    __syncthreads();  // the shared state is reused by the workgroup's next system
+}
+
+// One workgroup per system, striding over the systems of the range in use.
+__global__ __launch_bounds__(256) void k_sys_divide(Pool p, Range r, int32_t mode) {
+    const int hi = r.one_sys ? r.sys_hi : min(r.sys_hi, *p.nsys);
+    for (int s = r.sys_lo + blockIdx.x; s < hi; s += gridDim.x) divide_system(p, s, mode);
 }
 
 // Compact list of the live cell slots of a multi-system range (cells c < n of the
 // systems that step), rebuilt at the start of every step: the step's cell kernels
-// walk it instead of every slot of each system's room.  One workgroup per system:
-// system s writes its slots from the offset given by the live cells of the systems
-// before it; the last workgroup writes the count.
+// walk it instead of every slot of each system's room.  Every workgroup computes
+// the offsets of all systems in use (an exclusive scan of their live counts, 1024
+// at a time) and writes the slots of the systems s with s mod gridDim.x =
+// blockIdx.x, all lanes on one system at a time; workgroup 0 writes the count.
 __device__ __forceinline__ int live_cells(const SysMeta* m) { return sys_live(m) ? m->n : 0; }
 
-__global__ __launch_bounds__(256) void k_live(Pool p, Range r) {
-    __shared__ int32_t red[256];
-    const int tid = threadIdx.x;
-    const int s = r.sys_lo + blockIdx.x;
-    const int hi = s < r.sys_hi ? s : r.sys_hi;
-    int acc = 0;
-    for (int j = r.sys_lo + tid; j < hi; j += 256) acc += live_cells(p.meta + j);
-    red[tid] = acc;
+constexpr int kLiveThreads = 1024;
+
+__global__ __launch_bounds__(kLiveThreads) void k_live(Pool p, Range r) {
+    __shared__ int32_t wsum[kLiveThreads / 64];
+    __shared__ int32_t offs[kLiveThreads];
+    __shared__ int32_t carry;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int lo = r.sys_lo, hi = min(r.sys_hi, *p.nsys);
+    if (tid == 0) carry = 0;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (tid < o) red[tid] += red[tid + o];
+    for (int c0 = lo; c0 < hi; c0 += kLiveThreads) {
+        const int s = c0 + tid;
+        const int n = s < hi ? live_cells(p.meta + s) : 0;
+        // inclusive scan within the wave, then over the wave totals
+        int v = n;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(v, o);
+            if (lane >= o) v += t;
+        }
+        if (lane == 63) wsum[wv] = v;
+        __syncthreads();
+        if (tid == 0) {
+            int acc = carry;
+            for (int w = 0; w < kLiveThreads / 64; ++w) {
+                const int t = wsum[w];
+                wsum[w] = acc;
+                acc += t;
+            }
+            carry = acc;
+        }
+        __syncthreads();
+        offs[tid] = wsum[wv] + v - n;  // exclusive offset of system s
+        __syncthreads();
+        const int m = min(kLiveThreads, hi - c0);
+        for (int k = 0; k < m; ++k) {
+            const int sk = c0 + k;
+            if (sk % (int)gridDim.x != (int)blockIdx.x) continue;
+            const SysMeta* M = p.meta + sk;
+            const int nk = live_cells(M);
+            const int off = offs[k], base = M->base;
+            for (int c = tid; c < nk; c += kLiveThreads) p.LIVE[off + c] = base + c;
+        }
         __syncthreads();
     }
-    const int off = red[0];
-    const int n = s < r.sys_hi ? live_cells(p.meta + s) : 0;
-    const int base = s < r.sys_hi ? p.meta[s].base : 0;
-    for (int c = tid; c < n; c += 256) p.LIVE[off + c] = base + c;
-    if (blockIdx.x == gridDim.x - 1 && tid == 0) *p.n_live = off + n;
+    if (blockIdx.x == 0 && tid == 0) *p.n_live = carry;
+}
+
+// Sets the count of system ids in use (stream-ordered with the steps).
+__global__ void k_set_nsys(Pool p, int32_t n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p.nsys = n;
 }
 
 // CorrectCoMMotion (propagate.cu:766-776) for the systems whose division phase
@@ -2007,10 +2113,13 @@ static int grid_cells(const Range& r, int per_cu) {
     return n < 1 ? 1 : (n < cap ? n : cap);
 }
 
+// workgroups of k_live and of the division phase over several systems
+constexpr int kSysBlocks = 256;
+
 void launch_live(const Pool& p, const Range& r, hipStream_t st) {
-    const int nsys = r.sys_hi - r.sys_lo;
-    launch(k_live, dim3(nsys > 0 ? nsys : 1), dim3(256), st, p, r);
+    launch(k_live, dim3(kSysBlocks / 4), dim3(kLiveThreads), st, p, r);
 }
+void launch_set_nsys(const Pool& p, int n, hipStream_t st) { launch(k_set_nsys, dim3(1), dim3(64), st, p, n); }
 
 void launch_integrate(const Pool& p, const Range& r, int flags, int64_t step_override, hipStream_t st) {
     if (r.live)
@@ -2054,7 +2163,7 @@ void launch_dis(const Pool& p, const Range& r, int mode, int64_t step_override, 
 }
 void launch_divide(const Pool& p, const Range& r, int mode, hipStream_t st) {
     const int ns = r.sys_hi - r.sys_lo;
-    launch(k_sys_divide, dim3(ns > 0 ? ns : 1), dim3(256), st, p, r, mode);
+    launch(k_sys_divide, dim3(ns < 1 ? 1 : (ns < kSysBlocks ? ns : kSysBlocks)), dim3(256), st, p, r, mode);
 }
 void launch_com_shift(const Pool& p, const Range& r, hipStream_t st) {
     launch(k_com_shift, dim3(grid_cells(r, p.wg_per_cu)), dim3(kSlots), st, p, r);

@@ -910,11 +910,16 @@ int main(int argc, char** argv) {
     s.p.correct_com = s.correct_com ? 1 : 0;
     if (check_walls(s, n_cells, Z) != 0) return -1;
 
-    // --dump-setup: the parameter block and mesh tables exactly as handed to the engine
-    // (cs3d_params and cs3d_mesh bytes), for tools that re-run the same system
+    // --dump-setup: the parameter block, mesh tables and initial positions exactly as
+    // handed to the engine (cs3d_params and cs3d_mesh bytes; x, y and z of the placed
+    // cells as float32, n_cells * 192 each), for tools that re-run the same system
+    const size_t pos_bytes = (size_t)n_cells * S * sizeof(float);
     if (!dump_setup.empty() && (!write_bytes(dump_setup + ".params", &s.p, sizeof(s.p)) ||
-                                !write_bytes(dump_setup + ".mesh", &mesh, sizeof(mesh)))) {
-        std::printf("ERROR: cannot write %s.params / .mesh\n", dump_setup.c_str());
+                                !write_bytes(dump_setup + ".mesh", &mesh, sizeof(mesh)) ||
+                                !write_bytes(dump_setup + ".x", X.data(), pos_bytes) ||
+                                !write_bytes(dump_setup + ".y", Y.data(), pos_bytes) ||
+                                !write_bytes(dump_setup + ".z", Z.data(), pos_bytes))) {
+        std::printf("ERROR: cannot write %s.params / .mesh / .x / .y / .z\n", dump_setup.c_str());
         return -1;
     }
 

@@ -43,6 +43,7 @@ SIGNATURES = {
                         st_t),
     "cs3d_system_add_raw": ([vp, PP, C.c_int32, C.c_int32, C.c_int64, C.POINTER(C.c_int32)], st_t),
     "cs3d_system_remove": ([vp, C.c_int32], st_t),
+    "cs3d_system_grow": ([vp, C.c_int32, C.c_int32, C.c_int64], st_t),
     "cs3d_system_startup": ([vp, C.c_int32], st_t),
     "cs3d_system_set_horizon": ([vp, C.c_int32, C.c_int64], st_t),
     "cs3d_system_info": ([vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_uint32)], st_t),
@@ -57,6 +58,8 @@ SIGNATURES = {
     "cs3d_set_graph_steps": ([vp, C.c_int32], st_t),
     "cs3d_set_launch": ([vp, C.c_int32], st_t),
     "cs3d_systems_poll": ([vp, C.c_int32, C.c_int32, vp, vp, vp], st_t),
+    "cs3d_systems_work": ([vp, C.c_int32, C.c_int32, vp], st_t),
+    "cs3d_graph_prepare": ([vp, C.c_int32], st_t),
     "cs3d_step_timed": ([vp, C.c_int32, C.c_int32, vp], st_t),
     "cs3d_time_kernel": ([vp, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double)], st_t),
     "cs3d_k_integrate": ([vp, C.c_int32], st_t),
@@ -224,6 +227,17 @@ class Context:
                                       st.ctypes.data_as(vp)), "cs3d_systems_poll")
         return n, s, st
 
+    def work(self, first, count):
+        """Cell-steps (node-updates / 180) done by systems first .. first+count-1."""
+        w = np.zeros(count, dtype=np.int64)
+        check(lib().cs3d_systems_work(self.h, int(first), int(count), w.ctypes.data_as(vp)), "cs3d_systems_work")
+        return w
+
+    def prepare_graph(self, target=abi.ALL):
+        """Captures and uploads the graph that step() replays for target, without stepping."""
+        sid = target.id if isinstance(target, System) else target
+        check(lib().cs3d_graph_prepare(self.h, sid), "cs3d_graph_prepare")
+
     def copy_calibrate(self, nbytes, reps=10):
         """Average ms of a plain copy kernel over nbytes (reads nbytes, writes nbytes)."""
         out = C.c_double()
@@ -257,6 +271,11 @@ class System:
     def remove(self):
         check(lib().cs3d_system_remove(self.ctx.h, self.id), "cs3d_system_remove")
         self.ctx.systems.pop(self.id, None)
+
+    def grow(self, max_cells, voxels=0):
+        """Moves the system to a room of max_cells cells (results unchanged)."""
+        check(lib().cs3d_system_grow(self.ctx.h, self.id, int(max_cells), int(voxels)), "cs3d_system_grow")
+        self.max_cells = int(max_cells)
 
     def info(self):
         n, st, status = C.c_int32(), C.c_int64(), C.c_uint32()

@@ -30,10 +30,9 @@ NODE_BYTES = 4 * 192  # one float per slot
 
 def build(n_tissues=256, seed=1, nmin=1, nmax=512, steps=5000, grid_interval=10, phi=0.30, room=2):
     """Seeded ensemble: (device mesh, tissues).  Each tissue dict has params,
-    X, Y, Z, n, max_cells = room * n + 2.  room = 2 lets every cell divide once,
-    enough for 1000 steps; a tissue of a few free cells doubles about every 1000
-    steps, so longer runs need more room (a tissue that reaches max_cells stops
-    with the capacity status)."""
+    X, Y, Z, n, max_cells = room * n + 2.  room = 2 lets every cell divide once;
+    the runners below move a tissue to a larger room (cs3d_system_grow) whenever
+    its cells could otherwise not all divide once more."""
     m, tissues = configs.ensemble(n_tissues=n_tissues, seed=seed, nmin=nmin, nmax=nmax, phi=phi, steps=steps)
     gm = engine.build_mesh(meshmod.tables(m), configs.rest_positions(m))
     for t in tissues:
@@ -43,62 +42,113 @@ def build(n_tissues=256, seed=1, nmin=1, nmax=512, steps=5000, grid_interval=10,
 
 
 def node_updates(tissues, steps):
+    """Node-updates of the initial cell counts (a lower bound for growing tissues)."""
     return float(sum(t["n"] for t in tissues)) * 180.0 * steps
 
 
-def _context(gm, tissues):
-    cap = sum(t["max_cells"] for t in tissues)
-    vox = sum(max(4096, 32 * t["max_cells"]) * (1 + t["params"].voxel_capacity) for t in tissues)
-    bins = sum(min(max(16384, 640 * t["max_cells"]), 1 << 40) for t in tissues) * abi.BIN_WORDS
+def grow_rooms(ctx, systems, headroom=2.5):
+    """Moves every system whose live cells could not all divide once more to a
+    room of headroom * n + 2 cells.  A tissue's cells at most double between two
+    calls when the calls are closer together than a cell's regrowth after
+    division (about 1000 steps at the default growth rate).  Returns the number
+    of systems moved."""
+    if not systems:
+        return 0
+    hi = max(s.id for s in systems) + 1
+    n, _, _ = ctx.poll(0, hi)
+    moved = 0
+    for s in systems:
+        k = int(n[s.id])
+        if 2 * k + 2 > s.max_cells:
+            s.grow(int(headroom * k) + 2)
+            moved += 1
+    return moved
+
+
+def run_chunks(ctx, target, steps, systems, chunk=100):
+    """steps steps of target in chunks, with rooms grown between chunks.
+    Returns the number of room moves."""
+    moved = 0
+    done = 0
+    while done < steps:
+        k = min(chunk, steps - done)
+        ctx.step(k, target)
+        done += k
+        if done < steps:
+            moved += grow_rooms(ctx, systems)
+    return moved
+
+
+def _context(gm, tissues, room=1.0):
+    """A context for the tissues with room x their cell slots (growth headroom)."""
+    cap = int(room * sum(t["max_cells"] for t in tissues))
+    vox = int(room * sum(max(4096, 32 * t["max_cells"]) * (1 + t["params"].voxel_capacity) for t in tissues))
+    bins = int(room * sum(min(max(16384, 640 * t["max_cells"]), 1 << 40) for t in tissues)) * abi.BIN_WORDS
     return engine.Context(gm, capacity_cells=cap, voxel_budget=vox, bin_budget=bins)
 
 
-def run_batched(gm, tissues, steps, wg_per_cu=None, graph_steps=None, warmup=0, ctx=None):
-    """All tissues in one context, stepped together.  Returns (seconds, ctx, systems)."""
+def run_batched(gm, tissues, steps, wg_per_cu=None, graph_steps=None, ctx=None, chunk=100, room=4.0):
+    """All tissues in one context, stepped together through the all-systems graph
+    (captured before the timed region), in chunks of `chunk` steps with rooms
+    grown between chunks.  Returns (seconds, ctx, systems, info) with info =
+    {"cell_steps", "moves", "final_cells"}."""
     own = ctx is None
     if own:
-        ctx = _context(gm, tissues)
+        ctx = _context(gm, tissues, room)
     if wg_per_cu:
         ctx.set_launch(wg_per_cu)
     if graph_steps:
         ctx.set_graph_steps(graph_steps)
     systems = [ctx.add_system(t["params"], t["X"], t["Y"], t["Z"], t["n"], max_cells=t["max_cells"]) for t in tissues]
-    if warmup:
-        ctx.step(warmup)
+    ctx.prepare_graph()
     ctx.sync()
     t0 = time.perf_counter()
-    ctx.step(steps)
+    moves = run_chunks(ctx, abi.ALL, steps, systems, chunk)
     ctx.sync()
     dt = time.perf_counter() - t0
+    hi = max(s.id for s in systems) + 1
+    n, _, status = ctx.poll(0, hi)
     for s in systems:
-        st = s.info()[2]
-        if st & abi.SYS_FATAL_MASK:
-            raise RuntimeError("system %d status 0x%x" % (s.id, st))
-    return dt, ctx, systems
+        if status[s.id] & abi.SYS_FATAL_MASK:
+            raise RuntimeError("system %d status 0x%x" % (s.id, status[s.id]))
+    work = ctx.work(0, hi)
+    info = {"cell_steps": int(sum(int(work[s.id]) for s in systems)), "moves": moves,
+            "final_cells": int(sum(int(n[s.id]) for s in systems))}
+    return dt, ctx, systems, info
 
 
-def run_plain(gm, tissues, steps, sample_steps=None, warmup=10):
-    """The tissues one after another, each alone on the drop-in path.  With
-    sample_steps, each tissue runs that many timed steps and its time is scaled
-    to `steps` (stated as extrapolated by the caller).  Returns seconds."""
-    cap = max(t["max_cells"] for t in tissues)
-    big = max(tissues, key=lambda t: t["max_cells"])
-    ctx = _context(gm, [big])
+def sample_tissues(tissues, every):
+    """Every `every`-th tissue in order of size (a sample stratified by size)."""
+    order = sorted(range(len(tissues)), key=lambda i: (tissues[i]["n"], i))
+    return [order[k] for k in range(every // 2, len(order), every)]
+
+
+def run_plain(gm, tissues, steps, indices=None, chunk=100, room=16.0):
+    """The tissues one after another, each alone on the drop-in path (its own
+    graph), for the full `steps`, with the same room growth as the batched run.
+    indices selects a sample (all by default).  Returns (seconds, cell_steps) over
+    the tissues run."""
+    idx = list(range(len(tissues))) if indices is None else list(indices)
+    big = max((tissues[i] for i in idx), key=lambda t: t["max_cells"])
+    ctx = _context(gm, [big], room)
     total = 0.0
-    k = steps if sample_steps is None else min(steps, sample_steps)
-    for t in tissues:
+    cell_steps = 0
+    for i in idx:
+        t = tissues[i]
         s = ctx.add_system(t["params"], t["X"], t["Y"], t["Z"], t["n"], max_cells=t["max_cells"])
-        if warmup:
-            s.step(warmup)
+        ctx.prepare_graph(s)
         ctx.sync()
         t0 = time.perf_counter()
-        s.step(k)
+        run_chunks(ctx, s.id, steps, [s], chunk)
         ctx.sync()
-        total += (time.perf_counter() - t0) * (steps / k)
+        total += time.perf_counter() - t0
+        st = s.info()[2]
+        if st & abi.SYS_FATAL_MASK:
+            raise RuntimeError("tissue %d status 0x%x on the plain path" % (i, st))
+        cell_steps += int(ctx.work(s.id, 1)[0])
         s.remove()
     ctx.close()
-    del cap
-    return total
+    return total, cell_steps
 
 
 def autotune(gm, tissues, probe_steps=100, wg_options=(4, 8, 16, 32, 64), graph_options=(10, 25, 50), log=None):
@@ -112,7 +162,8 @@ def autotune(gm, tissues, probe_steps=100, wg_options=(4, 8, 16, 32, 64), graph_
         for gs in graph_options:
             ctx.set_launch(wg)
             ctx.set_graph_steps(gs)
-            ctx.step(gs)  # capture + warm
+            ctx.prepare_graph()
+            ctx.step(gs)  # warm
             ctx.sync()
             k = max(gs, (probe_steps // gs) * gs)
             t0 = time.perf_counter()
@@ -132,46 +183,66 @@ def autotune(gm, tissues, probe_steps=100, wg_options=(4, 8, 16, 32, 64), graph_
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--tissues", type=int, default=256)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--room", type=int, default=2,
-                    help="cell slots per initial cell of each tissue (2: every cell divides once)")
+    ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--nmax", type=int, default=512)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--grid-interval", type=int, default=10)
-    ap.add_argument("--plain-sample-steps", type=int, default=200,
-                    help="timed steps per tissue on the plain path (0 = full run)")
+    ap.add_argument("--plain-every", type=int, default=8,
+                    help="the plain path runs every k-th tissue by size for the full length (1 = all)")
     ap.add_argument("--no-tune", action="store_true")
     args = ap.parse_args(argv)
 
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
-    gm, tissues = build(args.tissues, args.seed, 1, args.nmax, args.steps, args.grid_interval, room=args.room)
+    gm, tissues = build(args.tissues, args.seed, 1, args.nmax, args.steps, args.grid_interval)
     cells = sum(t["n"] for t in tissues)
     log("ensemble: %d tissues, %d cells, %d steps" % (len(tissues), cells, args.steps))
     wg, gs, table = (16, 10, [])
     if not args.no_tune:
         wg, gs, table = autotune(gm, tissues, log=log)
-    dt_b, ctx, _ = run_batched(gm, tissues, args.steps, wg_per_cu=wg, graph_steps=gs, warmup=gs)
-    ctx.close()
-    nu = node_updates(tissues, args.steps)
-    log("batched: %.3f s (%.4g node-updates/s)" % (dt_b, nu / dt_b))
-    sample = args.plain_sample_steps or None
-    dt_p = run_plain(gm, tissues, args.steps, sample_steps=sample)
-    log("plain: %.3f s%s" % (dt_p, " (extrapolated from %d steps per tissue)" % sample if sample else ""))
-    out = {
-        "workload": "%d independent tissues, N ~ log-U[1, %d] (%d cells), stiffness, growth rate and adhesion "
-                    "x U[0.7, 1.3], %d steps, FP32" % (len(tissues), args.nmax, cells, args.steps),
-        "batched": {"seconds": dt_b, "sims_per_hour": len(tissues) / dt_b * 3600.0,
-                    "node_updates_per_s": nu / dt_b, "wg_per_cu": wg, "graph_steps": gs},
-        "plain": {"seconds": dt_p, "sims_per_hour": len(tissues) / dt_p * 3600.0,
-                  "node_updates_per_s": nu / dt_p,
-                  "extrapolated_from_steps_per_tissue": sample},
-        "batched_over_plain": dt_p / dt_b,
-        "autotune": table,
-    }
+    out = measure(gm, tissues, args.steps, wg, gs, args.plain_every, log)
+    out["autotune"] = table
     print(json.dumps(out), flush=True)
     return out
+
+
+def measure(gm, tissues, steps, wg, gs, plain_every=8, log=None):
+    """Batched run of every tissue and plain runs of a size-stratified sample,
+    both for the full length with divisions; node-updates from the cells actually
+    stepped (counted on the device)."""
+    dt_b, ctx, _, info = run_batched(gm, tissues, steps, wg_per_cu=wg, graph_steps=gs)
+    ctx.close()
+    nu = 180.0 * info["cell_steps"]
+    if log:
+        log("ensemble batched: %.3f s, %d -> %d cells, %d room moves, %.4g node-updates/s" %
+            (dt_b, sum(t["n"] for t in tissues), info["final_cells"], info["moves"], nu / dt_b))
+    idx = sample_tissues(tissues, plain_every) if plain_every > 1 else None
+    dt_s, cs_s = run_plain(gm, tissues, steps, idx)
+    ntis = len(idx) if idx is not None else len(tissues)
+    # the sample is one tissue in plain_every by size: its time scales by that factor
+    scale = len(tissues) / float(ntis)
+    dt_p = dt_s * scale
+    if log:
+        log("ensemble plain: %d tissues in %.3f s (x%.2f -> %.3f s)" % (ntis, dt_s, scale, dt_p))
+    return {
+        "workload": "%d independent tissues, N ~ log-U[1, 512] (%d cells at the start, %d at the end), stiffness, "
+                    "growth rate and adhesion x U[0.7, 1.3], %d steps with growth and division, FP32" % (
+                        len(tissues), sum(t["n"] for t in tissues), info["final_cells"], steps),
+        "batched_sims_per_hour": round(len(tissues) / dt_b * 3600.0, 1),
+        "batched_node_updates_per_s": nu / dt_b,
+        "batched_seconds": round(dt_b, 4),
+        "room_moves": info["moves"],
+        "plain_sims_per_hour": round(len(tissues) / dt_p * 3600.0, 1),
+        "plain_node_updates_per_s": nu / dt_p,
+        "plain_seconds": round(dt_p, 4),
+        "plain_sample": "%d of %d tissues (every %d-th by size), each for the full %d steps with divisions, "
+                        "%.3f s, time scaled by %.2f" % (ntis, len(tissues), plain_every, steps, dt_s, scale),
+        "node_updates": nu,
+        "node_updates_counted": "cells stepped in every step, counted on the device (growing tissues)",
+        "batched_over_plain": round(dt_p / dt_b, 2),
+        "autotuned": {"wg_per_cu": wg, "graph_steps": gs},
+    }
 
 
 if __name__ == "__main__":

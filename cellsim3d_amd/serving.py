@@ -83,6 +83,8 @@ class Server:
         if max_cells_total:
             cap = min(cap, int(max_cells_total))
         self.capacity = cap
+        self.budget = hbm_budget_bytes
+        self.peak_jobs = 0
         self.ctx = engine.Context(gm, capacity_cells=cap,
                                   voxel_budget=cap * VOX_WORDS_PER_CELL + SMALL_SYSTEMS * VOX_WORDS_MIN,
                                   bin_budget=cap * BIN_WORDS_PER_CELL + SMALL_SYSTEMS * BIN_WORDS_MIN)
@@ -137,6 +139,7 @@ class Server:
                 if qi < len(queue):
                     time.sleep(max(0.0, queue[qi].arrival * time_scale - (time.perf_counter() - t0)))
                 continue
+            self.peak_jobs = max(self.peak_jobs, len(active))
             self.ctx.step(self.chunk)
             hi = max(active) + 1
             ncell, steps, status = self.ctx.poll(0, hi)
@@ -154,15 +157,44 @@ class Server:
         return time.perf_counter() - t0
 
 
+def summary(jobs, srv, wall):
+    """p50/p99 completion latency (result on the host minus arrival), aggregate
+    node-updates/s and the utilisation rho = offered node-updates/s over served
+    node-updates/s (above 1 the requests queue, as SURVEY-size streams do)."""
+    lat = np.array([j.finished - j.arrival for j in jobs])
+    span = max(j.arrival for j in jobs) if jobs else 0.0
+    offered = sum(j.n * 180.0 * j.horizon for j in jobs)
+    served = srv.node_updates / wall
+    waited = [j.admitted - j.arrival for j in jobs]
+    return {
+        "workload": "%d requests, Poisson arrivals over %.1f s, N ~ log-U[%d, %d] (mean %.0f), horizon ~ "
+                    "log-U[%.0e, %.0e] steps (mean %.0f), FP32, growth off, admitted against a %.0f GB HBM budget "
+                    "(pool of %d cells)" % (
+                        len(jobs), span, min(j.n for j in jobs), max(j.n for j in jobs),
+                        float(np.mean([j.n for j in jobs])), min(j.horizon for j in jobs),
+                        max(j.horizon for j in jobs), float(np.mean([j.horizon for j in jobs])),
+                        srv.budget / 1e9, srv.capacity),
+        "requests": len(jobs),
+        "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+        "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+        "p50_queue_wait_s": round(float(np.percentile(waited, 50)), 4),
+        "wall_s": round(wall, 3),
+        "node_updates_per_s": served,
+        "offered_node_updates_per_s": offered / span if span > 0 else None,
+        "rho": round((offered / span) / served, 3) if span > 0 and served > 0 else None,
+        "peak_resident_jobs": srv.peak_jobs,
+    }
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--requests", type=int, default=64)
+    ap.add_argument("--requests", type=int, default=256)
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--nmax", type=int, default=4096)
     ap.add_argument("--hmin", type=float, default=1e3)
     ap.add_argument("--hmax", type=float, default=1e5)
     ap.add_argument("--budget-gb", type=float, default=200.0)
-    ap.add_argument("--max-cells", type=int, default=200000, help="cap of the device pool in cells")
+    ap.add_argument("--max-cells", type=int, default=0, help="cap of the device pool in cells (0: the budget decides)")
     ap.add_argument("--chunk", type=int, default=50)
     ap.add_argument("--seed", type=int, default=1)
     args = ap.parse_args(argv)
@@ -174,25 +206,12 @@ def main(argv=None):
     gm = engine.build_mesh(meshmod.tables(m), configs.rest_positions(m))
     reqs = configs.serving_requests(args.requests, args.seed, args.rate, 8, args.nmax, args.hmin, args.hmax)
     jobs = make_jobs(reqs)
-    offered = sum(j.n * 180.0 * j.horizon for j in jobs)
-    srv = Server(gm, args.budget_gb * 1e9, max_cells_total=args.max_cells, chunk_steps=args.chunk)
+    srv = Server(gm, args.budget_gb * 1e9, max_cells_total=args.max_cells or None, chunk_steps=args.chunk)
     log("serving: %d requests, %d cells in total, pool of %d cells" % (len(jobs), sum(j.n for j in jobs),
                                                                       srv.capacity))
     wall = srv.run(jobs, log=log)
     srv.close()
-    lat = np.array([j.finished - j.arrival for j in jobs])
-    span = jobs[-1].arrival if jobs else 0.0
-    out = {
-        "workload": "%d requests, Poisson %.0f/s, N ~ log-U[8, %d], horizon ~ log-U[%.0e, %.0e] steps, "
-                    "FP32, growth off" % (
-            len(jobs), args.rate, args.nmax, args.hmin, args.hmax),
-        "p50_latency_s": float(np.percentile(lat, 50)),
-        "p99_latency_s": float(np.percentile(lat, 99)),
-        "wall_s": wall,
-        "node_updates_per_s": srv.node_updates / wall,
-        "offered_node_updates_per_s": offered / span if span > 0 else None,
-        "offered_load": (offered / span) / (srv.node_updates / wall) if span > 0 else None,
-    }
+    out = summary(jobs, srv, wall)
     print(json.dumps(out), flush=True)
     return out
 

@@ -223,6 +223,14 @@ cs3d_status cs3d_system_add(cs3d_ctx* ctx, const cs3d_params* p, int32_t n_cells
 cs3d_status cs3d_system_add_raw(cs3d_ctx* ctx, const cs3d_params* p, int32_t n_cells, int32_t max_cells,
                                 int64_t voxels, cs3d_sys* out);
 cs3d_status cs3d_system_remove(cs3d_ctx* ctx, cs3d_sys sys);
+/* Moves a system to a larger room of max_cells cell slots (voxels: its voxel
+ * budget, 0 = auto, never below the current one) and frees the old slots.  Cell
+ * indices, results and the system id are unchanged.  For batched ensembles and
+ * served jobs whose tissues outgrow their room between steps; the reference
+ * sizes every array for MaxNoofC180s up front (GPUbounce.cu:358-415) and has no
+ * counterpart.  CAPACITY when the pools cannot hold the new room (the system is
+ * then left as it was).  Synchronous. */
+cs3d_status cs3d_system_grow(cs3d_ctx* ctx, cs3d_sys sys, int32_t max_cells, int64_t voxels);
 /* Runs the start-up sequence of cs3d_system_add again on the system's current
  * state: CenterOfMass, bounding boxes, grid, initial CalculateConForce and
  * CalculateDisForce (GPUbounce.cu:739-763, 978-1004).  For callers that change
@@ -277,6 +285,16 @@ cs3d_status cs3d_set_launch(cs3d_ctx* ctx, int32_t wg_per_cu);
  * no counterpart in the reference. */
 cs3d_status cs3d_systems_poll(cs3d_ctx* ctx, int32_t first, int32_t count, int32_t* n_cells, int64_t* steps,
                               uint32_t* status);
+/* Work done by systems first .. first+count-1: for each, the sum over its
+ * completed steps of the cells it stepped (node-updates / 180; -1 for unused
+ * ids), counted on the device at the end of every step, so growing systems are
+ * counted with their actual cell counts.  No counterpart in the reference. */
+cs3d_status cs3d_systems_work(cs3d_ctx* ctx, int32_t first, int32_t count, int64_t* cell_steps);
+/* Captures, instantiates and uploads the HIP graph that cs3d_step replays for
+ * `sys` (or CS3D_ALL) without running it, so that a timed region holds replays
+ * only.  The all-systems graph reads the systems in use on the device and stays
+ * valid when systems are added or removed.  No counterpart in the reference. */
+cs3d_status cs3d_graph_prepare(cs3d_ctx* ctx, cs3d_sys sys);
 
 /* Phases of one step, in launch order (one kernel launch each; the division phase adds
  * CorrectCoMMotion for systems with correct_com). */

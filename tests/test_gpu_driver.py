@@ -178,3 +178,95 @@ def test_soft_cells_at_setup_enter_the_start_up_forces(tmp_path):
     assert not all(np.array_equal(drv.get(f)[:n * 192], runs[1].get(f)[:n * 192])
                    for f in (abi.F_VX, abi.F_VY, abi.F_VZ))
     ctx.close()
+
+
+def _ascii_frames(path):
+    """Frames of an ASCII trajectory (GPUbounce.cu:2412-2444): (step, n, text of
+    each cell's 192 position lines)."""
+    lines = path.read_text().splitlines()
+    out, i = [], lines.index("Header End") + 1  # the header (GPUbounce.cu:1019-1032) comes first
+    while i < len(lines):
+        n = int(lines[i]) // 192
+        step = int(lines[i + 1].split()[1])
+        cells = []
+        i += 2
+        for c in range(n):
+            cells.append(lines[i + 1:i + 1 + 192])
+            i += 193
+        out.append((step, n, cells))
+    return out
+
+
+@pytest.mark.parametrize("binary", [1, 0])
+def test_outputs_match_oracle(tmp_path, binary):
+    """The driver's output files against the CPU oracle run from the driver's own
+    parameters, mesh and frame 0 (--dump-setup): every trajectory frame (binary
+    floats bit for bit, or the ASCII text line for line), every forces-CSV row
+    (GPUbounce.cu:2567-2598: the total conservative force, its magnitude, the
+    velocity and its magnitude, position, pressure and volume, as text) and the
+    mitotic-index file (GPUbounce.cu:1515-1547) from the oracle's cell counts."""
+    from cellsim3d_amd import abi, configs
+    from cellsim3d_amd import mesh as M
+    from oracle import oracle as O
+    v0 = configs.mesh_volume(configs.unit_mesh())
+    d = json.loads(D._strip_comments(D.INP))
+    d["core"].update(division_Vol=float(1.05 * v0), div_time_steps=300, non_div_time_steps=0, trajWriteInt=50,
+                     write_cont_force=1, binaryOutput=binary, MaxNoofC180s=64)
+    d["counting"].update({"cell_count_int": 100, "overwrite_mit_ind_file?": 1})
+    (tmp_path / "inp.json").write_text(json.dumps(d))
+    M.write_reference_files(str(tmp_path), configs.unit_mesh())
+    print("celldiv 9 cells, 300 steps, binary %d" % binary, flush=True)
+    r = subprocess.run([CELLDIV, "9", "inp.json", "0", "--dump-setup", "setup"], cwd=str(tmp_path),
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    p = abi.Params.from_buffer_copy((tmp_path / "setup.params").read_bytes())
+    mesh = abi.Mesh.from_buffer_copy((tmp_path / "setup.mesh").read_bytes())
+    mc = d["core"]["MaxNoofC180s"]
+    X, Y, Z = (np.fromfile(str(tmp_path / ("setup." + a)), dtype=np.float32) for a in "xyz")
+    n = len(X) // 192
+    frames = D._read_frames(tmp_path / "t.xyz")[1] if binary else _ascii_frames(tmp_path / "t.xyz")
+    assert frames[0][0] == 0 and n == 9
+    vcells = max(1 << 18, 32 * mc)
+    sim = O.Sim(p, mesh, X, Y, Z, n, max_cells=mc, budget=vcells)
+    sim.init()
+    csv = (tmp_path / "f.csv").read_text().splitlines()
+    assert csv[0].startswith("step,num_cells,cell_ind")
+    rows = {}
+    for line in csv[1:]:
+        rows.setdefault(int(line.split(",", 1)[0]), []).append(line)
+    counts = {}
+    done = 0
+    for k, fr in enumerate(frames):
+        step = fr[0]
+        if step > done:
+            sim.step(step - done)
+            done = step
+        ns = sim.n
+        counts[step] = ns
+        Xo, Yo, Zo = (sim.get(f)[:ns * 192] for f in (abi.F_X, abi.F_Y, abi.F_Z))
+        if binary:
+            _, _, nf, Xf, Yf, Zf, _ = fr
+            assert nf == ns
+            for a, b in ((Xf, Xo), (Yf, Yo), (Zf, Zo)):
+                assert np.array_equal(a.ravel(), b), "frame at step %d differs from the oracle" % step
+        else:
+            _, nf, cells = fr
+            assert nf == ns
+            want = ["%.7f,  %.7f,  %.7f" % (Xo[c * 192 + q], Yo[c * 192 + q], Zo[c * 192 + q])
+                    for c in range(ns) for q in range(192)]
+            assert [l for c in cells for l in c] == want, "ASCII frame at step %d differs" % step
+        F = [sim.get(f)[:ns * 192].reshape(ns, 192) for f in (abi.F_FCX, abi.F_FCY, abi.F_FCZ, abi.F_VX, abi.F_VY,
+                                                                 abi.F_VZ, abi.F_X, abi.F_Y, abi.F_Z)]
+        P, V = sim.get(abi.F_PRESSURE)[:ns], sim.get(abi.F_VOLUME)[:ns]
+        want = []
+        for c in range(ns):
+            for q in range(180):
+                fx, fy, fz, vx, vy, vz, x, y, z = (a[c, q] for a in F)
+                fm = np.sqrt(fx * fx + fy * fy + fz * fz)
+                vm = np.sqrt(vx * vx + vy * vy + vz * vz)
+                want.append("%d,%d,%d,%d,%d,%s" % (step, ns, c, q, c * 192 + q, ",".join(
+                    "%.8f" % v for v in (fx, fy, fz, fm, vx, vy, vz, vm, x, y, z, P[c], V[c]))))
+        assert rows[step] == want, "forces CSV rows of step %d differ from the oracle" % step
+    assert counts[300] > counts[0], "no division: the test does not exercise the counting"
+    mi = [float(x) for x in (tmp_path / "m.dat").read_text().split()]
+    assert mi == _expected_mi(counts, steps=300, every=100)

@@ -56,7 +56,7 @@ def test_ensemble_batched_equals_plain():
     from cellsim3d_amd import abi, ensemble
     gm, tissues = ensemble.build(n_tissues=6, seed=3, nmin=1, nmax=40, steps=60, grid_interval=10)
     print("ensemble sizes:", [t["n"] for t in tissues], flush=True)
-    dt, ctx, systems = ensemble.run_batched(gm, tissues, 60, wg_per_cu=4, graph_steps=10)
+    dt, ctx, systems, _ = ensemble.run_batched(gm, tissues, 60, wg_per_cu=4, graph_steps=10)
     batched = [s.get(abi.F_X)[:s.n * 192].copy() for s in systems]
     ctx.close()
     for t, xb in zip(tissues, batched):

@@ -53,7 +53,7 @@ def test_ensemble_256_tissues_match_oracle():
     gm, tissues = ensemble.build(256, 1, 1, 512, steps=5000, grid_interval=10)
     cells = sum(t["n"] for t in tissues)
     print("ensemble: 256 tissues, %d cells, %d batched steps" % (cells, steps), flush=True)
-    dt, ctx, systems = ensemble.run_batched(gm, tissues, steps, wg_per_cu=16, graph_steps=10)
+    dt, ctx, systems, _ = ensemble.run_batched(gm, tissues, steps, wg_per_cu=16, graph_steps=10)
     got = []
     for s in systems:
         n, step, status = s.info()
@@ -87,7 +87,7 @@ def test_batched_divisions_match_oracle_and_alone():
         tissues.append({"params": p, "X": X, "Y": Y, "Z": Z, "n": n, "max_cells": 2 * n + 2})
     steps = 400
     cap = sum(t["max_cells"] for t in tissues)
-    ctx = engine.Context(gm, capacity_cells=cap)
+    ctx = engine.Context(gm, capacity_cells=cap, voxel_budget=1 << 22)
     systems = [ctx.add_system(t["params"], t["X"], t["Y"], t["Z"], t["n"], max_cells=t["max_cells"]) for t in tissues]
     ctx.set_graph_steps(10)
     for k in range(steps // 50):
@@ -106,7 +106,7 @@ def test_batched_divisions_match_oracle_and_alone():
         assert (n, step, status) == (sim.n, sim.stepno, st)
         _same("batched tissue %d" % i, arrays, sim, n, NODE + ("F_PRESSURE",))
         grew += n > t["n"]
-        c1 = engine.Context(gm, capacity_cells=t["max_cells"] + 4)
+        c1 = engine.Context(gm, capacity_cells=t["max_cells"] + 4, voxel_budget=1 << 20)
         s1 = c1.add_system(t["params"], t["X"], t["Y"], t["Z"], t["n"], max_cells=t["max_cells"])
         s1.step(steps)
         c1.sync()
@@ -118,12 +118,13 @@ def test_batched_divisions_match_oracle_and_alone():
 
 
 def test_served_requests_match_oracle():
-    """Ten requests; a 40-cell pool makes later ones wait for finished ones, and
-    requests are admitted while others are being stepped."""
+    """Ten requests arriving at once; a 40-cell pool admits the first few and
+    makes the others wait for finished ones, so later requests are admitted
+    while earlier ones are still being stepped."""
     from cellsim3d_amd import abi, serving
     gm = _mesh()
-    specs = [(0.0, 1, 60), (0.0, 8, 140), (0.0, 20, 90), (0.005, 3, 230), (0.01, 12, 110), (0.01, 2, 75),
-             (0.02, 27, 50), (0.03, 5, 300), (0.04, 9, 45), (0.05, 4, 160)]
+    specs = [(0.0, 1, 60), (0.0, 8, 140), (0.0, 20, 90), (0.0, 3, 230), (0.0, 12, 110), (0.0, 2, 75),
+             (0.0, 27, 50), (0.0, 5, 300), (0.0, 9, 45), (0.0, 4, 160)]
     reqs = [{"arrival": a, "n": n, "horizon": h, "seed": 21 + i} for i, (a, n, h) in enumerate(specs)]
     jobs = serving.make_jobs(reqs, grid_interval=10)
     print("serving: %d requests into a 40-cell pool" % len(jobs), flush=True)
@@ -140,3 +141,36 @@ def test_served_requests_match_oracle():
         sim, st = _oracle_run(gm, j.params, j.X, j.Y, j.Z, j.n, j.max_cells, j.horizon)
         assert j.result["n"] == sim.n
         _same("request %d (%d cells, %d steps)" % (j.id, j.n, j.horizon), j.result, sim, sim.n, NODE)
+
+
+def test_room_growth_keeps_results():
+    """Tissues that outgrow their room are moved to larger rooms between chunks
+    (cs3d_system_grow) while the others keep stepping: every tissue ends bit for
+    bit where its oracle run (with room to spare) ends."""
+    from cellsim3d_amd import abi, configs, engine, ensemble
+    gm = _mesh()
+    tissues = []
+    for i, n in enumerate((1, 2, 4, 3)):
+        _, X, Y, Z, info = configs.lattice_tissue(n, 0.30, seed=60 + i)
+        p = configs.base_params(9 + i)
+        p.division_vol = np.float32(1.02 * info["v0"])
+        p.div_time_steps = 10 ** 6
+        p.voxel_capacity = 64
+        tissues.append({"params": p, "X": X, "Y": Y, "Z": Z, "n": n, "max_cells": 2 * n + 2})
+    steps = 600
+    ctx = engine.Context(gm, capacity_cells=512, voxel_budget=1 << 24)
+    systems = [ctx.add_system(t["params"], t["X"], t["Y"], t["Z"], t["n"], max_cells=t["max_cells"]) for t in tissues]
+    ctx.set_graph_steps(10)
+    moves = ensemble.run_chunks(ctx, abi.ALL, steps, systems, chunk=10)
+    ctx.sync()
+    print("room moves: %d, cells %s, rooms %s" % (moves, [s.n for s in systems], [s.max_cells for s in systems]),
+          flush=True)
+    assert moves >= 2
+    for i, (t, s) in enumerate(zip(tissues, systems)):
+        n, step, status = s.info()
+        sim, st = _oracle_run(gm, t["params"], t["X"], t["Y"], t["Z"], t["n"], 64, steps)
+        assert (n, step, status) == (sim.n, sim.stepno, st), "tissue %d" % i
+        _same("grown tissue %d" % i, s, sim, n, NODE + ("F_PRESSURE", "F_VOLUME"))
+    work = ctx.work(0, len(systems))
+    assert all(w > 0 for w in work)
+    ctx.close()
