@@ -503,6 +503,7 @@ extern "C" cs3d_status cs3d_system_add_raw(cs3d_ctx* c, const cs3d_params* prm, 
         m.ext[2 * a + 1] = ih >= 0 ? ih : ih ^ 0x7FFFFFFF;
     }
     cs3d_ranmar_init(&m.rng, prm->ranmar_ij, prm->ranmar_kl);
+    kick_consts(prm->mass, prm->dt, m);
     launch_set_nsys(c->pool, c->nsys_high, c->stream);
     st = write_meta(c, sid, m);
     if (st != CS3D_OK) return st;

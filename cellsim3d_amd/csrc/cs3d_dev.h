@@ -69,8 +69,17 @@ struct SysMeta {
     int32_t shift_pending;  // 1 = the step that just ended computed sys_cm and its nodes are still to be
                             // shifted (set by the division phase, consumed by the shift kernel)
     int64_t cell_steps;   // sum over the completed steps of the cells stepped (node-updates / 180)
+    // the velocity kicks' double divisions by the particle mass as products with
+    // double-double constants, verified for every float significand (kick_consts):
+    // kick_a* = 1 / (2 m) for 0.5 * s / m, kick_b* = dt / (2 m) for 0.5 * dt * f / m
+    double kick_ah, kick_al, kick_bh, kick_bl;
+    int32_t kick_fast;    // 1: the constants above reproduce the divisions exactly
     cs3d_ranmar_state rng;
 };
+
+// Fills the kick constants of a system with particle mass m and time step dt and
+// checks them exhaustively (on the host; results are cached per (m, dt)).
+void kick_consts(float m, float dt, SysMeta& M);
 
 struct MeshDev {
     int32_t nn[3 * kSlots];

@@ -4,9 +4,13 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <utility>
 
 #include "../../include/cs3d.h"
+#include "cs3d_dev.h"
 #include "cs3d_internal.h"
 
 namespace cs3d {
@@ -252,3 +256,59 @@ extern "C" cs3d_status cs3d_traj_close(cs3d_traj* t, int32_t final_cells) {
     delete t;
     return st;
 }
+
+// ---- exact products for the velocity kicks' divisions by the particle mass ----
+// The reference divides in double: 0.5 * (double)s / (double)m with s a float
+// (Integrate, VelocityUpdateA, propagate.cu:698-705, 721-723) and
+// 0.5 * (double)dt * (double)f / (double)m with f a float (VelocityUpdateB,
+// :732-734).  Both numerators carry at most a float's 24-bit significand (0.5 * s
+// is exact; 0.5 * dt * f is an exact product of two floats), so each quotient is
+// a float times a fixed real c (1 / (2 m), or dt / (2 m)).  With c = ch + cl as a
+// double-double (ch = RN(c), cl = RN(c - ch)), fma(x, ch, x * cl) approximates
+// x * c to far below half an ulp of double and rounds to the same double as the
+// division except where x * c lies extremely close to a midpoint of two doubles;
+// scaling x by a power of two scales everything exactly (no double overflow or
+// underflow for float x), so whether that ever happens depends only on the 2^23
+// significands of x, and they are all checked here, once per (m, dt).  Only a
+// system whose check passes takes the products (kick_fast); otherwise the device
+// divides.  Zeros keep their sign on the device by a select; a non-finite force is
+// not covered (such a state is flagged NaN).
+namespace cs3d {
+namespace {
+std::mutex g_kick_mu;
+std::map<std::pair<uint32_t, uint32_t>, bool> g_kick_ok;
+
+bool kick_check(double m, double dt, double ah, double al, double bh, double bl) {
+    const double hdt = 0.5 * dt;
+    for (uint32_t M = 0; M < (1u << 23); ++M) {
+        const uint32_t bits = 0x3f800000u | M;
+        float f;
+        std::memcpy(&f, &bits, 4);
+        const double x = (double)f;
+        if (std::fma(x, ah, x * al) != 0.5 * x / m) return false;
+        if (std::fma(x, bh, x * bl) != hdt * x / m) return false;
+    }
+    return true;
+}
+}  // namespace
+
+void kick_consts(float m, float dt, SysMeta& M) {
+    const double md = (double)m, dd = (double)dt;
+    const double m2 = 2.0 * md;
+    M.kick_ah = 1.0 / m2;
+    M.kick_al = std::fma(-M.kick_ah, m2, 1.0) / m2;
+    M.kick_bh = 0.5 * dd / md;
+    M.kick_bl = std::fma(-M.kick_bh, md, 0.5 * dd) / md;
+    M.kick_fast = 0;
+    if (!(md > 0.0) || !(dd > 0.0) || !std::isfinite(M.kick_ah) || !std::isfinite(M.kick_bh)) return;
+    uint32_t km, kd;
+    std::memcpy(&km, &m, 4);
+    std::memcpy(&kd, &dt, 4);
+    std::lock_guard<std::mutex> lock(g_kick_mu);
+    auto it = g_kick_ok.find({km, kd});
+    if (it == g_kick_ok.end())
+        it = g_kick_ok.emplace(std::make_pair(km, kd), kick_check(md, dd, M.kick_ah, M.kick_al, M.kick_bh,
+                                                                  M.kick_bl)).first;
+    M.kick_fast = it->second ? 1 : 0;
+}
+}  // namespace cs3d

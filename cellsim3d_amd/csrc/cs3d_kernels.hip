@@ -151,11 +151,34 @@ __device__ __forceinline__ int f2o(float f) {
 }
 __device__ __forceinline__ float o2f(int i) { return __int_as_float(i >= 0 ? i : i ^ 0x7FFFFFFF); }
 
+// 0.5 * (double)s / (double)m of the velocity kicks (propagate.cu:698-705,
+// 721-723) for a float s.  On systems whose constants passed the host's
+// exhaustive check (kick_consts): s times the double-double 1 / (2 m), one
+// multiply and one fma with the division's exact result; a zero keeps its sign.
+__device__ __forceinline__ double kick_a(float s, float m, CMeta M) {
+    const double x = (double)s;
+    if (M->kick_fast) {
+        const double q = fma(x, M->kick_ah, x * M->kick_al);
+        return s == 0.0f ? x : q;
+    }
+    return 0.5 * x / (double)m;
+}
+// 0.5 * (double)dt * (double)f / (double)m of VelocityUpdateB (propagate.cu:732-734)
+__device__ __forceinline__ double kick_b(float f, float dt, float m, CMeta M) {
+    const double x = (double)f;
+    if (M->kick_fast) {
+        const double q = fma(x, M->kick_bh, x * M->kick_bl);
+        return f == 0.0f ? x : q;
+    }
+    return 0.5 * (double)dt * x / (double)m;
+}
+
 // Integrate's velocity half-kick of one component (propagate.cu:698-705): the
 // sum in float, the rest in double (0.5 is a double literal), rounded to float.
 // fr is the random force drawn in the previous step (0 without add_rands).
-__device__ __forceinline__ float half_kick(float v, float fc, float fd, float fr, float dt, float rdt, float m) {
-    return (float)((double)v + 0.5 * (double)(dt * fc + dt * fd + rdt * fr) / (double)m);
+__device__ __forceinline__ float half_kick(float v, float fc, float fd, float fr, float dt, float rdt, float m,
+                                           CMeta M) {
+    return (float)((double)v + kick_a(dt * fc + dt * fd + rdt * fr, m, M));
 }
 
 // ---- random force (CalculateRanForce, propagate.cu:667-676) --------------------
@@ -396,9 +419,10 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             const float dt = P->dt, m = P->mass;
             const float rdt = sqrtf(dt);
             const bool rnd = P->add_rands != 0;  // the previous step's random force (d_fRanList)
-            vx = half_kick(vx, fcx, fdx, rnd ? ldu(p.FRX + cb, ln) : 0.f, dt, rdt, m);
-            vy = half_kick(vy, fcy, fdy, rnd ? ldu(p.FRY + cb, ln) : 0.f, dt, rdt, m);
-            vz = half_kick(vz, fcz, fdz, rnd ? ldu(p.FRZ + cb, ln) : 0.f, dt, rdt, m);
+            const CMeta MC = cmeta(M);
+            vx = half_kick(vx, fcx, fdx, rnd ? ldu(p.FRX + cb, ln) : 0.f, dt, rdt, m, MC);
+            vy = half_kick(vy, fcy, fdy, rnd ? ldu(p.FRY + cb, ln) : 0.f, dt, rdt, m, MC);
+            vz = half_kick(vz, fcz, fdz, rnd ? ldu(p.FRZ + cb, ln) : 0.f, dt, rdt, m, MC);
             x = x + vx * dt;
             y = y + vy * dt;
             z = z + vz * dt;
@@ -1025,12 +1049,12 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                 }
                 const float vix = svx[tid], viy = svy[tid], viz = svz[tid];
                 // VelocityUpdateA (propagate.cu:721-723) then VelocityUpdateB (:732-734)
-                float vx = (float)((double)vix + 0.5 * (double)(dt * FX + rdt * frx) / (double)m);
-                float vy = (float)((double)viy + 0.5 * (double)(dt * FY + rdt * fry) / (double)m);
-                float vz = (float)((double)viz + 0.5 * (double)(dt * FZ + rdt * frz) / (double)m);
-                vx = (float)((double)vx + 0.5 * (double)dt * (double)fdx / (double)m);
-                vy = (float)((double)vy + 0.5 * (double)dt * (double)fdy / (double)m);
-                vz = (float)((double)vz + 0.5 * (double)dt * (double)fdz / (double)m);
+                float vx = (float)((double)vix + kick_a(dt * FX + rdt * frx, m, MC));
+                float vy = (float)((double)viy + kick_a(dt * FY + rdt * fry, m, MC));
+                float vz = (float)((double)viz + kick_a(dt * FZ + rdt * frz, m, MC));
+                vx = (float)((double)vx + kick_b(fdx, dt, m, MC));
+                vy = (float)((double)vy + kick_b(fdy, dt, m, MC));
+                vz = (float)((double)vz + kick_b(fdz, dt, m, MC));
                 stu(p.VX + cb, ln, vx); stu(p.VY + cb, ln, vy); stu(p.VZ + cb, ln, vz);
             }
         }
@@ -1240,9 +1264,10 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
                 // again next step when that is pair_skin/2 or more from the last search
                 const float dt = P->dt, m = P->mass, rdt = sqrtf(dt);
                 const bool rnd = P->add_rands != 0;
-                const float qx = X + half_kick(vix, fcX, fx, rnd ? ldu(p.FRX + cb, ln) : 0.f, dt, rdt, m) * dt - xbX;
-                const float qy = Y + half_kick(viy, fcY, fy, rnd ? ldu(p.FRY + cb, ln) : 0.f, dt, rdt, m) * dt - xbY;
-                const float qz = Z + half_kick(viz, fcZ, fz, rnd ? ldu(p.FRZ + cb, ln) : 0.f, dt, rdt, m) * dt - xbZ;
+                const CMeta MC = cmeta(M);
+                const float qx = X + half_kick(vix, fcX, fx, rnd ? ldu(p.FRX + cb, ln) : 0.f, dt, rdt, m, MC) * dt - xbX;
+                const float qy = Y + half_kick(viy, fcY, fy, rnd ? ldu(p.FRY + cb, ln) : 0.f, dt, rdt, m, MC) * dt - xbY;
+                const float qz = Z + half_kick(viz, fcZ, fz, rnd ? ldu(p.FRZ + cb, ln) : 0.f, dt, rdt, m, MC) * dt - xbZ;
                 const float h = 0.5f * P->pair_skin;
                 far = !(qx * qx + qy * qy + qz * qz <= h * h);
             }
@@ -2014,11 +2039,11 @@ __global__ __launch_bounds__(192) void k_velocity_ab(Pool p, Range r, int32_t wh
         if (which & 1) {  // VelocityUpdateA (propagate.cu:714-725)
 #pragma unroll
             for (int a = 0; a < 3; ++a)
-                v[a] = (float)((double)v[a] + 0.5 * (double)(dt * fc[a] + rdt * fr[a]) / (double)m);
+                v[a] = (float)((double)v[a] + kick_a(dt * fc[a] + rdt * fr[a], m, cmeta(M)));
         }
         if (which & 2) {  // VelocityUpdateB (propagate.cu:727-736)
 #pragma unroll
-            for (int a = 0; a < 3; ++a) v[a] = (float)((double)v[a] + 0.5 * (double)dt * (double)fd[a] / (double)m);
+            for (int a = 0; a < 3; ++a) v[a] = (float)((double)v[a] + kick_b(fd[a], dt, m, cmeta(M)));
         }
         p.VX[gi] = v[0]; p.VY[gi] = v[1]; p.VZ[gi] = v[2];
     }

@@ -3,8 +3,11 @@
 // A restatement, written for this project, of the reference kernel bodies of
 // SoftSimu/CellSim3D in their operation order.  Every function cites the
 // reference code it follows (paths relative to the reference root).  Built
-// with -ffp-contract=off so that each float operation rounds exactly as the
-// reference's separate IEEE operations do.
+// with -ffp-contract=off: every float operation rounds on its own.  That is this
+// project's canonical order, shared with the device kernels, not a property of
+// the reference's binary: nvcc contracts a*b+c into FMA by default
+// (makefile:1-13 does not pass --fmad=false) wherever it chooses, so the
+// reference's own last bits depend on those choices; the FP64 re-run bounds both.
 #include "cs3d_oracle.h"
 
 #include <algorithm>

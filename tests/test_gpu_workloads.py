@@ -87,7 +87,7 @@ def test_batched_divisions_match_oracle_and_alone():
         tissues.append({"params": p, "X": X, "Y": Y, "Z": Z, "n": n, "max_cells": 2 * n + 2})
     steps = 400
     cap = sum(t["max_cells"] for t in tissues)
-    ctx = engine.Context(gm, capacity_cells=cap, voxel_budget=1 << 22)
+    ctx = engine.Context(gm, capacity_cells=cap, voxel_budget=1 << 22, bin_budget=len(tissues) * 16384 * abi.BIN_WORDS + 64)
     systems = [ctx.add_system(t["params"], t["X"], t["Y"], t["Z"], t["n"], max_cells=t["max_cells"]) for t in tissues]
     ctx.set_graph_steps(10)
     for k in range(steps // 50):
@@ -106,7 +106,8 @@ def test_batched_divisions_match_oracle_and_alone():
         assert (n, step, status) == (sim.n, sim.stepno, st)
         _same("batched tissue %d" % i, arrays, sim, n, NODE + ("F_PRESSURE",))
         grew += n > t["n"]
-        c1 = engine.Context(gm, capacity_cells=t["max_cells"] + 4, voxel_budget=1 << 20)
+        c1 = engine.Context(gm, capacity_cells=t["max_cells"] + 4, voxel_budget=1 << 20,
+                            bin_budget=16384 * abi.BIN_WORDS + 64)
         s1 = c1.add_system(t["params"], t["X"], t["Y"], t["Z"], t["n"], max_cells=t["max_cells"])
         s1.step(steps)
         c1.sync()
