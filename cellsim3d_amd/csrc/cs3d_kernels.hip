@@ -372,18 +372,19 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     // a multi-system step walks the compact list of live slots (k_live)
     const int nit = kLive ? *p.n_live : r.slot_hi - r.slot_lo;
     for (int it = xcd_block(); it < nit; it += gridDim.x) {
-        const int slot = cell_slot<kLive>(p, r, it);
+        const int slot = kLive ? p.LIVE[it] : r.slot_lo + it;
         // the cell's node data are loaded before the system checks (every slot of the
         // range is allocated), so their latency overlaps the metadata reads
-        const size_t cb = (size_t)slot * kSlots;  // the cell's first node (uniform)
-        const uint32_t ln = lane_index();
-        float x = ldu(p.X + cb, ln), y = ldu(p.Y + cb, ln), z = ldu(p.Z + cb, ln);
+        // (plain 64-bit node indices here: measured faster for this short kernel than
+        // the uniform-base form of the other cell kernels)
+        const int64_t gi = (int64_t)slot * kSlots + tid;
+        float x = p.X[gi], y = p.Y[gi], z = p.Z[gi];
         const bool kick = flags & I_KICK;
         float vx = 0.f, vy = 0.f, vz = 0.f, fcx = 0.f, fcy = 0.f, fcz = 0.f, fdx = 0.f, fdy = 0.f, fdz = 0.f;
         if (kick) {
-            vx = ldu(p.VX + cb, ln); vy = ldu(p.VY + cb, ln); vz = ldu(p.VZ + cb, ln);
-            fcx = ldu(p.FCX + cb, ln); fcy = ldu(p.FCY + cb, ln); fcz = ldu(p.FCZ + cb, ln);
-            fdx = ldu(p.FDX + cb, ln); fdy = ldu(p.FDY + cb, ln); fdz = ldu(p.FDZ + cb, ln);
+            vx = p.VX[gi]; vy = p.VY[gi]; vz = p.VZ[gi];
+            fcx = p.FCX[gi]; fcy = p.FCY[gi]; fcz = p.FCZ[gi];
+            fdx = p.FDX[gi]; fdy = p.FDY[gi]; fdz = p.FDZ[gi];
         }
         const int s = sys_of(p, r, slot);
         if (s < 0) continue;
@@ -420,15 +421,15 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             const float rdt = sqrtf(dt);
             const bool rnd = P->add_rands != 0;  // the previous step's random force (d_fRanList)
             const CMeta MC = cmeta(M);
-            vx = half_kick(vx, fcx, fdx, rnd ? ldu(p.FRX + cb, ln) : 0.f, dt, rdt, m, MC);
-            vy = half_kick(vy, fcy, fdy, rnd ? ldu(p.FRY + cb, ln) : 0.f, dt, rdt, m, MC);
-            vz = half_kick(vz, fcz, fdz, rnd ? ldu(p.FRZ + cb, ln) : 0.f, dt, rdt, m, MC);
+            vx = half_kick(vx, fcx, fdx, rnd ? p.FRX[gi] : 0.f, dt, rdt, m, MC);
+            vy = half_kick(vy, fcy, fdy, rnd ? p.FRY[gi] : 0.f, dt, rdt, m, MC);
+            vz = half_kick(vz, fcz, fdz, rnd ? p.FRZ[gi] : 0.f, dt, rdt, m, MC);
             x = x + vx * dt;
             y = y + vy * dt;
             z = z + vz * dt;
-            stu(p.VAX + cb, ln, vx); stu(p.VAY + cb, ln, vy); stu(p.VAZ + cb, ln, vz);
-            if (flags & I_WRITEV) { stu(p.VX + cb, ln, vx); stu(p.VY + cb, ln, vy); stu(p.VZ + cb, ln, vz); }
-            stu(p.X + cb, ln, x); stu(p.Y + cb, ln, y); stu(p.Z + cb, ln, z);
+            p.VAX[gi] = vx; p.VAY[gi] = vy; p.VAZ[gi] = vz;
+            if (flags & I_WRITEV) { p.VX[gi] = vx; p.VY[gi] = vy; p.VZ[gi] = vz; }
+            p.X[gi] = x; p.Y[gi] = y; p.Z[gi] = z;
         }
         if (node && (flags & I_CLEAR)) p.bins[M->bbase + bin_of(M, x, y, z)] = 0;
         if ((flags & I_BINS) && (!(flags & I_IFSEARCH) || M->need_search)) {
@@ -439,7 +440,7 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             // cells that did not divide are binned at their last search position
             float qx = x, qy = y, qz = z;
             if ((flags & I_IFSEARCH) && M->need_search == 2 && node && !p.RESET[slot]) {
-                qx = ldu(p.XBX + cb, ln); qy = ldu(p.XBY + cb, ln); qz = ldu(p.XBZ + cb, ln);
+                qx = p.XBX[gi]; qy = p.XBY[gi]; qz = p.XBZ[gi];
             }
             const int b = node ? bin_of(M, qx, qy, qz) : -1;
             const int lane = tid & 63;
