@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -297,6 +298,41 @@ FieldLoc field_loc(cs3d_ctx* c, cs3d_sys s, cs3d_field f) {
 
 }  // namespace
 
+// ---- staging array of the allocation-free PressureReset -------------------------
+// cs3d_k_pressure_reset stages the caller's cell list in the context's scratch
+// buffer, which it allocates (hipMalloc) on first use and when a longer list
+// comes.  cs3d_k_pressure_reset_noalloc stages it in `scratch` instead: a
+// file-scope device array of kNoAllocCells entries, which every GPU the library
+// runs on holds until the process ends.  Its device address differs per GPU; it is
+// looked up once per device, with that device current, at context creation
+// (hipGetSymbolAddress) and kept in g_scratch_dev.  Host code passes only that
+// address to the device, never the array's host-side name.
+constexpr int32_t kNoAllocCells = 65536;
+static __device__ __align__(256) int32_t scratch[kNoAllocCells];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, int32_t*> g_scratch_dev;  // device -> address of `scratch` on it
+
+// with `device` current
+cs3d_status lookup_scratch(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev.count(device)) return CS3D_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(CS3D_HIP_ERROR, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[device] = static_cast<int32_t*>(ptr);
+    return CS3D_OK;
+}
+
+int32_t* scratch_of_device(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_dev.find(device);
+    return it == g_scratch_dev.end() ? nullptr : it->second;
+}
+}  // namespace
+
 extern "C" cs3d_status cs3d_ctx_create(int device, const cs3d_mesh* mesh, int64_t capacity_cells, int64_t voxel_budget,
                                        int64_t bin_budget, cs3d_ctx** out) {
     if (!mesh || !out || capacity_cells <= 0 || capacity_cells > (1ll << 30) / kSlots || voxel_budget <= 0 ||
@@ -345,6 +381,7 @@ extern "C" cs3d_status cs3d_ctx_create(int device, const cs3d_mesh* mesh, int64_
         return st;
     };
     if (hipSetDevice(device) != hipSuccess) return cleanup(fail(CS3D_HIP_ERROR, "hipSetDevice failed"));
+    if (cs3d_status ls = lookup_scratch(device); ls != CS3D_OK) return cleanup(ls);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(CS3D_HIP_ERROR, "hipStreamCreate failed"));
     Pool& p = c->pool;
@@ -1241,6 +1278,25 @@ extern "C" cs3d_status cs3d_k_pressure_reset(cs3d_ctx* c, cs3d_sys s, const int3
     if ((st = scratch_of(c, (size_t)count * sizeof(int32_t), &buf)) != CS3D_OK) return st;
     HIPCHK(hipMemcpyAsync(buf, cells, (size_t)count * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
     launch_pressure_reset(c->pool, range_of(c, s), (const int32_t*)buf, count, value, c->stream);
+    return finish(c);
+}
+
+extern "C" cs3d_status cs3d_k_pressure_reset_noalloc(cs3d_ctx* c, cs3d_sys s, const int32_t* cells, int32_t count,
+                                                     float value) {
+    if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
+    if (count < 0 || (count > 0 && !cells)) return fail(CS3D_INVALID_ARG, "bad cell list");
+    if (count > kNoAllocCells)
+        return fail(CS3D_INVALID_ARG, "cell list longer than the allocation-free staging array (65536 cells)");
+    if (count == 0) return CS3D_OK;
+    SysMeta m;
+    cs3d_status st = read_meta(c, s, m);
+    if (st != CS3D_OK) return st;
+    for (int32_t i = 0; i < count; ++i)
+        if (cells[i] < 0 || cells[i] >= m.n) return fail(CS3D_INVALID_ARG, "cell index out of range");
+    int32_t* buf = scratch_of_device(c->device);
+    if (!buf) return fail(CS3D_HIP_ERROR, "no staging array on this device");
+    HIPCHK(hipMemcpyAsync(buf, cells, (size_t)count * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    launch_pressure_reset(c->pool, range_of(c, s), buf, count, value, c->stream);
     return finish(c);
 }
 

@@ -71,6 +71,7 @@ SIGNATURES = {
     "cs3d_k_velocity_a": ([vp, C.c_int32], st_t),
     "cs3d_k_velocity_b": ([vp, C.c_int32], st_t),
     "cs3d_k_pressure_reset": ([vp, C.c_int32, vp, C.c_int32, C.c_float], st_t),
+    "cs3d_k_pressure_reset_noalloc": ([vp, C.c_int32, vp, C.c_int32, C.c_float], st_t),
     "cs3d_k_bboxes": ([vp, C.c_int32], st_t),
     "cs3d_k_extent": ([vp, C.c_int32, vp], st_t),
     "cs3d_get_state": ([vp, C.c_int32, vp, vp, vp, vp, vp, C.POINTER(C.c_int32), C.POINTER(C.c_uint32)], st_t),

@@ -344,6 +344,16 @@ cs3d_status cs3d_k_velocity_b(cs3d_ctx* ctx, cs3d_sys sys);
  * only read. */
 cs3d_status cs3d_k_pressure_reset(cs3d_ctx* ctx, cs3d_sys sys, const int32_t* cells, int32_t count,
                                   float value);
+/* PressureReset as cs3d_k_pressure_reset, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  The cell list is
+ * staged in a fixed device array of 65536 entries that the library holds on
+ * each GPU until the process ends (cs3d_k_pressure_reset grows a context buffer
+ * with hipMalloc instead); longer lists are refused with INVALID_ARG.  The call
+ * allocates nothing and returns after the reset completes.  Calls that overlap
+ * on one GPU (contexts on other host threads) share the array and race. */
+cs3d_status cs3d_k_pressure_reset_noalloc(cs3d_ctx* ctx, cs3d_sys sys, const int32_t* cells, int32_t count,
+                                          float value);
 /* CenterOfMass (centermass.cu:8-50). */
 cs3d_status cs3d_k_com(cs3d_ctx* ctx, cs3d_sys sys);
 /* volumes (volume.cu:5-163) for step `step`: VOLUME, AREA, DIVFLAG. */
