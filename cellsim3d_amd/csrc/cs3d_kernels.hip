@@ -625,6 +625,26 @@ __device__ __forceinline__ uint32_t angle_pair(float ax, float ay, float az, flo
     return ok ? 0u : (uint32_t)CS3D_SYS_ANGLE_NONFINITE;
 }
 
+// GetAngleForce pair (as angle_pair) with the squared lengths ra2, rb2 of the two
+// bond vectors and their square roots la, lb computed by the caller: the same
+// operations on the same vectors as in angle_pair (the springs of the node use
+// them too), so every bit is the same.
+__device__ __forceinline__ uint32_t angle_pair_l(float ax, float ay, float az, float ra2, float la, float bx, float by,
+                                                 float bz, float rb2, float lb, float th0, float k, float* out) {
+    const float idk = ax * bx + ay * by + az * bz;
+    const float inner = (float)((double)(ra2 * rb2 - idk * idk) + 1e-3);
+    const float c1 = -1.0f / sqrtf(inner);
+    const float den = la * lb;
+    const float theta = (float)acos((double)idk / ((double)den + 1e-3));
+    const float s = ((-k) * c1) * (theta - th0);
+    const float ca = idk / ra2, cb = idk / rb2;
+    out[0] = s * (bx - ca * ax); out[1] = s * (by - ca * ay); out[2] = s * (bz - ca * az);
+    out[3] = s * (ax - cb * bx); out[4] = s * (ay - cb * by); out[5] = s * (az - cb * bz);
+    bool ok = isfinite(out[0]) && isfinite(out[1]) && isfinite(out[2]) && isfinite(out[3]) && isfinite(out[4]) &&
+              isfinite(out[5]);
+    return ok ? 0u : (uint32_t)CS3D_SYS_ANGLE_NONFINITE;
+}
+
 // friction normal of node tid (propagate.cu:558-572), positions in LDS
 __device__ __forceinline__ void node_normal(const float* sx, const float* sy, const float* sz, const MeshDev* mesh,
                                             int tid, int N1, int N2, int N3, float& nx, float& ny, float& nz) {
@@ -914,39 +934,47 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         __syncthreads();
         uint32_t st = 0;
         const bool angles = forces && P->constrain_angles;
-        if (angles) {  // each angle once (propagate.cu:26-60), before anything else is live
-            if (node) {
-                const float k = P->angle_k;
-                const float* th = mesh->theta0 + 3 * tid;
-                const int n0 = mesh->nn[tid], n1 = mesh->nn[kSlots + tid], n2 = mesh->nn[2 * kSlots + tid];
-#pragma unroll 1
-                for (int q = 0; q < 3; ++q) {  // angles (n0,n1), (n1,n2), (n0,n2) of this node
-                    const int a = q == 1 ? n1 : n0;
-                    const int b = q == 0 ? n1 : n2;
-                    st |= angle_pair(sx[a] - X, sy[a] - Y, sz[a] - Z, sx[b] - X, sy[b] - Y, sz[b] - Z, th[q], k,
-                                     u.angF + (tid * 3 + q) * 6);
-                }
-            }
-            __syncthreads();
-        }
         float FX = 0.f, FY = 0.f, FZ = 0.f;
         int nn3[3] = {0, 0, 0};
         if (forces) {
             if (node && (isnan(X) || isnan(Y) || isnan(Z))) st |= CS3D_SYS_NAN;  // propagate.cu:241-248
-            // ---- springs, propagate.cu:298-314
-            const float stiffness = cld(p.E + slot);
             if (node) {
+                // the node's three bonds: vectors, squared lengths and lengths, used by the
+                // springs (propagate.cu:298-314) and by the node's own three angles
+                // (propagate.cu:26-60), which compute the same quantities from the same vectors
+                float bx[3], by[3], bz[3], b2[3], bl[3];
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
                     const int N1 = mesh->nn[i * kSlots + tid];
                     nn3[i] = N1;
-                    const float dX = sx[N1] - X, dY = sy[N1] - Y, dZ = sz[N1] - Z;
-                    const float R = sqrtf(dX * dX + dY * dY + dZ * dZ);
-                    const float R0 = mesh->R0[i * kSlots + tid];
-                    FX += stiffness * (R - R0) / R0 * dX / R;
-                    FY += stiffness * (R - R0) / R0 * dY / R;
-                    FZ += stiffness * (R - R0) / R0 * dZ / R;
+                    bx[i] = sx[N1] - X; by[i] = sy[N1] - Y; bz[i] = sz[N1] - Z;
+                    b2[i] = bx[i] * bx[i] + by[i] * by[i] + bz[i] * bz[i];
+                    bl[i] = sqrtf(b2[i]);
                 }
+                // ---- springs, propagate.cu:298-314
+                const float stiffness = cld(p.E + slot);
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    const float R = bl[i];
+                    const float R0 = mesh->R0[i * kSlots + tid];
+                    FX += stiffness * (R - R0) / R0 * bx[i] / R;
+                    FY += stiffness * (R - R0) / R0 * by[i] / R;
+                    FZ += stiffness * (R - R0) / R0 * bz[i] / R;
+                }
+                if (angles) {  // each angle once (propagate.cu:26-60): (n0,n1), (n1,n2), (n0,n2)
+                    const float k = P->angle_k;
+                    const float* th = mesh->theta0 + 3 * tid;
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        const int a = q == 1 ? 1 : 0;
+                        const int b = q == 0 ? 1 : 2;
+                        st |= angle_pair_l(bx[a], by[a], bz[a], b2[a], bl[a], bx[b], by[b], bz[b], b2[b], bl[b], th[q],
+                                           k, u.angF + (tid * 3 + q) * 6);
+                    }
+                }
+            }
+            if (angles) __syncthreads();  // the angle forces of every node are in LDS
+            if (node) {
                 // ---- pressure, propagate.cu:318-330 (centre of mass from the last CenterOfMass)
                 const float rx = X - cld(p.CMX + slot), ry = Y - cld(p.CMY + slot), rz = Z - cld(p.CMZ + slot);
                 const float inv = 1.0f / sqrtf(rx * rx + ry * ry + rz * rz);
