@@ -30,6 +30,7 @@
 #include <utility>
 #include <vector>
 
+#include "cs3d_acos.h"
 #include "cs3d_dev.h"
 
 namespace cs3d {
@@ -40,7 +41,7 @@ __device__ __forceinline__ int sys_of(const Pool& p, const Range& r, int slot) {
     return r.one_sys ? r.sys_lo : p.slot_sys[slot];
 }
 
-__device__ __forceinline__ bool sys_live(const SysMeta* m) {
+template <class MP> __device__ __forceinline__ bool sys_live(MP m) {
     return m->active && (m->horizon < 0 || m->step < m->horizon) && !(m->status & CS3D_SYS_FATAL_MASK);
 }
 
@@ -63,7 +64,7 @@ template <class T> __device__ __forceinline__ void stu(T* base, uint32_t i, T v)
 }
 
 // rGrowth of the population model, GPUbounce.cu:1077-1088
-__device__ __forceinline__ float rgrowth(const cs3d_params* P, int n) {
+template <class PP> __device__ __forceinline__ float rgrowth(PP P, int n) {
     if (P->do_pop_model == 1) {
         float r = (float)((double)P->growth_rate * (1 - (n * 1.0 / (double)P->max_pop)));
         if (r < 0) r = 0;
@@ -90,6 +91,17 @@ __device__ __forceinline__ CParams cparams(const cs3d_params* q) { return (CPara
 __device__ __forceinline__ CMeta cmeta(const SysMeta* q) { return (CMeta)q; }
 template <class T> __device__ __forceinline__ T cld(const T* q) {
     return *(__attribute__((address_space(4))) const T*)q;
+}
+
+// The kernel's Pool argument (the first kernel parameter) read again where it is
+// used: scalar loads from the kernel-argument segment behind an empty asm
+// statement, so that the compiler neither hoists them nor keeps the pointers live
+// (in scalar registers, or spilled to vector lanes) across the contact loop.
+typedef __attribute__((address_space(4))) const Pool* CPool;
+__device__ __forceinline__ CPool pool_late() {
+    CPool q = (CPool)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(q));
+    return q;
 }
 
 // The cell slot of loop iteration `it` of a cell kernel, made wave-uniform with
@@ -137,7 +149,7 @@ __device__ __forceinline__ int bin1(float x, float mn, float h, int div) {
 }
 __device__ __forceinline__ int wrap1(int s, int div) { return s < 0 ? s + div : (s >= div ? s - div : s); }
 
-__device__ __forceinline__ int bin_of(const SysMeta* M, float x, float y, float z) {
+template <class MP> __device__ __forceinline__ int bin_of(MP M, float x, float y, float z) {
     const int bx = bin1(x, M->bmin[0], M->bh, M->bdims[0]);
     const int by = bin1(y, M->bmin[1], M->bh, M->bdims[1]);
     const int bz = bin1(z, M->bmin[2], M->bh, M->bdims[2]);
@@ -248,7 +260,7 @@ __device__ __forceinline__ void noise3(uint32_t seed, uint32_t node, int64_t ste
 struct VoxRange {
     int sx, ex, sy, ey, sz, ez;
 };
-__device__ __forceinline__ VoxRange vox_range(const float* b, const SysMeta* M, const cs3d_params* P) {
+template <class MP, class PP> __device__ __forceinline__ VoxRange vox_range(const float* b, MP M, PP P) {
     const float DL = P->grid_dl, at = P->attraction_range;
     const float mx = M->gmin[0], my = M->gmin[1], mz = M->gmin[2];
     VoxRange v;
@@ -388,12 +400,15 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
         }
         const int s = sys_of(p, r, slot);
         if (s < 0) continue;
+        // metadata and parameters through scalar loads: this kernel writes no field it
+        // reads (only status bits, by atomics)
         SysMeta* M = p.meta + s;
-        if (!sys_live(M)) continue;
-        const int c = slot - M->base;
-        if (c >= M->n) continue;
-        const cs3d_params* P = p.prm + s;
-        if ((flags & I_GRIDCOUNT) && M->rebuild && !(M->status & CS3D_SYS_GRID_TOO_LARGE)) {
+        const CMeta MC = cmeta(M);
+        const CParams P = cparams(p.prm + s);
+        if (!sys_live(MC)) continue;
+        const int c = slot - MC->base;
+        if (c >= MC->n) continue;
+        if ((flags & I_GRIDCOUNT) && MC->rebuild && !(MC->status & CS3D_SYS_GRID_TOO_LARGE)) {
             // the grid was rebuilt at the end of the last step: this cell's new bbox
             // becomes the one the contact rule uses, and its voxels are counted
             // (makeNNlist's atomicAdd, propagatebound.cu:246-250) to catch overflow
@@ -402,12 +417,12 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
 #pragma unroll
             for (int a = 0; a < 6; ++a) b[a] = bn[a];
             if (tid < 6) p.BBOX[(int64_t)slot * 6 + tid] = b[tid];
-            const VoxRange v = vox_range(b, M, P);
+            const VoxRange v = vox_range(b, MC, P);
             store_vrange(p, slot, v, tid);
             const int nx = v.ex - v.sx + 1, ny = v.ey - v.sy + 1, nz = v.ez - v.sz + 1;
             const int total = (nx > 0 && ny > 0 && nz > 0) ? nx * ny * nz : 0;
-            int32_t* counts = p.vox + M->vbase;
-            const int xd = M->dims[0], yd = M->dims[1], cap = M->vcap;
+            int32_t* counts = p.vox + MC->vbase;
+            const int xd = MC->dims[0], yd = MC->dims[1], cap = MC->vcap;
             bool over = false;
             for (int t = tid; t < total; t += blockDim.x) {
                 const int j3 = v.sz + t % nz, j2 = v.sy + (t / nz) % ny, j1 = v.sx + t / (nz * ny);
@@ -420,7 +435,6 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             const float dt = P->dt, m = P->mass;
             const float rdt = sqrtf(dt);
             const bool rnd = P->add_rands != 0;  // the previous step's random force (d_fRanList)
-            const CMeta MC = cmeta(M);
             vx = half_kick(vx, fcx, fdx, rnd ? p.FRX[gi] : 0.f, dt, rdt, m, MC);
             vy = half_kick(vy, fcy, fdy, rnd ? p.FRY[gi] : 0.f, dt, rdt, m, MC);
             vz = half_kick(vz, fcz, fdz, rnd ? p.FRZ[gi] : 0.f, dt, rdt, m, MC);
@@ -431,18 +445,18 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             if (flags & I_WRITEV) { p.VX[gi] = vx; p.VY[gi] = vy; p.VZ[gi] = vz; }
             p.X[gi] = x; p.Y[gi] = y; p.Z[gi] = z;
         }
-        if (node && (flags & I_CLEAR)) p.bins[M->bbase + bin_of(M, x, y, z)] = 0;
-        if ((flags & I_BINS) && (!(flags & I_IFSEARCH) || M->need_search)) {
+        if (node && (flags & I_CLEAR)) p.bins[MC->bbase + bin_of(MC, x, y, z)] = 0;
+        if ((flags & I_BINS) && (!(flags & I_IFSEARCH) || MC->need_search)) {
             // insertion into the search bins: the lanes of a wave that share a bin
             // take one atomic between them (issued by the group's first lane), and
             // each lane's slot is the group's base plus its rank in the group.  A
             // partial search (after divisions) works on search positions: the nodes of
             // cells that did not divide are binned at their last search position
             float qx = x, qy = y, qz = z;
-            if ((flags & I_IFSEARCH) && M->need_search == 2 && node && !p.RESET[slot]) {
+            if ((flags & I_IFSEARCH) && MC->need_search == 2 && node && !p.RESET[slot]) {
                 qx = p.XBX[gi]; qy = p.XBY[gi]; qz = p.XBZ[gi];
             }
-            const int b = node ? bin_of(M, qx, qy, qz) : -1;
+            const int b = node ? bin_of(MC, qx, qy, qz) : -1;
             const int lane = tid & 63;
             uint64_t todo = __ballot(node);
             int leader_of = 0, rank = 0, group = 0;
@@ -457,13 +471,13 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
                 }
                 todo &= ~same;
             }
-            int32_t* counts = p.bins + M->bbase;
+            int32_t* counts = p.bins + MC->bbase;
             int k0 = 0;
             if (node && lane == leader_of) k0 = atomicAdd(&counts[b], group);
             const int k = __shfl(k0, leader_of) + rank;
             if (node) {
                 if (k < kBCap) {
-                    float4* ent = reinterpret_cast<float4*>(p.bins + M->ebase);
+                    float4* ent = reinterpret_cast<float4*>(p.bins + MC->ebase);
                     ent[(int64_t)b * kBCap + k] = make_float4(qx, qy, qz, __int_as_float(c * kSlots + tid));
                 } else {
                     flag(M, CS3D_SYS_BIN_OVERFLOW);
@@ -471,8 +485,8 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             }
         }
         if (tid == 0 && (flags & I_PRESSURE)) {
-            const int64_t stepno = step_override >= 0 ? step_override : M->step + 1;
-            const float inc = rgrowth(P, M->n) * p.GS[slot];
+            const int64_t stepno = step_override >= 0 ? step_override : MC->step + 1;
+            const float inc = rgrowth(P, MC->n) * p.GS[slot];
             const float mx = P->max_pressure;
             float pr = p.P[slot];
             if (P->use_different_stiffnesses) {
@@ -488,7 +502,7 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             }
             p.P[slot] = pr;
         }
-        if (kick && (flags & I_IFSEARCH) && M->need_search == 2 && p.RESET[slot]) {
+        if (kick && (flags & I_IFSEARCH) && MC->need_search == 2 && p.RESET[slot]) {
             // a cell that divided or left the skin in the last step publishes the box of
             // its new positions: the partial search re-searches the cells within reach
             box_of_cell(x, y, z, node, ibox, p.SBOX + (int64_t)slot * 6);
@@ -615,7 +629,7 @@ __device__ __forceinline__ uint32_t angle_pair(float ax, float ay, float az, flo
     const float inner = (float)((double)(ra2 * rb2 - idk * idk) + 1e-3);
     const float c1 = -1.0f / sqrtf(inner);
     const float den = sqrtf(ra2) * sqrtf(rb2);
-    const float theta = (float)acos((double)idk / ((double)den + 1e-3));
+    const float theta = (float)cs3d_acos((double)idk / ((double)den + 1e-3));
     const float s = ((-k) * c1) * (theta - th0);
     const float ca = idk / ra2, cb = idk / rb2;
     out[0] = s * (bx - ca * ax); out[1] = s * (by - ca * ay); out[2] = s * (bz - ca * az);
@@ -635,7 +649,7 @@ __device__ __forceinline__ uint32_t angle_pair_l(float ax, float ay, float az, f
     const float inner = (float)((double)(ra2 * rb2 - idk * idk) + 1e-3);
     const float c1 = -1.0f / sqrtf(inner);
     const float den = la * lb;
-    const float theta = (float)acos((double)idk / ((double)den + 1e-3));
+    const float theta = (float)cs3d_acos((double)idk / ((double)den + 1e-3));
     const float s = ((-k) * c1) * (theta - th0);
     const float ca = idk / ra2, cb = idk / rb2;
     out[0] = s * (bx - ca * ax); out[1] = s * (by - ca * ay); out[2] = s * (bz - ca * az);
@@ -1049,7 +1063,7 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                                                  k, rX + sb, rY + sb, rZ + sb, rVX + sb, rVY + sb, rVZ + sb, cfx, cfy,
                                                  cfz, fdx, fdy, fdz);
             }
-            stu(p.NPAIR + cb, ln, np);
+            stu(pool_late()->NPAIR + cb, ln, np);
             if (forces) {
                 FX += cfx; FY += cfy; FZ += cfz;
                 if (P->use_rigid_box) {  // propagate.cu:473-510 (lower wall at 0)
@@ -1074,7 +1088,8 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                     noise3(P->noise_seed, (uint32_t)(c * kSlots + tid), MC->step + 1, zr);
                     const float sc = P->rand_scale_factor;
                     frx = sc * zr[0]; fry = sc * zr[1]; frz = sc * zr[2];
-                    stu(p.FRX + cb, ln, frx); stu(p.FRY + cb, ln, fry); stu(p.FRZ + cb, ln, frz);
+                    const CPool q = pool_late();
+                    stu(q->FRX + cb, ln, frx); stu(q->FRY + cb, ln, fry); stu(q->FRZ + cb, ln, frz);
                 }
                 const float vix = svx[tid], viy = svy[tid], viz = svz[tid];
                 // VelocityUpdateA (propagate.cu:721-723) then VelocityUpdateB (:732-734)
@@ -1084,10 +1099,14 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                 vx = (float)((double)vx + kick_b(fdx, dt, m, MC));
                 vy = (float)((double)vy + kick_b(fdy, dt, m, MC));
                 vz = (float)((double)vz + kick_b(fdz, dt, m, MC));
-                stu(p.VX + cb, ln, vx); stu(p.VY + cb, ln, vy); stu(p.VZ + cb, ln, vz);
+                const CPool q = pool_late();
+                stu(q->VX + cb, ln, vx); stu(q->VY + cb, ln, vy); stu(q->VZ + cb, ln, vz);
             }
         }
-        if (node && forces) { stu(p.FCX + cb, ln, FX); stu(p.FCY + cb, ln, FY); stu(p.FCZ + cb, ln, FZ); }
+        if (node && forces) {
+            const CPool q = pool_late();
+            stu(q->FCX + cb, ln, FX); stu(q->FCY + cb, ln, FY); stu(q->FCZ + cb, ln, FZ);
+        }
         if (st) flag(M, st);
         __syncthreads();
     }
@@ -1247,11 +1266,15 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
         }
         const int s = sys_of(p, r, slot);
         if (s < 0) continue;
+        // metadata and parameters through scalar loads: the fields read here are not
+        // written by this kernel (search_next and n_reset, which it updates by atomics,
+        // are read by atomic loads)
         SysMeta* M = p.meta + s;
-        if (!sys_live(M)) continue;
-        const int c = slot - M->base;
-        if (c >= M->n) continue;
-        const cs3d_params* P = p.prm + s;
+        const CMeta MC = cmeta(M);
+        const CParams P = cparams(p.prm + s);
+        if (!sys_live(MC)) continue;
+        const int c = slot - MC->base;
+        if (c >= MC->n) continue;
         if (tid == 0) moved = 0;
         sx[tid] = X; sy[tid] = Y; sz[tid] = Z;
         if (do_dis) { svx[tid] = vX; svy[tid] = vY; svz[tid] = vZ; }
@@ -1281,7 +1304,7 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
             k.kr = 0.f;
             k.gext = P->viscotic_damping;
             const int32_t* lst = p.PAIRS + (size_t)slot * kPMax * kSlots;
-            const size_t sb = (size_t)M->base * kSlots;  // the system's first node
+            const size_t sb = (size_t)MC->base * kSlots;  // the system's first node
             pair_friction(lst, (int)ln, np, vix, viy, viz, nx, ny, nz, k.gext, rVX + sb, rVY + sb, rVZ + sb, fx, fy, fz);
             fx = fx - P->gamma_visc * vix;  // drag, propagate.cu:658
             fy = fy - P->gamma_visc * viy;
@@ -1293,7 +1316,6 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
                 // again next step when that is pair_skin/2 or more from the last search
                 const float dt = P->dt, m = P->mass, rdt = sqrtf(dt);
                 const bool rnd = P->add_rands != 0;
-                const CMeta MC = cmeta(M);
                 const float qx = X + half_kick(vix, fcX, fx, rnd ? ldu(p.FRX + cb, ln) : 0.f, dt, rdt, m, MC) * dt - xbX;
                 const float qy = Y + half_kick(viy, fcY, fy, rnd ? ldu(p.FRY + cb, ln) : 0.f, dt, rdt, m, MC) * dt - xbY;
                 const float qz = Z + half_kick(viz, fcZ, fz, rnd ? ldu(p.FRZ + cb, ln) : 0.f, dt, rdt, m, MC) * dt - xbZ;
@@ -1312,17 +1334,17 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
                 atomicOr(&M->search_next, 1);
             if (__hip_atomic_load(&M->n_reset, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= kMaxPartial) {
                 const int k = atomicAdd(&M->n_reset, 1);
-                if (k < kMaxPartial) p.RSL[M->base + k] = c;
+                if (k < kMaxPartial) p.RSL[MC->base + k] = c;
             }
         }
-        if (clear_bins && (mode != M_STEP || M->need_search)) {
+        if (clear_bins && (mode != M_STEP || MC->need_search)) {
             // the search of this step is done: this cell's share of the system's
             // search-bin counters (those of the current bin grid, which every
             // insertion since the last clear used) is cleared in one coalesced run
-            const int64_t nb = min((int64_t)M->bbudget, (int64_t)M->bdims[0] * M->bdims[1] * M->bdims[2]);
-            const int64_t per = (nb + M->n - 1) / M->n;
+            const int64_t nb = min((int64_t)MC->bbudget, (int64_t)MC->bdims[0] * MC->bdims[1] * MC->bdims[2]);
+            const int64_t per = (nb + MC->n - 1) / MC->n;
             const int64_t lo = (int64_t)c * per, hi = min(nb, lo + per);
-            int32_t* counts = p.bins + M->bbase;
+            int32_t* counts = p.bins + MC->bbase;
             for (int64_t b = lo + tid; b < hi; b += blockDim.x) counts[b] = 0;
         }
         if (fused) {  // this cell's bbox at the end of the step (bounding_boxes, propagatebound.cu:10-141)
@@ -1342,9 +1364,9 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
         }  // combined after the next barrier
 
         const bool do_com = mode == M_STEP || mode == M_COM;
-        const int64_t stepno = step_override >= 0 ? step_override : M->step + 1;
+        const int64_t stepno = step_override >= 0 ? step_override : MC->step + 1;
         const bool do_vol =
-            mode == M_VOLUMES || (mode == M_STEP && stepno <= P->div_time_steps && rgrowth(P, M->n) > 0.0f);
+            mode == M_VOLUMES || (mode == M_STEP && stepno <= P->div_time_steps && rgrowth(P, MC->n) > 0.0f);
         if (do_com) {
             // 256-slot tree of centermass.cu:32-41: levels 128 and 64 through LDS, 32..1 by shuffles
             if (tid < 128) {
@@ -1549,18 +1571,29 @@ __device__ void divide_system(const Pool& p, int s, int32_t mode) {
     const int tid = threadIdx.x;
     SysMeta* M = p.meta + s;
     const bool step_mode = mode == M_STEP;
+    // the metadata and parameters that a step without divisions needs, read up front
+    // in one batch of scalar loads (this kernel writes these fields only after
+    // reading them): one memory round trip instead of one per branch
+    const CMeta MC = cmeta(M);
+    const CParams PC = cparams(p.prm + s);
+    const int32_t m_active = MC->active, n0 = MC->n, base = MC->base, cap = MC->cap;
+    const int64_t m_step = MC->step, m_horizon = MC->horizon, m_cell_steps = MC->cell_steps;
+    const uint32_t m_status = MC->status;
+    const int32_t m_nreset = MC->n_reset, m_search_next = MC->search_next;
+    const int64_t div_time = PC->div_time_steps;
+    const int32_t m_gint = PC->grid_interval, m_ccom = PC->correct_com;
+    const float m_rgrowth = rgrowth(PC, n0);
+    const bool live = m_active && (m_horizon < 0 || m_step < m_horizon) && !(m_status & CS3D_SYS_FATAL_MASK);
     // a stopped system keeps a pending grid adoption: its bboxes are adopted when it
     // resumes or when a caller reads the grid
-    if (!M->active || (step_mode && !sys_live(M))) {
+    if (!m_active || (step_mode && !live)) {
         if (tid == 0 && step_mode) M->shift_pending = 0;
         return;
     }
     const cs3d_params* P = p.prm + s;
-    const int base = M->base;
-    const int n0 = M->n;
-    const int64_t stepno = M->step + 1;
+    const int64_t stepno = m_step + 1;
     bool growth = true;
-    if (step_mode) growth = stepno <= P->div_time_steps && rgrowth(P, n0) > 0.0f;
+    if (step_mode) growth = stepno <= div_time && m_rgrowth > 0.0f;
     if (tid == 0) { nev = 0; ncur = n0; stop = 0; bad = 0; }
     __syncthreads();
     if (growth) {
@@ -1592,7 +1625,7 @@ __device__ void divide_system(const Pool& p, int s, int32_t mode) {
                 else random_vector(&M->rng, nrm);
                 const float nn = sqrtf(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
                 if ((double)fabsf(nn - 1) > 1e-3) bad |= CS3D_SYS_BAD_NORMAL;
-                if (ncur >= M->cap) { bad |= CS3D_SYS_CAPACITY; stop = 1; }
+                if (ncur >= cap) { bad |= CS3D_SYS_CAPACITY; stop = 1; }
             }
             __syncthreads();
             if (stop) break;
@@ -1683,7 +1716,7 @@ __device__ void divide_system(const Pool& p, int s, int32_t mode) {
     }
     __syncthreads();
     const int events = ncur - n0;
-    const int gint = P->grid_interval > 0 ? P->grid_interval : 1;
+    const int gint = m_gint > 0 ? m_gint : 1;
     const bool rebuild = step_mode && (events > 0 || stepno % gint == 0);
     if (rebuild) {
         // grid rebuild from the end-of-step bboxes: minmaxpre/minmaxpost
@@ -1707,7 +1740,7 @@ __device__ void divide_system(const Pool& p, int s, int32_t mode) {
         int32_t* counts = p.vox + M->vbase;
         for (int64_t v = tid; v < nclear; v += blockDim.x) counts[v] = 0;
     }
-    const bool shift = step_mode && P->correct_com;
+    const bool shift = step_mode && m_ccom;
     if (shift) {
         // calc_sys_CM (GPUbounce.cu:2504-2520): float sums of the cell centres in
         // cell order (daughters of this step still hold 0), divided by the count;
@@ -1736,7 +1769,7 @@ __device__ void divide_system(const Pool& p, int s, int32_t mode) {
         if (step_mode) {
             M->rebuild = rebuild ? 1 : 0;
             M->step = stepno;
-            M->cell_steps += n0;  // the n0 cells that were stepped
+            M->cell_steps = m_cell_steps + n0;  // the n0 cells that were stepped
             // After divisions, the divided cells and the cells listed by k_dis_post (a
             // node leaves the skin next step) are searched again next step, with the
             // cells near them (partial search).  Without divisions a node leaving the
@@ -1747,10 +1780,10 @@ __device__ void divide_system(const Pool& p, int s, int32_t mode) {
             // (their search positions are not shifted).
             // RSL holds one entry per cell slot of the system (M->cap entries at
             // RSL[base]): a list that would not fit there means a full search
-            int nr = M->n_reset;
-            const int lim = min(kMaxPartial, M->cap);
-            bool full = (events == 0 && M->search_next) || nr > lim || nr + 2 * events > lim ||
-                        (shift && (events > 0 || M->search_next));
+            int nr = m_nreset;
+            const int lim = min(kMaxPartial, cap);
+            bool full = (events == 0 && m_search_next) || nr > lim || nr + 2 * events > lim ||
+                        (shift && (events > 0 || m_search_next));
             if (!full)
                 for (int e = 0; e < events; ++e) {
                     const int pc = p.EVL[base + e], dc = n0 + e;

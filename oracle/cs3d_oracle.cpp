@@ -10,6 +10,8 @@
 // reference's own last bits depend on those choices; the FP64 re-run bounds both.
 #include "cs3d_oracle.h"
 
+#include "../cellsim3d_amd/csrc/cs3d_acos.h"
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -73,9 +75,11 @@ template <class T> V3<T> angle_force(V3<T> iPos, V3<T> kPos, T theta_o, T k, uin
         // `+ 1e-3` is a double literal: the sum is double, then sqrtf takes it as float (:32)
         float inner = (float)((double)(ri_2 * rk_2 - i_dot_k * i_dot_k) + 1e-3);
         c1 = (T)(-1.0f / sqrtf(inner));
-        // acos of a double argument, result stored into a float (:36)
+        // acos of a double argument, result stored into a float (:36); the acos is
+        // the one the device kernels call (cs3d_acos.h), so that the float angle is
+        // the same where acos lies near a float midpoint
         float den = sqrtf((float)ri_2) * sqrtf((float)rk_2);
-        theta = (T)(float)std::acos((double)i_dot_k / ((double)den + 1e-3));
+        theta = (T)(float)cs3d_acos((double)i_dot_k / ((double)den + 1e-3));
     } else {
         c1 = T(-1) / std::sqrt(ri_2 * rk_2 - i_dot_k * i_dot_k + T(1e-3));
         theta = std::acos(i_dot_k / (std::sqrt(ri_2) * std::sqrt(rk_2) + T(1e-3)));
@@ -618,6 +622,10 @@ void pressure_update(int n, const cs3d_params* p, T* P, const T* stiff, const T*
 
 // ============================== RANMAR ==========================================
 // rmarin, marsaglia.h:54-113
+extern "C" void or_acos(const double* x, double* y, int n) {
+    for (int i = 0; i < n; ++i) y[i] = cs3d_acos(x[i]);
+}
+
 extern "C" int or_ranmar_init(cs3d_ranmar_state* st, int ij, int kl) {
     if (ij < 0 || ij > 31328 || kl < 0 || kl > 30081) return 1;
     int i = (ij / 177) % 177 + 2;

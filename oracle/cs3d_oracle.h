@@ -33,6 +33,10 @@ void or_ranmar(cs3d_ranmar_state* st, float* out, int n);
 void or_random_vector(cs3d_ranmar_state* st, float* n3);
 void or_random_vector_basis(cs3d_ranmar_state* st, float* n3, const float* basis3);
 
+/* The double acos of the angle force (propagate.cu:36), as the device kernels
+   compute it (cellsim3d_amd/csrc/cs3d_acos.h), over n arguments. */
+void or_acos(const double* x, double* y, int n);
+
 /* R0 (GPUbounce.cu:1892-1908) and theta0 (GPUbounce.cu:909-938) of one placed cell. */
 void or_mesh_rest(const int32_t* nn, const float* x, const float* y, const float* z,
                   float* R0, float* theta0);

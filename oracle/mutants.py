@@ -22,7 +22,11 @@ SRC = os.path.join(HERE, "cs3d_oracle.cpp")
 CXXFLAGS = ["-O2", "-march=x86-64-v3", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC",
             "-std=c++17", "-shared"]
 
-# name, original text (must occur exactly once), replacement
+ACOS_H = os.path.join(ROOT, "cellsim3d_amd", "csrc", "cs3d_acos.h")
+ACOS_INCLUDE = '#include "../cellsim3d_amd/csrc/cs3d_acos.h"'
+
+# name, original text (must occur exactly once), replacement[, file: the oracle source by
+# default, or ACOS_H for the acos the oracle shares with the kernels]
 MUTANTS = [
     ("angle_map",
      "t1 = th0[3 * ni + 0];  // aij\n        t2 = th0[3 * ni + 1];  // ajk",
@@ -52,18 +56,34 @@ MUTANTS = [
      "for (int a = 0; a < 3; ++a) c3[a] = c3[a] / (T)n;",
      "for (int a = 0; a < 3; ++a) c3[a] = c3[a] / (T)(n + 1);"),
     ("sys_centre_sign", "X[g] = X[g] - c3[0];", "X[g] = X[g] + c3[0];"),
+    ("acos_coefficient", "f = fma(f, z, 0x1.6db6db6e31f13p-5);", "f = fma(f, z, 0x1.6db6db6e31f13p-6);", ACOS_H),
+    ("acos_negative_branch", "const int ki = big ? (x > 0.0 ? -2 : 2) : 1;",
+     "const int ki = big ? (x > 0.0 ? -2 : 1) : 1;", ACOS_H),
+    ("acos_half_angle", "fma(-0.5, ax, 0.5) :", "fma(-0.5, ax, 0.5000001) :", ACOS_H),
+    ("acos_pi_low", "-m * 0x1.1a62633145c07p-54", "m * 0x1.1a62633145c07p-54", ACOS_H),
+    ("acos_sign", "const double s = big ? sqrt(z) : x;", "const double s = big ? sqrt(z) : ax;", ACOS_H),
 ]
 
 
-def run(name, old, new, tmp):
-    """Builds the oracle with `old` replaced by `new` (unchanged when old is None) and runs the
-    pin tests on it.  The result starts with "killed" (a test failed or the process crashed),
-    "passed" or "error"."""
+def run(name, old, new, tmp, path=SRC):
+    """Builds the oracle with `old` replaced by `new` in `path` (unchanged when old is None) and
+    runs the pin tests on it.  The result starts with "killed" (a test failed or the process
+    crashed), "passed" or "error"."""
     text = open(SRC).read()
     if old is not None:
-        if text.count(old) != 1:
+        target = open(path).read()
+        if target.count(old) != 1:
             return "error: anchor not found once"
-        text = text.replace(old, new, 1)
+        target = target.replace(old, new, 1)
+        if path == SRC:
+            text = target
+        else:  # a mutated copy of the shared acos header, included by absolute path
+            hdr = os.path.join(tmp, name + "_acos.h")
+            with open(hdr, "w") as f:
+                f.write(target)
+            if text.count(ACOS_INCLUDE) != 1:
+                return "error: acos include not found once"
+            text = text.replace(ACOS_INCLUDE, '#include "%s"' % hdr)
     cpp = os.path.join(tmp, name + ".cpp")
     so = os.path.join(tmp, name + ".so")
     with open(cpp, "w") as f:
@@ -88,7 +108,7 @@ def check(names=(), workers=4):
     default), `workers` at a time; returns {name: result}."""
     chosen = [("control", None, None)] + [m for m in MUTANTS if not names or m[0] in names]
     with tempfile.TemporaryDirectory() as tmp, ThreadPoolExecutor(workers) as ex:
-        futs = [(m[0], ex.submit(run, *m, tmp)) for m in chosen]
+        futs = [(m[0], ex.submit(run, m[0], m[1], m[2], tmp, *m[3:])) for m in chosen]
         return {name: f.result() for name, f in futs}
 
 

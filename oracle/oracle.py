@@ -35,6 +35,8 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("oracle library missing: build it with `make -C oracle`")
         L = C.CDLL(LIB_PATH)
+        f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+        L.or_acos.argtypes = [f64p, f64p, C.c_int]
         L.or_ranmar_init.argtypes = [RP, C.c_int, C.c_int]
         L.or_ranmar_init.restype = C.c_int
         L.or_ranmar.argtypes = [RP, f32p, C.c_int]
@@ -122,6 +124,14 @@ class Ranmar:
         out = np.zeros(3, dtype=np.float32)
         lib().or_random_vector_basis(C.byref(self.st), out, np.asarray(basis, dtype=np.float32))
         return out
+
+
+def acos(x):
+    """The angle force's double acos (cellsim3d_amd/csrc/cs3d_acos.h) of each element."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    lib().or_acos(x, y, x.size)
+    return y
 
 
 def mesh_rest(nn, x, y, z):

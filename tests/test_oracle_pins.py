@@ -727,3 +727,36 @@ def test_correct_com_shifts_every_node_by_the_mean_centre(gp30):
         assert np.all(sim.get(abi.F_X).reshape(n, S)[:, 180:] == 0)
         if k == 0:
             assert np.all(np.abs(sc) > 1.0)  # the first step moves the system to the origin
+
+
+# ------------------------------------------------------------------ angle acos
+def test_acos_against_extended_precision():
+    """The angle force's double acos (propagate.cu:36; cellsim3d_amd/csrc/cs3d_acos.h,
+    shared by the kernels and the oracle) against long double arccos: within 1.2
+    double ulps everywhere, the same float after rounding wherever the true value is
+    not within 2 double ulps of a float midpoint, exact special values, monotone."""
+    rng = np.random.default_rng(5)
+    n = 400000
+    x = np.concatenate([rng.uniform(-1, 1, n), -0.5 + 0.05 * rng.uniform(-1, 1, n),
+                        -0.3 + 0.2 * rng.uniform(-1, 1, n), 1 - 1e-3 * rng.uniform(0, 1, n),
+                        -1 + 1e-3 * rng.uniform(0, 1, n), 1e-6 * rng.uniform(-1, 1, n),
+                        np.nextafter(0.5, np.array([0.0, 1.0])), np.nextafter(-0.5, np.array([0.0, -1.0]))])
+    y = O.acos(x)
+    ref = np.arccos(x.astype(np.longdouble))
+    r64 = ref.astype(np.float64)
+    ulp = np.spacing(np.abs(r64))
+    err = np.abs((y.astype(np.longdouble) - ref) / ulp.astype(np.longdouble)).astype(np.float64)
+    assert err.max() <= 1.2, (err.max(), x[np.argmax(err)])
+    # float rounding: equal unless the true value lies within 2 double ulps of a float midpoint
+    f_ours = y.astype(np.float32)
+    f_true = ref.astype(np.float32)
+    lo = np.minimum(f_ours, f_true).astype(np.float64)
+    mid = lo + np.spacing(lo.astype(np.float32)).astype(np.float64) / 2
+    near = np.abs(r64 - mid) <= 2 * ulp
+    assert np.all((f_ours == f_true) | near)
+    sp = np.array([1.0, -1.0, 0.0, -0.0, 1.0000000000000002, -1.0000000000000002, 2.0, np.nan])
+    ys = O.acos(sp)
+    assert ys[0] == 0.0 and ys[1] == np.pi and ys[2] == np.pi / 2 and ys[3] == np.pi / 2
+    assert np.all(np.isnan(ys[4:]))
+    g = np.linspace(-1, 1, 200001)
+    assert np.all(np.diff(O.acos(g)) <= 0)
