@@ -37,6 +37,8 @@ void launch_pressure_reset(const Pool& p, const Range& r, const int32_t* cells, 
                            hipStream_t st);
 void launch_set_flags(const Pool& p, const Range& r, int rebuild, int search, hipStream_t st);
 void launch_copy_calibrate(const void* src, void* dst, int64_t n4, hipStream_t st);
+void launch_check_sqrt(unsigned long long* bad, hipStream_t st);
+void launch_acos(const double* x, double* y, int64_t n, hipStream_t st);
 }  // namespace cs3d
 
 using namespace cs3d;
@@ -1385,6 +1387,41 @@ extern "C" cs3d_status cs3d_time_kernel(cs3d_ctx* c, cs3d_sys s, int32_t which, 
     const cs3d_status st = timed_groups(c, 1, [&](int) { for (int q = 0; q < reps; ++q) launch(); }, &ms);
     if (st != CS3D_OK) return st;
     *avg_ms = ms / reps;
+    return CS3D_OK;
+}
+
+extern "C" cs3d_status cs3d_check_sqrt(cs3d_ctx* c, uint64_t* mismatches) {
+    if (!c || !mismatches) return fail(CS3D_INVALID_ARG, "bad self-check arguments");
+    unsigned long long* bad = nullptr;
+    HIPCHK(hipMalloc(&bad, sizeof(*bad)));
+    hipError_t e = hipMemsetAsync(bad, 0, sizeof(*bad), c->stream);
+    if (e == hipSuccess) {
+        launch_check_sqrt(bad, c->stream);
+        e = hipGetLastError();
+    }
+    unsigned long long h = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, bad, sizeof(h), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(bad);
+    if (e != hipSuccess) return fail(CS3D_HIP_ERROR, hipGetErrorString(e));
+    *mismatches = (uint64_t)h;
+    return CS3D_OK;
+}
+
+extern "C" cs3d_status cs3d_device_acos(cs3d_ctx* c, const double* x, double* y, int64_t n) {
+    if (!c || n < 0 || (n > 0 && (!x || !y))) return fail(CS3D_INVALID_ARG, "bad self-check arguments");
+    if (n == 0) return CS3D_OK;
+    double* d = nullptr;
+    HIPCHK(hipMalloc(&d, 2 * n * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(d, x, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        launch_acos(d, d + n, n, c->stream);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(y, d + n, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(CS3D_HIP_ERROR, hipGetErrorString(e));
     return CS3D_OK;
 }
 

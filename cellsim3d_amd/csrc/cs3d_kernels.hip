@@ -63,6 +63,24 @@ template <class T> __device__ __forceinline__ void stu(T* base, uint32_t i, T v)
     *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (uint32_t)sizeof(T)) = v;
 }
 
+// Correctly rounded float square root: the hardware estimate and one correction
+// step (the neighbours one ulp either side of the estimate, chosen by the signs of
+// the exact fma residuals x - s' s).  Inputs below 2^-96, where the residuals
+// would lose bits, are scaled by 2^32 and the result by 2^-16 (exact).  This is
+// the library's sequence without its fix-ups for 0 and infinity, which the
+// correction step already returns unchanged (as it does NaN; a negative input
+// gives NaN).  Equal to sqrtf bit for bit over all 2^32 inputs (cs3d_check_sqrt).
+__device__ __forceinline__ float sqrt_cr(float x) {
+    const bool tiny = x < 0x1p-96f;
+    const float xs = tiny ? x * 0x1p32f : x;
+    const float s = __builtin_amdgcn_sqrtf(xs);
+    const float sm = __int_as_float(__float_as_int(s) - 1), sp = __int_as_float(__float_as_int(s) + 1);
+    const float rm = fmaf(-sm, s, xs), rp = fmaf(-sp, s, xs);
+    float t = rm <= 0.f ? sm : s;
+    t = rp > 0.f ? sp : t;
+    return tiny ? t * 0x1p-16f : t;
+}
+
 // rGrowth of the population model, GPUbounce.cu:1077-1088
 template <class PP> __device__ __forceinline__ float rgrowth(PP P, int n) {
     if (P->do_pop_model == 1) {
@@ -247,7 +265,7 @@ __device__ __forceinline__ void noise3(uint32_t seed, uint32_t node, int64_t ste
             const float v2 = 2.0f * (((float)(c[2 * pr + 1] >> 9) + 0.5f) * 1.1920929e-07f) - 1.0f;
             const float s = v1 * v1 + v2 * v2;
             if (got < 3 && s > 0.0f && s < 1.0f) {
-                const float f = sqrtf(-2.0f * ln01(s) / s);
+                const float f = sqrt_cr(-2.0f * ln01(s) / s);
                 z[got++] = v1 * f;
                 if (got < 3) z[got++] = v2 * f;
             }
@@ -433,7 +451,7 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
         }
         if (node && kick) {
             const float dt = P->dt, m = P->mass;
-            const float rdt = sqrtf(dt);
+            const float rdt = sqrt_cr(dt);
             const bool rnd = P->add_rands != 0;  // the previous step's random force (d_fRanList)
             vx = half_kick(vx, fcx, fdx, rnd ? p.FRX[gi] : 0.f, dt, rdt, m, MC);
             vy = half_kick(vy, fcy, fdy, rnd ? p.FRY[gi] : 0.f, dt, rdt, m, MC);
@@ -516,8 +534,9 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
 // the 27 search bins around the node.  Keys b*192 + j (local cell ids) are left
 // ascending in lst[0 .. n)[tid]; returns n.
 // ------------------------------------------------------------------------------
-__device__ int find_candidates(const Pool& p, const SysMeta* M, const cs3d_params* P, int c, float X, float Y,
-                               float Z, int32_t (*lst)[kSlots], int tid, uint32_t& st) {
+template <class MP, class PP>
+__device__ int find_candidates(const Pool& p, MP M, PP P, int c, float X, float Y, float Z, int32_t (*lst)[kSlots],
+                               int tid, uint32_t& st) {
     int nc = 0;
     const float rc = P->attraction_range + P->pair_skin;
     const float rc2 = rc * rc;
@@ -627,8 +646,8 @@ __device__ __forceinline__ uint32_t angle_pair(float ax, float ay, float az, flo
     const float ra2 = ax * ax + ay * ay + az * az;
     const float rb2 = bx * bx + by * by + bz * bz;
     const float inner = (float)((double)(ra2 * rb2 - idk * idk) + 1e-3);
-    const float c1 = -1.0f / sqrtf(inner);
-    const float den = sqrtf(ra2) * sqrtf(rb2);
+    const float c1 = -1.0f / sqrt_cr(inner);
+    const float den = sqrt_cr(ra2) * sqrt_cr(rb2);
     const float theta = (float)cs3d_acos((double)idk / ((double)den + 1e-3));
     const float s = ((-k) * c1) * (theta - th0);
     const float ca = idk / ra2, cb = idk / rb2;
@@ -647,7 +666,7 @@ __device__ __forceinline__ uint32_t angle_pair_l(float ax, float ay, float az, f
                                                  float bz, float rb2, float lb, float th0, float k, float* out) {
     const float idk = ax * bx + ay * by + az * bz;
     const float inner = (float)((double)(ra2 * rb2 - idk * idk) + 1e-3);
-    const float c1 = -1.0f / sqrtf(inner);
+    const float c1 = -1.0f / sqrt_cr(inner);
     const float den = la * lb;
     const float theta = (float)cs3d_acos((double)idk / ((double)den + 1e-3));
     const float s = ((-k) * c1) * (theta - th0);
@@ -665,7 +684,7 @@ __device__ __forceinline__ void node_normal(const float* sx, const float* sy, co
     const float Ax = sx[N2] - sx[N1], Ay = sy[N2] - sy[N1], Az = sz[N2] - sz[N1];
     const float Bx = sx[N3] - sx[N1], By = sy[N3] - sy[N1], Bz = sz[N3] - sz[N1];
     const float cx = Ay * Bz - Az * By, cy = Az * Bx - Ax * Bz, cz = Ax * By - Ay * Bx;
-    const float inv = 1.0f / sqrtf(cx * cx + cy * cy + cz * cz);
+    const float inv = 1.0f / sqrt_cr(cx * cx + cy * cy + cz * cz);
     const float sg = (float)mesh->sign[tid];
     nx = sg * (inv * cx);
     ny = sg * (inv * cy);
@@ -778,7 +797,7 @@ __device__ __forceinline__ int accept_forces(const int32_t* cand, int tid, int n
         stu(rec, (uint32_t)(np * kSlots + tid), key);
         ++np;
         if (CON && R2 < k.ar2) {
-            const float Rl = sqrtf(R2);
+            const float Rl = sqrt_cr(R2);
             if (Rl < k.ar) {
                 cfx += k.ka * (k.ar - Rl) / Rl * dX;
                 cfy += k.ka * (k.ar - Rl) / Rl * dY;
@@ -816,7 +835,7 @@ __global__ __launch_bounds__(192) void k_search(Pool p, Range r, int32_t always)
     const int tid = threadIdx.x;
     const bool node = tid < kNodes;
     if (r.one_sys && !always) {  // one system: its flag decides for every workgroup
-        const SysMeta* M0 = p.meta + r.sys_lo;
+        const CMeta M0 = cmeta(p.meta + r.sys_lo);
         if (!M0->need_search || !sys_live(M0)) return;
     }
     // a multi-system step walks the compact list of live slots (k_live)
@@ -825,16 +844,19 @@ __global__ __launch_bounds__(192) void k_search(Pool p, Range r, int32_t always)
         const int slot = cell_slot<kLive>(p, r, it);
         const int s = sys_of(p, r, slot);
         if (s < 0) continue;
+        // metadata and parameters through scalar loads: this kernel writes none of
+        // the fields it reads (only status bits, by atomics)
         SysMeta* M = p.meta + s;
-        if (!sys_live(M) || !(always || M->need_search)) continue;
-        const int c = slot - M->base;
-        if (c >= M->n) continue;
-        const cs3d_params* P = p.prm + s;
+        const CMeta MC = cmeta(M);
+        const CParams P = cparams(p.prm + s);
+        if (!sys_live(MC) || !(always || MC->need_search)) continue;
+        const int c = slot - MC->base;
+        if (c >= MC->n) continue;
         const size_t cb = (size_t)slot * kSlots;  // the cell's first node (uniform)
         const uint32_t ln = lane_index();
         float X = ldu(p.X + cb, ln), Y = ldu(p.Y + cb, ln), Z = ldu(p.Z + cb, ln);
         bool reset = true;
-        if (!always && M->need_search == 2) {
+        if (!always && MC->need_search == 2) {
             // Partial search, in search positions.  The cells listed in RSL (RESET: a
             // node of the cell would leave the skin, or the cell divided) take their
             // current positions as search positions; every other node is within
@@ -853,12 +875,12 @@ __global__ __launch_bounds__(192) void k_search(Pool p, Range r, int32_t always)
             if (!own) {
                 const float* ob = p.SBOX + (int64_t)slot * 6;
                 const float b[6] = {ob[0], ob[1], ob[2], ob[3], ob[4], ob[5]};
-                const int nr = min(M->n_reset, kMaxPartial);
+                const int nr = min(MC->n_reset, kMaxPartial);
                 const float mg = P->attraction_range + P->pair_skin;
                 bool hit = false;
                 if (tid < nr) {
-                    const int q = p.RSL[M->base + tid];
-                    const float* o = p.SBOX + (int64_t)(M->base + q) * 6;
+                    const int q = p.RSL[MC->base + tid];
+                    const float* o = p.SBOX + (int64_t)(MC->base + q) * 6;
                     hit = b[0] - mg <= o[1] && o[0] <= b[1] + mg && b[2] - mg <= o[3] && o[2] <= b[3] + mg &&
                           b[4] - mg <= o[5] && o[4] <= b[5] + mg;
                 }
@@ -879,7 +901,7 @@ __global__ __launch_bounds__(192) void k_search(Pool p, Range r, int32_t always)
         }
         if (node) {
             uint32_t st = 0;
-            const int nc = find_candidates(p, M, P, c, X, Y, Z, lst, tid, st);
+            const int nc = find_candidates(p, MC, P, c, X, Y, Z, lst, tid, st);
             int32_t* cstore = p.CAND + (size_t)slot * kPMax * kSlots;
             for (int i = 0; i < nc; ++i) stu(cstore, (uint32_t)(i * kSlots) + ln, lst[i][tid]);
             stu(p.NCAND + cb, ln, nc);
@@ -963,7 +985,7 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                     nn3[i] = N1;
                     bx[i] = sx[N1] - X; by[i] = sy[N1] - Y; bz[i] = sz[N1] - Z;
                     b2[i] = bx[i] * bx[i] + by[i] * by[i] + bz[i] * bz[i];
-                    bl[i] = sqrtf(b2[i]);
+                    bl[i] = sqrt_cr(b2[i]);
                 }
                 // ---- springs, propagate.cu:298-314
                 const float stiffness = cld(p.E + slot);
@@ -991,7 +1013,7 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
             if (node) {
                 // ---- pressure, propagate.cu:318-330 (centre of mass from the last CenterOfMass)
                 const float rx = X - cld(p.CMX + slot), ry = Y - cld(p.CMY + slot), rz = Z - cld(p.CMZ + slot);
-                const float inv = 1.0f / sqrtf(rx * rx + ry * ry + rz * rz);
+                const float inv = 1.0f / sqrt_cr(rx * rx + ry * ry + rz * rz);
                 const float g = 3.0f * cld(p.P + slot);
                 FX += g * (inv * rx);
                 FY += g * (inv * ry);
@@ -1081,7 +1103,7 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
             }
             if (fused) {
                 const float dt = P->dt, m = P->mass;
-                const float rdt = sqrtf(dt);
+                const float rdt = sqrt_cr(dt);
                 float frx = 0.f, fry = 0.f, frz = 0.f;
                 if (P->add_rands) {  // CalculateRanForce for this step (propagate.cu:667-676)
                     float zr[3];
@@ -1183,14 +1205,14 @@ __global__ __launch_bounds__(192) void k_bonded(Pool p, Range r) {
                 for (int i = 0; i < 3; ++i) {
                     const int N1 = nb[i];
                     const float dX = sx[N1] - X, dY = sy[N1] - Y, dZ = sz[N1] - Z;
-                    const float R = sqrtf(dX * dX + dY * dY + dZ * dZ);
+                    const float R = sqrt_cr(dX * dX + dY * dY + dZ * dZ);
                     const float R0 = r0[i];
                     FX += stiffness * (R - R0) / R0 * dX / R;
                     FY += stiffness * (R - R0) / R0 * dY / R;
                     FZ += stiffness * (R - R0) / R0 * dZ / R;
                 }
                 const float rx = X - p.CMX[slot], ry = Y - p.CMY[slot], rz = Z - p.CMZ[slot];
-                const float inv = 1.0f / sqrtf(rx * rx + ry * ry + rz * rz);
+                const float inv = 1.0f / sqrt_cr(rx * rx + ry * ry + rz * rz);
                 const float g = 3.0f * p.P[slot];
                 FX += g * (inv * rx);
                 FY += g * (inv * ry);
@@ -1314,7 +1336,7 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
                 // the next step's Integrate moves this node to X + v' dt with v' from the
                 // values now in V, F_con and F_dis (bit-identical to k_integrate): search
                 // again next step when that is pair_skin/2 or more from the last search
-                const float dt = P->dt, m = P->mass, rdt = sqrtf(dt);
+                const float dt = P->dt, m = P->mass, rdt = sqrt_cr(dt);
                 const bool rnd = P->add_rands != 0;
                 const float qx = X + half_kick(vix, fcX, fx, rnd ? ldu(p.FRX + cb, ln) : 0.f, dt, rdt, m, MC) * dt - xbX;
                 const float qy = Y + half_kick(viy, fcY, fy, rnd ? ldu(p.FRY + cb, ln) : 0.f, dt, rdt, m, MC) * dt - xbY;
@@ -1428,7 +1450,7 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
                         const float x1 = sx[a] - avX, y1 = sy[a] - avY, z1 = sz[a] - avZ;
                         const float x2 = sx[b] - avX, y2 = sy[b] - avY, z2 = sz[b] - avZ;
                         const float xx = y1 * z2 - z1 * y2, yy = z1 * x2 - x1 * z2, zz = x1 * y2 - y1 * x2;
-                        faceArea = (float)((double)faceArea + 0.5 * (double)sqrtf(xx * xx + yy * yy + zz * zz));
+                        faceArea = (float)((double)faceArea + 0.5 * (double)sqrt_cr(xx * xx + yy * yy + zz * zz));
                     }
                 }
             }
@@ -1496,8 +1518,8 @@ __device__ void random_vector(cs3d_ranmar_state* st, float* n) {
         x2 = 2.0f * x2 - 1.0f;
         s = x1 * x1 + x2 * x2;
     } while (s >= 1.0f);
-    n[0] = 2.0f * x1 * sqrtf(1.0f - x1 * x1 - x2 * x2);
-    n[1] = 2.0f * x2 * sqrtf(1.0f - x1 * x1 - x2 * x2);
+    n[0] = 2.0f * x1 * sqrt_cr(1.0f - x1 * x1 - x2 * x2);
+    n[1] = 2.0f * x2 * sqrt_cr(1.0f - x1 * x1 - x2 * x2);
     n[2] = 1.0f - 2.0f * (x1 * x1 + x2 * x2);
 }
 
@@ -1515,9 +1537,9 @@ __device__ void random_vector_basis(cs3d_ranmar_state* st, float* n, const float
     }
     float f = (w[0] * v[0] + w[1] * v[1] + w[2] * w[2]) / (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
     w[0] = w[0] - f * v[0]; w[1] = w[1] - f * v[1]; w[2] = w[2] - f * v[2];
-    f = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    f = sqrt_cr(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
     v[0] = v[0] / f; v[1] = v[1] / f; v[2] = v[2] / f;
-    f = sqrtf(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    f = sqrt_cr(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
     w[0] = w[0] / f; w[1] = w[1] / f; w[2] = w[2] / f;
     n[0] = Nv * v[0] + Nw * w[0];
     n[1] = Nv * v[1] + Nw * w[1];
@@ -1623,7 +1645,7 @@ __device__ void divide_system(const Pool& p, int s, int32_t mode) {
             if (tid == 0) {
                 if (P->use_div_plane_basis) random_vector_basis(&M->rng, nrm, P->div_plane_basis);
                 else random_vector(&M->rng, nrm);
-                const float nn = sqrtf(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+                const float nn = sqrt_cr(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
                 if ((double)fabsf(nn - 1) > 1e-3) bad |= CS3D_SYS_BAD_NORMAL;
                 if (ncur >= cap) { bad |= CS3D_SYS_CAPACITY; stop = 1; }
             }
@@ -2092,7 +2114,7 @@ __global__ __launch_bounds__(192) void k_velocity_ab(Pool p, Range r, int32_t wh
         if (slot - M->base >= M->n || tid >= kNodes) continue;
         const cs3d_params* P = p.prm + s;
         const int64_t gi = (int64_t)slot * kSlots + tid;
-        const float dt = P->dt, m = P->mass, rdt = sqrtf(dt);
+        const float dt = P->dt, m = P->mass, rdt = sqrt_cr(dt);
         float v[3] = {p.VX[gi], p.VY[gi], p.VZ[gi]};
         const float fc[3] = {p.FCX[gi], p.FCY[gi], p.FCZ[gi]};
         const float fd[3] = {p.FDX[gi], p.FDY[gi], p.FDZ[gi]};
@@ -2152,6 +2174,24 @@ __global__ __launch_bounds__(256) void k_pressure_reset(Pool p, Range r, const i
 }
 
 // plain streaming copy used to calibrate HBM byte counters
+// Arithmetic self-checks of the kernels' own functions (for tests): sqrt_cr against
+// sqrtf over all 2^32 inputs (count of differing results, NaNs equal), and
+// cs3d_acos of n arguments.
+__global__ __launch_bounds__(256) void k_check_sqrt(unsigned long long* bad) {
+    unsigned long long nb = 0;
+    for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < (1ull << 32);
+         u += (uint64_t)gridDim.x * blockDim.x) {
+        const float x = __uint_as_float((uint32_t)u);
+        const float a = sqrt_cr(x), b = sqrtf(x);
+        nb += !(__float_as_uint(a) == __float_as_uint(b) || (isnan(a) && isnan(b)));
+    }
+    if (nb) atomicAdd(bad, nb);
+}
+__global__ __launch_bounds__(256) void k_acos(const double* __restrict__ x, double* __restrict__ y, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = cs3d_acos(x[i]);
+}
+
 __global__ __launch_bounds__(256) void k_copy_calibrate(const float4* __restrict__ src, float4* __restrict__ dst,
                                                         int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -2301,6 +2341,12 @@ void launch_set_flags(const Pool& p, const Range& r, int rebuild, int search, hi
 }
 void launch_copy_calibrate(const void* src, void* dst, int64_t n4, hipStream_t st) {
     launch(k_copy_calibrate, dim3(2048), dim3(256), st, (const float4*)src, (float4*)dst, n4);
+}
+void launch_check_sqrt(unsigned long long* bad, hipStream_t st) {
+    launch(k_check_sqrt, dim3(16384), dim3(256), st, bad);
+}
+void launch_acos(const double* x, double* y, int64_t n, hipStream_t st) {
+    launch(k_acos, dim3(n < 1 ? 1 : (n + 255) / 256 < 16384 ? (n + 255) / 256 : 16384), dim3(256), st, x, y, n);
 }
 
 }  // namespace cs3d

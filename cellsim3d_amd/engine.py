@@ -80,6 +80,8 @@ SIGNATURES = {
     "cs3d_k_grid": ([vp, C.c_int32], st_t),
     "cs3d_k_divide": ([vp, C.c_int32, C.POINTER(C.c_int32)], st_t),
     "cs3d_copy_calibrate": ([vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)], st_t),
+    "cs3d_check_sqrt": ([vp, C.POINTER(C.c_uint64)], st_t),
+    "cs3d_device_acos": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int64], st_t),
     "cs3d_traj_open": ([C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(vp)], st_t),
     "cs3d_traj_append": ([vp, C.c_int32, C.c_int32, C.c_int32, vp, vp, vp, vp], st_t),
     "cs3d_traj_close": ([vp, C.c_int32], st_t),
@@ -238,6 +240,21 @@ class Context:
         """Captures and uploads the graph that step() replays for target, without stepping."""
         sid = target.id if isinstance(target, System) else target
         check(lib().cs3d_graph_prepare(self.h, sid), "cs3d_graph_prepare")
+
+    def check_sqrt(self):
+        """Differing results of the kernels' float square root against sqrtf over all 2^32 inputs."""
+        out = C.c_uint64(0)
+        check(lib().cs3d_check_sqrt(self.h, C.byref(out)), "cs3d_check_sqrt")
+        return int(out.value)
+
+    def device_acos(self, x):
+        """The kernels' double acos (the angle force's) of each element of x, computed on the device."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        dp = C.POINTER(C.c_double)
+        check(lib().cs3d_device_acos(self.h, x.ctypes.data_as(dp), y.ctypes.data_as(dp), x.size),
+              "cs3d_device_acos")
+        return y
 
     def copy_calibrate(self, nbytes, reps=10):
         """Average ms of a plain copy kernel over nbytes (reads nbytes, writes nbytes)."""

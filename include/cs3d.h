@@ -383,6 +383,16 @@ cs3d_status cs3d_k_divide(cs3d_ctx* ctx, cs3d_sys sys, int32_t* n_events);
  * Allocates and frees its buffers. */
 cs3d_status cs3d_copy_calibrate(cs3d_ctx* ctx, int64_t bytes, int32_t reps, double* avg_ms);
 
+/* ---- arithmetic self-checks (no counterpart in the reference; for tests) --
+ * cs3d_check_sqrt: the kernels' correctly rounded float square root (sqrt_cr)
+ * against sqrtf over all 2^32 inputs on the context's device; *mismatches is
+ * the number of differing results (NaN equals NaN).
+ * cs3d_device_acos: the kernels' double acos of the angle force
+ * (cellsim3d_amd/csrc/cs3d_acos.h, propagate.cu:36) of n host arguments, into
+ * host y.  Both allocate and free device buffers. */
+cs3d_status cs3d_check_sqrt(cs3d_ctx* ctx, uint64_t* mismatches);
+cs3d_status cs3d_device_acos(cs3d_ctx* ctx, const double* x, double* y, int64_t n);
+
 /* ---- trajectory (binary format of GPUbounce.cu:1007-1018, 2446-2477) ----- */
 typedef struct cs3d_traj cs3d_traj;
 cs3d_status cs3d_traj_open(const char* path, int32_t max_cells, int32_t variable_stiffness,
