@@ -60,6 +60,14 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def graph_steps_for(k):
+    """Steps per captured graph for a timed run of k steps: k itself up to 100, else
+    the largest divisor of k up to 100 (so that k is a whole number of replays)."""
+    if k <= 100:
+        return max(1, k)
+    return max(d for d in range(1, 101) if k % d == 0)
+
+
 def load_traffic():
     """HBM bytes per launch measured with rocprofv3 counters (committed summary)."""
     path = os.path.join(ROOT, "profiles", "traffic_single_tissue_2000.json")
@@ -255,7 +263,7 @@ def main():
     from cellsim3d_amd import abi, configs, engine, mesh
 
     cfg = configs.single_tissue(args.cells, args.phi, seed=1, grid_interval=args.grid_interval,
-                                steps=args.steps + args.warmup)
+                                steps=args.steps + max(args.warmup, graph_steps_for(args.steps)) + 100)
     if args.pair_skin is not None:
         cfg["params"].pair_skin = args.pair_skin
     tables = mesh.tables(cfg["mesh"])
@@ -268,10 +276,18 @@ def main():
     log("bench: %d cells, phi %.2f, grid every %d steps" % (n, args.phi, args.grid_interval))
     ctx = engine.Context(gm, capacity_cells=n, voxel_budget=1 << 24)
     s = ctx.add_system(cfg["params"], cfg["X"], cfg["Y"], cfg["Z"], n, max_cells=n, voxels=1 << 18)
-    ctx.prepare_graph(s)  # capture, instantiate and upload before any timed step
-    ctx.step(args.warmup, s)
+    # the timed K steps replay one captured graph of G steps K/G times: G = K when K
+    # is at most 100, else the largest divisor of K up to 100.  The graph is
+    # captured, instantiated and uploaded before the warm-up, and the warm-up runs
+    # at least one replay of it, so that the timed region holds warm replays only
+    # (W warm-up steps, rounded up to a multiple of G)
+    gs = graph_steps_for(args.steps)
+    ctx.set_graph_steps(gs)
+    ctx.prepare_graph(s)
+    warm = -(-max(args.warmup, gs) // gs) * gs
+    ctx.step(warm, s)
     ctx.sync()
-    log("bench: graph prepared, warm-up done")
+    log("bench: graph of %d steps prepared, %d warm-up steps done" % (gs, warm))
     t0 = time.perf_counter()
     ctx.step(args.steps, s)
     ctx.sync()
@@ -349,6 +365,8 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "warmup_steps_run": warm,
+        "graph_steps": gs,
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",

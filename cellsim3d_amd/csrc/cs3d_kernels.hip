@@ -919,11 +919,13 @@ __global__ __launch_bounds__(192) void k_search(Pool p, Range r, int32_t always)
 // partners), M_CON / M_INIT (FC, accepted partners), M_BONDED (spring +
 // pressure + angle -> FC), M_PAIRS (accepted partners only).
 // ------------------------------------------------------------------------------
-// 4 waves per SIMD: with the pipelined candidate loop the allocator would otherwise
-// take 132 VGPRs (3 waves); at this bound it fits 127 without spills
+// 6 waves per SIMD (80 VGPRs, a few registers spilled to scratch): 2048 workgroups
+// fit on the chip at once, so 2000 cells run in one round instead of 1.5.  Measured
+// against 4 waves (100 VGPRs, no spills): this kernel 28.0 instead of 29.5 us, and
+// k_dis_post, which reads its outputs next, 11.5 instead of 12.8 us.
 // kLive: walk Pool::LIVE (a multi-system step) instead of the range's slots
 template <int kLive>
-__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_con_dis(Pool p, Range r, int32_t mode, const float* __restrict__ rX,
+__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_con_dis(Pool p, Range r, int32_t mode, const float* __restrict__ rX,
                                                  const float* __restrict__ rY, const float* __restrict__ rZ,
                                                  const float* __restrict__ rVX, const float* __restrict__ rVY,
                                                  const float* __restrict__ rVZ, const float* __restrict__ rBB) {
