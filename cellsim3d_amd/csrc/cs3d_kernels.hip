@@ -292,10 +292,10 @@ template <class MP, class PP> __device__ __forceinline__ VoxRange vox_range(cons
 }
 
 // voxel range of a cell's bbox into VRANGE (lanes 0..5 of the caller store)
-__device__ __forceinline__ void store_vrange(const Pool& p, int64_t slot, const VoxRange& v, int lane) {
+__device__ __forceinline__ void store_vrange(int32_t* vrange, int64_t slot, const VoxRange& v, int lane) {
     if (lane < 6) {
         const int w[6] = {v.sx, v.ex, v.sy, v.ey, v.sz, v.ez};
-        p.VRANGE[slot * 6 + lane] = w[lane];
+        vrange[slot * 6 + lane] = w[lane];
     }
 }
 
@@ -426,20 +426,21 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
         if (!sys_live(MC)) continue;
         const int c = slot - MC->base;
         if (c >= MC->n) continue;
+        const CPool q = pool_late();  // pointers read again where used (see pool_late)
         if ((flags & I_GRIDCOUNT) && MC->rebuild && !(MC->status & CS3D_SYS_GRID_TOO_LARGE)) {
             // the grid was rebuilt at the end of the last step: this cell's new bbox
             // becomes the one the contact rule uses, and its voxels are counted
             // (makeNNlist's atomicAdd, propagatebound.cu:246-250) to catch overflow
-            const float* bn = p.BBOXN + (int64_t)slot * 6;
+            const float* bn = q->BBOXN + (int64_t)slot * 6;
             float b[6];
 #pragma unroll
             for (int a = 0; a < 6; ++a) b[a] = bn[a];
-            if (tid < 6) p.BBOX[(int64_t)slot * 6 + tid] = b[tid];
+            if (tid < 6) q->BBOX[(int64_t)slot * 6 + tid] = b[tid];
             const VoxRange v = vox_range(b, MC, P);
-            store_vrange(p, slot, v, tid);
+            store_vrange(q->VRANGE, slot, v, tid);
             const int nx = v.ex - v.sx + 1, ny = v.ey - v.sy + 1, nz = v.ez - v.sz + 1;
             const int total = (nx > 0 && ny > 0 && nz > 0) ? nx * ny * nz : 0;
-            int32_t* counts = p.vox + MC->vbase;
+            int32_t* counts = q->vox + MC->vbase;
             const int xd = MC->dims[0], yd = MC->dims[1], cap = MC->vcap;
             bool over = false;
             for (int t = tid; t < total; t += blockDim.x) {
@@ -453,17 +454,17 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             const float dt = P->dt, m = P->mass;
             const float rdt = sqrt_cr(dt);
             const bool rnd = P->add_rands != 0;  // the previous step's random force (d_fRanList)
-            vx = half_kick(vx, fcx, fdx, rnd ? p.FRX[gi] : 0.f, dt, rdt, m, MC);
-            vy = half_kick(vy, fcy, fdy, rnd ? p.FRY[gi] : 0.f, dt, rdt, m, MC);
-            vz = half_kick(vz, fcz, fdz, rnd ? p.FRZ[gi] : 0.f, dt, rdt, m, MC);
+            vx = half_kick(vx, fcx, fdx, rnd ? q->FRX[gi] : 0.f, dt, rdt, m, MC);
+            vy = half_kick(vy, fcy, fdy, rnd ? q->FRY[gi] : 0.f, dt, rdt, m, MC);
+            vz = half_kick(vz, fcz, fdz, rnd ? q->FRZ[gi] : 0.f, dt, rdt, m, MC);
             x = x + vx * dt;
             y = y + vy * dt;
             z = z + vz * dt;
-            p.VAX[gi] = vx; p.VAY[gi] = vy; p.VAZ[gi] = vz;
-            if (flags & I_WRITEV) { p.VX[gi] = vx; p.VY[gi] = vy; p.VZ[gi] = vz; }
-            p.X[gi] = x; p.Y[gi] = y; p.Z[gi] = z;
+            q->VAX[gi] = vx; q->VAY[gi] = vy; q->VAZ[gi] = vz;
+            if (flags & I_WRITEV) { q->VX[gi] = vx; q->VY[gi] = vy; q->VZ[gi] = vz; }
+            q->X[gi] = x; q->Y[gi] = y; q->Z[gi] = z;
         }
-        if (node && (flags & I_CLEAR)) p.bins[MC->bbase + bin_of(MC, x, y, z)] = 0;
+        if (node && (flags & I_CLEAR)) q->bins[MC->bbase + bin_of(MC, x, y, z)] = 0;
         if ((flags & I_BINS) && (!(flags & I_IFSEARCH) || MC->need_search)) {
             // insertion into the search bins: the lanes of a wave that share a bin
             // take one atomic between them (issued by the group's first lane), and
@@ -471,8 +472,8 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             // partial search (after divisions) works on search positions: the nodes of
             // cells that did not divide are binned at their last search position
             float qx = x, qy = y, qz = z;
-            if ((flags & I_IFSEARCH) && MC->need_search == 2 && node && !p.RESET[slot]) {
-                qx = p.XBX[gi]; qy = p.XBY[gi]; qz = p.XBZ[gi];
+            if ((flags & I_IFSEARCH) && MC->need_search == 2 && node && !q->RESET[slot]) {
+                qx = q->XBX[gi]; qy = q->XBY[gi]; qz = q->XBZ[gi];
             }
             const int b = node ? bin_of(MC, qx, qy, qz) : -1;
             const int lane = tid & 63;
@@ -489,13 +490,13 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
                 }
                 todo &= ~same;
             }
-            int32_t* counts = p.bins + MC->bbase;
+            int32_t* counts = q->bins + MC->bbase;
             int k0 = 0;
             if (node && lane == leader_of) k0 = atomicAdd(&counts[b], group);
             const int k = __shfl(k0, leader_of) + rank;
             if (node) {
                 if (k < kBCap) {
-                    float4* ent = reinterpret_cast<float4*>(p.bins + MC->ebase);
+                    float4* ent = reinterpret_cast<float4*>(q->bins + MC->ebase);
                     ent[(int64_t)b * kBCap + k] = make_float4(qx, qy, qz, __int_as_float(c * kSlots + tid));
                 } else {
                     flag(M, CS3D_SYS_BIN_OVERFLOW);
@@ -504,13 +505,13 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
         }
         if (tid == 0 && (flags & I_PRESSURE)) {
             const int64_t stepno = step_override >= 0 ? step_override : MC->step + 1;
-            const float inc = rgrowth(P, MC->n) * p.GS[slot];
+            const float inc = rgrowth(P, MC->n) * q->GS[slot];
             const float mx = P->max_pressure;
-            float pr = p.P[slot];
+            float pr = q->P[slot];
             if (P->use_different_stiffnesses) {
                 if (stepno < P->phase_count) {
                     if (pr < mx) pr += inc;
-                } else if (p.E[slot] == P->stiffness1) {
+                } else if (q->E[slot] == P->stiffness1) {
                     if (pr < mx) pr += inc;
                 } else if (pr != 0.f) {
                     pr = 0.f;
@@ -518,12 +519,12 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             } else if (pr < mx) {
                 pr += inc;
             }
-            p.P[slot] = pr;
+            q->P[slot] = pr;
         }
-        if (kick && (flags & I_IFSEARCH) && MC->need_search == 2 && p.RESET[slot]) {
+        if (kick && (flags & I_IFSEARCH) && MC->need_search == 2 && q->RESET[slot]) {
             // a cell that divided or left the skin in the last step publishes the box of
             // its new positions: the partial search re-searches the cells within reach
-            box_of_cell(x, y, z, node, ibox, p.SBOX + (int64_t)slot * 6);
+            box_of_cell(x, y, z, node, ibox, q->SBOX + (int64_t)slot * 6);
         }
     }
 }
@@ -1333,16 +1334,17 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
             fx = fx - P->gamma_visc * vix;  // drag, propagate.cu:658
             fy = fy - P->gamma_visc * viy;
             fz = fz - P->gamma_visc * viz;
-            stu(p.FDX + cb, ln, fx); stu(p.FDY + cb, ln, fy); stu(p.FDZ + cb, ln, fz);
+            const CPool q = pool_late();  // pointers read again where used (see pool_late)
+            stu(q->FDX + cb, ln, fx); stu(q->FDY + cb, ln, fy); stu(q->FDZ + cb, ln, fz);
             if (fused) {
                 // the next step's Integrate moves this node to X + v' dt with v' from the
                 // values now in V, F_con and F_dis (bit-identical to k_integrate): search
                 // again next step when that is pair_skin/2 or more from the last search
                 const float dt = P->dt, m = P->mass, rdt = sqrt_cr(dt);
                 const bool rnd = P->add_rands != 0;
-                const float qx = X + half_kick(vix, fcX, fx, rnd ? ldu(p.FRX + cb, ln) : 0.f, dt, rdt, m, MC) * dt - xbX;
-                const float qy = Y + half_kick(viy, fcY, fy, rnd ? ldu(p.FRY + cb, ln) : 0.f, dt, rdt, m, MC) * dt - xbY;
-                const float qz = Z + half_kick(viz, fcZ, fz, rnd ? ldu(p.FRZ + cb, ln) : 0.f, dt, rdt, m, MC) * dt - xbZ;
+                const float qx = X + half_kick(vix, fcX, fx, rnd ? ldu(q->FRX + cb, ln) : 0.f, dt, rdt, m, MC) * dt - xbX;
+                const float qy = Y + half_kick(viy, fcY, fy, rnd ? ldu(q->FRY + cb, ln) : 0.f, dt, rdt, m, MC) * dt - xbY;
+                const float qz = Z + half_kick(viz, fcZ, fz, rnd ? ldu(q->FRZ + cb, ln) : 0.f, dt, rdt, m, MC) * dt - xbZ;
                 const float h = 0.5f * P->pair_skin;
                 far = !(qx * qx + qy * qy + qz * qz <= h * h);
             }
@@ -1353,12 +1355,12 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
         if (far) moved = 1;
         __syncthreads();
         if (fused && tid == 0 && moved) {
-            p.RESET[slot] = 1;
+            pool_late()->RESET[slot] = 1;
             if (__hip_atomic_load(&M->search_next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
                 atomicOr(&M->search_next, 1);
             if (__hip_atomic_load(&M->n_reset, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= kMaxPartial) {
                 const int k = atomicAdd(&M->n_reset, 1);
-                if (k < kMaxPartial) p.RSL[MC->base + k] = c;
+                if (k < kMaxPartial) pool_late()->RSL[MC->base + k] = c;
             }
         }
         if (clear_bins && (mode != M_STEP || MC->need_search)) {
@@ -1368,7 +1370,7 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
             const int64_t nb = min((int64_t)MC->bbudget, (int64_t)MC->bdims[0] * MC->bdims[1] * MC->bdims[2]);
             const int64_t per = (nb + MC->n - 1) / MC->n;
             const int64_t lo = (int64_t)c * per, hi = min(nb, lo + per);
-            int32_t* counts = p.bins + MC->bbase;
+            int32_t* counts = pool_late()->bins + MC->bbase;
             for (int64_t b = lo + tid; b < hi; b += blockDim.x) counts[b] = 0;
         }
         if (fused) {  // this cell's bbox at the end of the step (bounding_boxes, propagatebound.cu:10-141)
@@ -1412,18 +1414,22 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
                 }
                 if (tid == 0) {
                     cm[0] = a / 180.0f; cm[1] = b / 180.0f; cm[2] = d / 180.0f;
-                    p.CMX[slot] = cm[0]; p.CMY[slot] = cm[1]; p.CMZ[slot] = cm[2];
+                    const CPool q = pool_late();
+                    q->CMX[slot] = cm[0]; q->CMY[slot] = cm[1]; q->CMZ[slot] = cm[2];
                 }
             }
             if (fused && tid < 6) {
                 float b = bpart[0][tid];
                 b = (tid & 1) ? fmaxf(b, bpart[1][tid]) : fminf(b, bpart[1][tid]);
                 b = (tid & 1) ? fmaxf(b, bpart[2][tid]) : fminf(b, bpart[2][tid]);
-                p.BBOXN[(int64_t)slot * 6 + tid] = b;
+                pool_late()->BBOXN[(int64_t)slot * 6 + tid] = b;
             }
             __syncthreads();
         } else if (do_vol) {
-            if (tid == 0) { cm[0] = p.CMX[slot]; cm[1] = p.CMY[slot]; cm[2] = p.CMZ[slot]; }
+            if (tid == 0) {
+                const CPool q = pool_late();
+                cm[0] = q->CMX[slot]; cm[1] = q->CMY[slot]; cm[2] = q->CMZ[slot];
+            }
             __syncthreads();
         }
         if (do_vol) {
@@ -1472,17 +1478,17 @@ __global__ __launch_bounds__(192) void k_dis_post(Pool p, Range r, int32_t mode,
                 }
                 if (tid == 0) {  // volume.cu:129-161
                     const float volume = (float)((double)a / 6.0);
-                    p.VOL[slot] = volume;
+                    pool_late()->VOL[slot] = volume;
                     if (!isfinite(volume)) st |= CS3D_SYS_VOLUME_NONFINITE;
                     int8_t fl = volume > P->division_vol ? 1 : 0;
                     if (sph) {
-                        p.AREA[slot] = b;
+                        pool_late()->AREA[slot] = b;
                         const float cbr = cbrtf(volume);
                         const float psi = (float)(4.835975862049408 * (double)cbr * (double)cbr / (double)b);
                         if ((double)fabsf(1.0f - psi) > 0.05) fl = 0;
                     }
-                    if (P->use_different_stiffnesses && P->recalc_r0 && p.E[slot] != P->stiffness1) fl = 0;
-                    p.FLAG[slot] = fl;
+                    if (P->use_different_stiffnesses && P->recalc_r0 && pool_late()->E[slot] != P->stiffness1) fl = 0;
+                    pool_late()->FLAG[slot] = fl;
                 }
             }
         }
@@ -2076,7 +2082,7 @@ __global__ __launch_bounds__(192) void k_grid_adopt(Pool p, Range r) {
         if (!M->rebuild || slot - M->base >= M->n) continue;
         if (threadIdx.x < 6) p.BBOX[(int64_t)slot * 6 + threadIdx.x] = p.BBOXN[(int64_t)slot * 6 + threadIdx.x];
         if (!(M->status & CS3D_SYS_GRID_TOO_LARGE))
-            store_vrange(p, slot, vox_range(p.BBOXN + (int64_t)slot * 6, M, p.prm + s), threadIdx.x);
+            store_vrange(p.VRANGE, slot, vox_range(p.BBOXN + (int64_t)slot * 6, M, p.prm + s), threadIdx.x);
     }
 }
 
