@@ -27,6 +27,10 @@ Further objects in the same line:
                    sample of the same tissues run one after another (plain path)
   serving          256 requests of a Poisson stream through the
                    continuous-batching loop, admitted against a 200 GB budget
+  c1_plain_small   8 cells, 10^4 steps with division at 2 V0, on the plain
+                   path at full length; the oracle on the run's first steps
+  c3_colony        one cell grown to about 20,000 by division; the oracle
+                   stepping the colony's final positions
 """
 import argparse
 import json
@@ -213,9 +217,132 @@ def bonded_roofline(gm, n_cells=20000, log=None):
     return res
 
 
+def c1_workload(gm, log, cpu=True, cpu_seconds=10.0, chunk=500):
+    """The plain small configuration at its stated length (8 cells on a 2x2x2
+    lattice, 10^4 steps, division at 2 V0): the GPU's drop-in path (one system, its
+    own captured graph) over the whole run, timed per chunk; the oracle on the
+    same run from its start for about cpu_seconds.  Both runs are bit-identical,
+    so the oracle's node-updates over its prefix are the device-counted ones."""
+    from cellsim3d_amd import abi, configs, engine
+    cfg = configs.plain_small(seed=1, steps=10000)
+    p, n, mc, steps = cfg["params"], cfg["n"], cfg["max_cells"], cfg["steps"]
+    marks = []  # (steps, seconds, cell-steps) at chunk ends
+    with engine.Context(gm, capacity_cells=mc + 8, voxel_budget=1 << 24) as ctx:
+        s = ctx.add_system(p, cfg["X"], cfg["Y"], cfg["Z"], n, max_cells=mc, voxels=1 << 16)
+        ctx.set_graph_steps(100)
+        ctx.prepare_graph(s)
+        ctx.sync()
+        t = 0.0
+        done = 0
+        while done < steps:
+            k = min(chunk, steps - done)
+            t0 = time.perf_counter()
+            s.step(k)
+            ctx.sync()
+            t += time.perf_counter() - t0
+            done += k
+            marks.append((done, t, int(ctx.work(s.id, 1)[0])))
+        n_end, _, status = s.info()
+        if status & abi.SYS_FATAL_MASK:
+            raise SystemExit("C1: system status 0x%x" % status)
+    work = marks[-1][2] * 180.0
+    out = {"workload": "8 unit-radius C180 cells on a 2x2x2 lattice (spacing 2.2, +-0.01 jitter), 10^4 steps, "
+                       "dt 1e-4, FP32, division at 2 V0 (8 -> %d cells)" % n_end,
+           "gpu_plain": {"seconds": round(t, 4), "ms_per_step": t / steps * 1e3, "node_updates": work,
+                         "node_updates_per_s": work / t, "final_cells": int(n_end)}}
+    log("bench: C1 GPU %.3f s for %d steps (%d cells at the end)" % (t, steps, n_end))
+    if cpu:
+        from oracle import oracle as O
+        from cellsim3d_amd import mesh as meshmod
+        om = O.build_mesh(meshmod.tables(cfg["mesh"]), configs.rest_positions(cfg["mesh"]))
+        sim = O.Sim(p, om, cfg["X"], cfg["Y"], cfg["Z"], n, max_cells=mc, budget=1 << 16)
+        sim.init()
+        tc = 0.0
+        done = 0
+        while done < steps and tc < cpu_seconds:
+            t0 = time.perf_counter()
+            sim.step(chunk)
+            tc += time.perf_counter() - t0
+            done += chunk
+        mk = next(m for m in marks if m[0] == done)
+        cw = mk[2] * 180.0
+        threads = O.get_threads()
+        O.set_threads(1)
+        try:
+            sim1 = O.Sim(p, om, cfg["X"], cfg["Y"], cfg["Z"], n, max_cells=mc, budget=1 << 16)
+            sim1.init()
+            k1 = min(done, 2000)
+            t0 = time.perf_counter()
+            sim1.step(k1)
+            t1 = time.perf_counter() - t0
+        finally:
+            O.set_threads(threads)
+        w1 = next(m for m in marks if m[0] == k1)[2] * 180.0
+        out["cpu_oracle"] = {"cores": threads, "sample": "the first %d steps of the same run (%d cells at its end)"
+                                                         % (done, sim.n),
+                             "seconds": round(tc, 3), "node_updates_per_s": cw / tc,
+                             "gpu_same_steps_seconds": round(mk[1], 4), "gpu_over_cpu_same_steps": round(tc / mk[1], 1),
+                             "single_thread": {"sample": "the first %d steps" % k1,
+                                               "node_updates_per_s": w1 / t1}}
+        log("bench: C1 oracle %d steps in %.2f s" % (done, tc))
+    return out
+
+
+def c3_workload(log, cpu=True, cpu_seconds=10.0):
+    """The growing colony (one cell to about 20,000 by division) on the GPU, and
+    the oracle stepping the colony's final state for about cpu_seconds: the CPU
+    rate at the final size, beside the GPU's over its last chunk."""
+    import numpy as np
+    from cellsim3d_amd import abi, colony, configs
+    from cellsim3d_amd import mesh as meshmod
+    res = colony.run(keep_state=cpu, log=log)
+    st = res.pop("state", None)
+    log("bench: C3 colony %d cells in %d steps, %.1f s" % (res["final_cells"], res["steps"], res["wall_s"]))
+    if cpu and st:
+        from oracle import oracle as O
+        m = st["mesh"]
+        om = O.build_mesh(meshmod.tables(m), configs.rest_positions(m))
+        n = st["n"]
+        cap = n + 64
+        sim = O.Sim(st["params"], om, st["X"][:n * 192], st["Y"][:n * 192], st["Z"][:n * 192], n,
+                    max_cells=cap, budget=1 << 17)
+        sim.set(abi.F_GROWTH_SCALE, st["growth_scale"][:cap].astype(np.float32))
+        sim.init()
+        sim.step(1)  # warm-up
+        t0 = time.perf_counter()
+        k = 0
+        cell_steps = 0
+        while time.perf_counter() - t0 < cpu_seconds:
+            c0 = sim.n
+            sim.step(1)
+            cell_steps += c0
+            k += 1
+        tc = time.perf_counter() - t0
+        threads = O.get_threads()
+        O.set_threads(1)
+        try:
+            c0 = sim.n
+            t1 = time.perf_counter()
+            sim.step(1)
+            t1 = time.perf_counter() - t1
+        finally:
+            O.set_threads(threads)
+        res["cpu_oracle_at_final_size"] = {
+            "cores": threads, "sample": "%d steps from the positions of the colony's final state (%d cells; velocities "
+                                        "and pressures from rest), after one warm-up step" % (k, n),
+            "node_updates_per_s": cell_steps * 180.0 / tc,
+            "single_thread": {"sample": "1 step", "node_updates_per_s": c0 * 180.0 / t1}}
+        res["gpu_over_cpu_at_final_size"] = round(res["final_chunk"]["node_updates_per_s"] /
+                                                  (cell_steps * 180.0 / tc), 1)
+        log("bench: C3 oracle %d steps at %d cells in %.2f s" % (k, n, tc))
+    return res
+
+
 def measure_extras(args, gm, log):
     from cellsim3d_amd import configs, ensemble, serving
     out = {"bonded_roofline": bonded_roofline(gm, log=log)}
+    out["c1_plain_small"] = c1_workload(gm, log, cpu=not args.no_cpu)
+    out["c3_colony"] = c3_workload(log, cpu=not args.no_cpu)
     g, tissues = ensemble.build(256, 1, 1, 512, args.ensemble_steps, 10)
     wg, gs, _ = ensemble.autotune(g, tissues, probe_steps=50)
     out["ensemble"] = ensemble.measure(g, tissues, args.ensemble_steps, wg, gs, args.plain_every, log)

@@ -141,6 +141,15 @@ __device__ __forceinline__ uint32_t lane_index() {
     return t;
 }
 
+// A value read from LDS that the compiler may not replace by a register copy of
+// what was stored there: reading a node's own position again where it is needed
+// ends the live range of the register that held it.
+__device__ __forceinline__ float lds_reread(const float* a, int i) {
+    float v = a[i];
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 // XCD-aware first cell of a workgroup.  The dispatcher hands workgroup v to XCD
 // v mod 8; this maps the 8 XCDs to 8 contiguous runs of workgroup indices, so the
 // cells one XCD works on are neighbours in index and, for tissues placed in
@@ -686,7 +695,7 @@ __device__ __forceinline__ void node_normal(const float* sx, const float* sy, co
     const float Bx = sx[N3] - sx[N1], By = sy[N3] - sy[N1], Bz = sz[N3] - sz[N1];
     const float cx = Ay * Bz - Az * By, cy = Az * Bx - Ax * Bz, cz = Ax * By - Ay * Bx;
     const float inv = 1.0f / sqrt_cr(cx * cx + cy * cy + cz * cz);
-    const float sg = (float)mesh->sign[tid];
+    const float sg = (float)ldu(mesh->sign, lane_index());
     nx = sg * (inv * cx);
     ny = sg * (inv * cy);
     nz = sg * (inv * cz);
@@ -920,10 +929,15 @@ __global__ __launch_bounds__(192) void k_search(Pool p, Range r, int32_t always)
 // partners), M_CON / M_INIT (FC, accepted partners), M_BONDED (spring +
 // pressure + angle -> FC), M_PAIRS (accepted partners only).
 // ------------------------------------------------------------------------------
-// 6 waves per SIMD (80 VGPRs, a few registers spilled to scratch): 2048 workgroups
-// fit on the chip at once, so 2000 cells run in one round instead of 1.5.  Measured
-// against 4 waves (100 VGPRs, no spills): this kernel 28.0 instead of 29.5 us, and
-// k_dis_post, which reads its outputs next, 11.5 instead of 12.8 us.
+// 6 waves per SIMD (80 VGPRs): 2048 workgroups fit on the chip at once, so 2000
+// cells run in one round instead of 1.5.  Measured against 4 waves (100 VGPRs):
+// this kernel 28.0 instead of 29.5 us, and k_dis_post, which reads its outputs
+// next, 11.5 instead of 12.8 us, even with 18 values spilled to scratch then.  The
+// spills are gone: the angles come before the springs, so the spring sums are not
+// live across them; the node's position is read again from LDS after them; the
+// neighbour, rest-length, rest-angle and gather tables are read at a uniform base
+// plus the opaque lane offset where they are used, not through per-lane 64-bit
+// pointers kept across the cell loop.
 // kLive: walk Pool::LIVE (a multi-system step) instead of the range's slots
 template <int kLive>
 __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_con_dis(Pool p, Range r, int32_t mode, const float* __restrict__ rX,
@@ -974,46 +988,48 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
         uint32_t st = 0;
         const bool angles = forces && P->constrain_angles;
         float FX = 0.f, FY = 0.f, FZ = 0.f;
-        int nn3[3] = {0, 0, 0};
         if (forces) {
             if (node && (isnan(X) || isnan(Y) || isnan(Z))) st |= CS3D_SYS_NAN;  // propagate.cu:241-248
             if (node) {
                 // the node's three bonds: vectors, squared lengths and lengths, used by the
-                // springs (propagate.cu:298-314) and by the node's own three angles
-                // (propagate.cu:26-60), which compute the same quantities from the same vectors
+                // node's own three angles (propagate.cu:26-60) and by the springs
+                // (propagate.cu:298-314), which compute the same quantities from the same
+                // vectors.  The angles write to LDS only, so evaluating them before the
+                // springs leaves every sum in its reference order.
                 float bx[3], by[3], bz[3], b2[3], bl[3];
+                const uint32_t lb = lane_index();
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
-                    const int N1 = mesh->nn[i * kSlots + tid];
-                    nn3[i] = N1;
+                    const int N1 = ldu(mesh->nn, (uint32_t)(i * kSlots) + lb);
                     bx[i] = sx[N1] - X; by[i] = sy[N1] - Y; bz[i] = sz[N1] - Z;
                     b2[i] = bx[i] * bx[i] + by[i] * by[i] + bz[i] * bz[i];
                     bl[i] = sqrt_cr(b2[i]);
+                }
+                if (angles) {  // each angle once (propagate.cu:26-60): (n0,n1), (n1,n2), (n0,n2)
+                    const float k = P->angle_k;
+                    const uint32_t lt = lane_index();
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        const int a = q == 1 ? 1 : 0;
+                        const int b = q == 0 ? 1 : 2;
+                        st |= angle_pair_l(bx[a], by[a], bz[a], b2[a], bl[a], bx[b], by[b], bz[b], b2[b], bl[b],
+                                           ldu(mesh->theta0, 3 * lt + q), k, u.angF + (tid * 3 + q) * 6);
+                    }
                 }
                 // ---- springs, propagate.cu:298-314
                 const float stiffness = cld(p.E + slot);
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
                     const float R = bl[i];
-                    const float R0 = mesh->R0[i * kSlots + tid];
+                    const float R0 = ldu(mesh->R0, (uint32_t)(i * kSlots) + lane_index());
                     FX += stiffness * (R - R0) / R0 * bx[i] / R;
                     FY += stiffness * (R - R0) / R0 * by[i] / R;
                     FZ += stiffness * (R - R0) / R0 * bz[i] / R;
                 }
-                if (angles) {  // each angle once (propagate.cu:26-60): (n0,n1), (n1,n2), (n0,n2)
-                    const float k = P->angle_k;
-                    const float* th = mesh->theta0 + 3 * tid;
-#pragma unroll
-                    for (int q = 0; q < 3; ++q) {
-                        const int a = q == 1 ? 1 : 0;
-                        const int b = q == 0 ? 1 : 2;
-                        st |= angle_pair_l(bx[a], by[a], bz[a], b2[a], bl[a], bx[b], by[b], bz[b], b2[b], bl[b], th[q],
-                                           k, u.angF + (tid * 3 + q) * 6);
-                    }
-                }
             }
             if (angles) __syncthreads();  // the angle forces of every node are in LDS
             if (node) {
+                const float X = lds_reread(sx, tid), Y = lds_reread(sy, tid), Z = lds_reread(sz, tid);
                 // ---- pressure, propagate.cu:318-330 (centre of mass from the last CenterOfMass)
                 const float rx = X - cld(p.CMX + slot), ry = Y - cld(p.CMY + slot), rz = Z - cld(p.CMZ + slot);
                 const float inv = 1.0f / sqrt_cr(rx * rx + ry * ry + rz * rz);
@@ -1032,10 +1048,10 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
                     ay = ay - (a[q * 6 + 1] + a[q * 6 + 4]);
                     az = az - (a[q * 6 + 2] + a[q * 6 + 5]);
                 }
-                const int32_t* gt = mesh->gather + tid * 6;
+                const uint32_t lg = lane_index() * 6;
 #pragma unroll
                 for (int q = 0; q < 6; ++q) {
-                    const float* f = u.angF + gt[q];
+                    const float* f = u.angF + ldu(mesh->gather, lg + q);
                     ax = ax + f[0];
                     ay = ay + f[1];
                     az = az + f[2];
@@ -1044,6 +1060,7 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             }
         }
         if (search && node) {
+            const float X = lds_reread(sx, tid), Y = lds_reread(sy, tid), Z = lds_reread(sz, tid);
             const int nc = nc_pre;
             const int32_t* cand = p.CAND + (size_t)slot * kPMax * kSlots;  // the cell's lists (uniform)
             const size_t sb = (size_t)sbase * kSlots;                      // the system's first node
@@ -1064,8 +1081,10 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             if (fused) {
                 const float vix = svx[tid], viy = svy[tid], viz = svz[tid];
                 const float gint = P->internal_damping;
+                int nn3[3];
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {  // bond damping, propagate.cu:547-554
+                    nn3[i] = ldu(mesh->nn, (uint32_t)(i * kSlots) + lane_index());
                     const int N = nn3[i];
                     fdx = fdx - gint * (vix - svx[N]);
                     fdy = fdy - gint * (viy - svy[N]);
