@@ -25,6 +25,7 @@ Further objects in the same line:
   ensemble         256 independent tissues for 5000 steps with growth and
                    division, batched in one context, against a size-stratified
                    sample of the same tissues run one after another (plain path)
+                   and the oracle on a smaller sample (sims/hour extrapolated)
   serving          256 requests of a Poisson stream through the
                    continuous-batching loop, admitted against a 200 GB budget
   c1_plain_small   8 cells, 10^4 steps with division at 2 V0, on the plain
@@ -338,15 +339,64 @@ def c3_workload(log, cpu=True, cpu_seconds=10.0):
     return res
 
 
+def ensemble_cpu(tissues, ens, steps, every=32, seconds=10.0, chunk=25):
+    """The oracle on a size-stratified sample of the ensemble's tissues (every
+    `every`-th by size), each from its start for an equal share of about `seconds`.
+    Each sampled tissue stands for `every` tissues of its size: the CPU time of the
+    whole ensemble is `every` x the sum over the sample of (its cells x steps x 180,
+    scaled by the ensemble's growth as counted on the GPU) / (its own CPU rate)."""
+    from cellsim3d_amd import configs, ensemble
+    from cellsim3d_amd import mesh as meshmod
+    from oracle import oracle as O
+    m = configs.unit_mesh()
+    om = O.build_mesh(meshmod.tables(m), configs.rest_positions(m))
+    idx = ensemble.sample_tissues(tissues, every)
+    share = seconds / len(idx)
+    growth = ens["node_updates"] / (180.0 * steps * sum(t["n"] for t in tissues))
+    cpu_hours = 0.0
+    cell_steps = 0
+    t_all = 0.0
+    steps_run = []
+    for i in idx:
+        t = tissues[i]
+        sim = O.Sim(t["params"], om, t["X"], t["Y"], t["Z"], t["n"], max_cells=t["max_cells"], budget=1 << 16)
+        sim.init()
+        k = 0
+        cs = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < share:
+            c0 = sim.n
+            sim.step(chunk)
+            cs += c0 * chunk  # cells at the start of the chunk (a lower bound while growing)
+            k += chunk
+        dt = time.perf_counter() - t0
+        t_all += dt
+        cell_steps += cs
+        steps_run.append(k)
+        rate_i = cs * 180.0 / dt
+        cpu_hours += every * growth * t["n"] * steps * 180.0 / rate_i / 3600.0
+    rate = cell_steps * 180.0 / t_all
+    return {"cores": O.get_threads(), "node_updates_per_s": rate,
+            "sims_per_hour_extrapolated": round(len(tissues) / cpu_hours, 1),
+            "sample": "%d of %d tissues (every %d-th by size), each from its start for %d-%d steps (%.1f s); "
+                      "the full run's CPU time from each sampled tissue's own rate, its size and the "
+                      "ensemble's growth (x%.2f)" % (len(idx), len(tissues), every, min(steps_run),
+                                                    max(steps_run), t_all, growth),
+            "batched_gpu_over_cpu_sims": round(ens["batched_sims_per_hour"] / (len(tissues) / cpu_hours), 1)}
+
+
 def measure_extras(args, gm, log):
     from cellsim3d_amd import configs, ensemble, serving
     out = {"bonded_roofline": bonded_roofline(gm, log=log)}
     out["c1_plain_small"] = c1_workload(gm, log, cpu=not args.no_cpu)
     out["c3_colony"] = c3_workload(log, cpu=not args.no_cpu)
     g, tissues = ensemble.build(256, 1, 1, 512, args.ensemble_steps, 10)
-    wg, gs, _ = ensemble.autotune(g, tissues, probe_steps=50)
-    out["ensemble"] = ensemble.measure(g, tissues, args.ensemble_steps, wg, gs, args.plain_every, log)
+    wg, gs, skin, _ = ensemble.autotune(g, tissues, probe_steps=50)
+    out["ensemble"] = ensemble.measure(g, tissues, args.ensemble_steps, wg, gs, args.plain_every, log, skin=skin)
     log("bench: ensemble x%.2f over the plain loop" % out["ensemble"]["batched_over_plain"])
+    if not args.no_cpu:
+        out["ensemble"]["cpu_oracle"] = ensemble_cpu(tissues, out["ensemble"], args.ensemble_steps)
+        log("bench: ensemble oracle %.3g node-updates/s" % out["ensemble"]["cpu_oracle"]["node_updates_per_s"])
     reqs = configs.serving_requests(args.serving_requests, 1, 20.0, 8, 4096, 1e3, 1e5)
     jobs = serving.make_jobs(reqs)
     srv = serving.Server(g, 200e9, chunk_steps=50)

@@ -151,25 +151,48 @@ def run_plain(gm, tissues, steps, indices=None, chunk=100, room=16.0):
     return total, cell_steps
 
 
-def autotune(gm, tissues, probe_steps=100, wg_options=(4, 8, 16, 32, 64), graph_options=(10, 25, 50), log=None):
-    """Picks the launch cap (workgroups per CU of the cell kernels, i.e. how
-    many cells a workgroup walks through) and the steps per captured graph by
-    timing short batched runs.  Returns (wg_per_cu, graph_steps, table)."""
+def with_skin(tissues, skin):
+    """Copies of the tissues with the contact-candidate skin set to `skin`.  The
+    skin sets the search tile (bin edge (attraction_range + skin) * 1.001) and how
+    often the candidates are searched again; results do not depend on it."""
+    if skin is None:
+        return tissues
+    out = []
+    for t in tissues:
+        u = dict(t)
+        u["params"] = t["params"].copy()
+        u["params"].pair_skin = skin
+        out.append(u)
+    return out
+
+
+def _probe(ctx, gs, probe_steps):
+    ctx.set_graph_steps(gs)
+    ctx.prepare_graph()
+    ctx.step(gs)  # warm
+    ctx.sync()
+    k = max(gs, (probe_steps // gs) * gs)
+    t0 = time.perf_counter()
+    ctx.step(k)
+    ctx.sync()
+    return (time.perf_counter() - t0) / k * 1e3
+
+
+def autotune(gm, tissues, probe_steps=100, wg_options=(4, 8, 16, 32, 64), graph_options=(10, 25, 50),
+             skin_options=(0.03, 0.05, 0.08), log=None):
+    """Picks, by timing short batched runs, the launch cap (workgroups per CU of
+    the cell kernels, i.e. how many cells a workgroup walks through) and the
+    steps per captured graph over their grid of options, then, at that pair, the
+    contact-search tile: the skin, which sets the edge of the search bins and how
+    often candidates are searched again.  Returns (wg_per_cu, graph_steps, skin,
+    table)."""
     ctx = _context(gm, tissues)
     systems = [ctx.add_system(t["params"], t["X"], t["Y"], t["Z"], t["n"], max_cells=t["max_cells"]) for t in tissues]
     table = []
     for wg in wg_options:
         for gs in graph_options:
             ctx.set_launch(wg)
-            ctx.set_graph_steps(gs)
-            ctx.prepare_graph()
-            ctx.step(gs)  # warm
-            ctx.sync()
-            k = max(gs, (probe_steps // gs) * gs)
-            t0 = time.perf_counter()
-            ctx.step(k)
-            ctx.sync()
-            ms = (time.perf_counter() - t0) / k * 1e3
+            ms = _probe(ctx, gs, probe_steps)
             table.append({"wg_per_cu": wg, "graph_steps": gs, "ms_per_step": ms})
             if log:
                 log("autotune: wg_per_cu %d graph_steps %d -> %.4f ms/step" % (wg, gs, ms))
@@ -177,7 +200,21 @@ def autotune(gm, tissues, probe_steps=100, wg_options=(4, 8, 16, 32, 64), graph_
         s.remove()
     ctx.close()
     best = min(table, key=lambda r: r["ms_per_step"])
-    return best["wg_per_cu"], best["graph_steps"], table
+    wg, gs = best["wg_per_cu"], best["graph_steps"]
+    skins = []
+    for skin in skin_options:
+        ctx = _context(gm, tissues)
+        ctx.set_launch(wg)
+        tk = with_skin(tissues, skin)
+        systems = [ctx.add_system(t["params"], t["X"], t["Y"], t["Z"], t["n"], max_cells=t["max_cells"]) for t in tk]
+        ms = _probe(ctx, gs, probe_steps)
+        skins.append({"wg_per_cu": wg, "graph_steps": gs, "pair_skin": skin, "ms_per_step": ms})
+        if log:
+            log("autotune: pair_skin %.3f -> %.4f ms/step" % (skin, ms))
+        ctx.close()
+    table += skins
+    skin = min(skins, key=lambda r: r["ms_per_step"])["pair_skin"] if skins else None
+    return wg, gs, skin, table
 
 
 def main(argv=None):
@@ -198,19 +235,20 @@ def main(argv=None):
     gm, tissues = build(args.tissues, args.seed, 1, args.nmax, args.steps, args.grid_interval)
     cells = sum(t["n"] for t in tissues)
     log("ensemble: %d tissues, %d cells, %d steps" % (len(tissues), cells, args.steps))
-    wg, gs, table = (16, 10, [])
+    wg, gs, skin, table = (16, 10, None, [])
     if not args.no_tune:
-        wg, gs, table = autotune(gm, tissues, log=log)
-    out = measure(gm, tissues, args.steps, wg, gs, args.plain_every, log)
+        wg, gs, skin, table = autotune(gm, tissues, log=log)
+    out = measure(gm, tissues, args.steps, wg, gs, args.plain_every, log, skin=skin)
     out["autotune"] = table
     print(json.dumps(out), flush=True)
     return out
 
 
-def measure(gm, tissues, steps, wg, gs, plain_every=8, log=None):
+def measure(gm, tissues, steps, wg, gs, plain_every=8, log=None, skin=None):
     """Batched run of every tissue and plain runs of a size-stratified sample,
-    both for the full length with divisions; node-updates from the cells actually
-    stepped (counted on the device)."""
+    both for the full length with divisions and with the same contact skin;
+    node-updates from the cells actually stepped (counted on the device)."""
+    tissues = with_skin(tissues, skin)
     dt_b, ctx, _, info = run_batched(gm, tissues, steps, wg_per_cu=wg, graph_steps=gs)
     ctx.close()
     nu = 180.0 * info["cell_steps"]
@@ -241,7 +279,8 @@ def measure(gm, tissues, steps, wg, gs, plain_every=8, log=None):
         "node_updates": nu,
         "node_updates_counted": "cells stepped in every step, counted on the device (growing tissues)",
         "batched_over_plain": round(dt_p / dt_b, 2),
-        "autotuned": {"wg_per_cu": wg, "graph_steps": gs},
+        "autotuned": {"wg_per_cu": wg, "graph_steps": gs,
+                      "pair_skin": float(skin) if skin is not None else float(tissues[0]["params"].pair_skin)},
     }
 
 

@@ -605,6 +605,51 @@ def test_staggered_divisions_in_a_tissue_match_oracle(oracle_mesh):
     ctx.close()
 
 
+def test_results_do_not_depend_on_the_skin(oracle_mesh):
+    """The contact-candidate skin is a performance knob only (the autotuner picks it:
+    it sets the edge of the search bins and how often candidates are searched
+    again).  The 64-cell tissue of the staggered-division test, stepped 100 steps
+    through its divisions with skins 0.01, 0.05 and 0.15 as three systems of one
+    batch (graph path), ends bit for bit where the oracle ends; the oracle scans
+    every node of every cell in the voxel lists and has no skin."""
+    A, E = _abi(), _engine()
+    from cellsim3d_amd import configs
+    m, X, Y, Z, info = configs.lattice_tissue(64, 0.40, seed=11)
+    p = configs.base_params(11)
+    p.division_vol = np.float32(1.02 * info["v0"])
+    p.div_time_steps = 10 ** 6
+    p.voxel_capacity = 64
+    n, mc = 64, 256
+    gs = np.ones(mc, np.float32)
+    gs[:n] = np.random.default_rng(5).uniform(0.5, 1.5, n).astype(np.float32)
+    O = _oracle()
+    sim = O.Sim(p, oracle_mesh, X, Y, Z, n, max_cells=mc, budget=1 << 16)
+    sim.set(A.F_GROWTH_SCALE, gs)
+    sim.init()
+    ctx = E.Context(oracle_mesh, capacity_cells=3 * mc + 16, voxel_budget=1 << 24)
+    systems = []
+    for skin in (0.01, 0.05, 0.15):
+        q = p.copy()
+        q.pair_skin = np.float32(skin)
+        sy = ctx.add_system(q, X, Y, Z, n, max_cells=mc)
+        sy.set(A.F_GROWTH_SCALE, gs[:n])
+        systems.append((skin, sy))
+    sim.step(100)
+    ctx.step(100)
+    ctx.sync()
+    nn = sim.n
+    assert nn > n, "no division happened"
+    for skin, sy in systems:
+        n1, st1, status = sy.info()
+        print("skin %.2f: %d cells after %d steps, status 0x%x (oracle %d cells)" % (skin, n1, st1, status, nn),
+              flush=True)
+        assert status & A.SYS_FATAL_MASK == 0
+        assert n1 == nn and st1 == sim.stepno
+        _cmp(sim, sy, [A.F_X, A.F_Y, A.F_Z, A.F_VX, A.F_VY, A.F_VZ, A.F_FCX, A.F_FDX, A.F_PRESSURE], nn,
+             label="skin %.2f" % skin)
+    ctx.close()
+
+
 def test_growing_colony_matches_oracle(oracle_mesh):
     """The growing-colony workload at reduced size: one cell with per-cell growth
     scales U[0.8, 1.2], division at twice the cell volume, voxel lists of 128, the
