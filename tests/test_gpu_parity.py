@@ -650,6 +650,57 @@ def test_results_do_not_depend_on_the_skin(oracle_mesh):
     ctx.close()
 
 
+def test_division_burst_with_aliased_bins_matches_oracle(oracle_mesh):
+    """The large-tissue paths of the colony at a size the oracle steps in seconds: a
+    400-cell tissue at volume fraction 0.40 whose cells reach the division volume
+    within a few dozen steps of each other (up to about 20 divisions in one step, 400
+    -> about 780 cells in 200 steps), voxel lists of 128, the grid rebuilt every 10
+    steps and after divisions, and a bin budget of 98,304 search bins where the
+    tissue's extent needs about 128,000 bins of edge attraction_range + skin: the
+    periodic bins alias distant regions into the same bins (at most 16 nodes share a
+    bin through the run, by a host model of the binning; a budget of 32,768 overflows
+    the 32-node bins once cells divide, which the engine flags), and lists of divided
+    and moved cells longer than the partial search takes force full searches.  Cell
+    counts must agree with the oracle every 10 steps, and the state and the voxel grid
+    bit for bit at the end."""
+    A, E, O = _abi(), _engine(), _oracle()
+    from cellsim3d_amd import configs
+    n, mc, bins = 400, 1024, 98304
+    m, X, Y, Z, info = configs.lattice_tissue(n, 0.40, seed=9)
+    p = configs.base_params(9)
+    p.division_vol = np.float32(1.01 * info["v0"])
+    p.div_time_steps = 10 ** 6
+    p.voxel_capacity = 128
+    p.grid_interval = 10
+    h = (float(p.attraction_range) + float(p.pair_skin)) * 1.001
+    need = np.prod([(float(q.max()) - float(q.min())) / h + 5
+                    for q in (X.reshape(n, 192)[:, :180], Y.reshape(n, 192)[:, :180], Z.reshape(n, 192)[:, :180])])
+    assert need > 1.2 * bins, "the bin budget does not alias: %d bins needed" % need
+    budget = 1 << 15
+    sim = O.Sim(p, oracle_mesh, X, Y, Z, n, max_cells=mc, budget=budget)
+    assert sim.init() == 0
+    ctx = E.Context(oracle_mesh, capacity_cells=mc, voxel_budget=budget * (p.voxel_capacity + 1),
+                    bin_budget=bins * A.BIN_WORDS + 64)
+    s = ctx.add_system(p, X, Y, Z, n, max_cells=mc, voxels=budget)
+    counts = []
+    for k in range(20):
+        assert sim.step(10) & A.SYS_FATAL_MASK == 0
+        s.step(10)
+        ctx.sync()
+        n1, st1, status = s.info()
+        assert status & A.SYS_FATAL_MASK == 0, hex(status)
+        assert n1 == sim.n and st1 == sim.stepno, "after %d steps: gpu %d cells, oracle %d" % (st1, n1, sim.n)
+        counts.append(n1)
+    print("cell counts every 10 steps:", counts, flush=True)
+    jumps = [b - a for a, b in zip([n] + counts, counts)]
+    assert max(jumps) >= 30 and counts[-1] > 1.8 * n, counts
+    nn = sim.n
+    _cmp(sim, s, [A.F_X, A.F_Y, A.F_Z, A.F_VX, A.F_VY, A.F_VZ, A.F_FCX, A.F_FCY, A.F_FCZ, A.F_FDX, A.F_FDY,
+                  A.F_FDZ, A.F_PRESSURE, A.F_VOLUME, A.F_CMX, A.F_CMY, A.F_CMZ], nn, label="division burst")
+    _cmp_grid(sim, s)
+    ctx.close()
+
+
 def test_growing_colony_matches_oracle(oracle_mesh):
     """The growing-colony workload at reduced size: one cell with per-cell growth
     scales U[0.8, 1.2], division at twice the cell volume, voxel lists of 128, the
