@@ -1609,7 +1609,7 @@ __device__ __forceinline__ void block_minmax12(float* v, float (*part)[12], floa
 // Division phase of system s by one 256-lane workgroup (every lane calls it; the
 // early returns depend on the system only, so the workgroup stays together).
 __device__ void divide_system(const Pool& p, int s, int32_t mode) {
-    __shared__ int32_t scan[256];
+    __shared__ int32_t wsum[4];
     __shared__ int32_t nev, ncur, stop;
     __shared__ float nrm[3];
     __shared__ int32_t bad;
@@ -1646,26 +1646,44 @@ __device__ void divide_system(const Pool& p, int s, int32_t mode) {
     if (tid == 0) { nev = 0; ncur = n0; stop = 0; bad = 0; }
     __syncthreads();
     if (growth) {
-        // ascending event list in EVL[base .. base + nev)
-        for (int c0 = 0; c0 < n0; c0 += 256) {
-            const int c = c0 + tid;
-            const int f = (c < n0 && p.FLAG[base + c] == 1) ? 1 : 0;
-            scan[tid] = f;
-            __syncthreads();
-            for (int o = 1; o < 256; o <<= 1) {
-                const int t = tid >= o ? scan[tid - o] : 0;
-                __syncthreads();
-                scan[tid] += t;
-                __syncthreads();
+        // ascending event list in EVL[base .. base + nev), 4096 cells per pass: each
+        // thread loads the flags of 16 consecutive cells at once and counts them, a scan
+        // of the counts (shuffles within a wave, then the four wave totals) gives the
+        // thread's first event, and it lists its events in cell order
+        constexpr int kRun = 16;
+        int listed = 0;  // events listed by the passes before (the same in every thread)
+        for (int c0 = 0; c0 < n0; c0 += kRun * 256) {
+            const int cr = c0 + kRun * tid;
+            int8_t f[kRun];
+#pragma unroll
+            for (int j = 0; j < kRun; ++j) f[j] = cr + j < n0 ? p.FLAG[base + cr + j] : (int8_t)0;
+            int cnt = 0;
+#pragma unroll
+            for (int j = 0; j < kRun; ++j) cnt += f[j] == 1 ? 1 : 0;
+            const int lane = tid & 63, wave = tid >> 6;
+            int inc = cnt;  // inclusive scan within the wave
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(inc, o);
+                if (lane >= o) inc += t;
             }
-            if (f) {
-                p.EVL[base + nev + scan[tid] - 1] = c;
-                p.FLAG[base + c] = 0;
+            if (lane == 63) wsum[wave] = inc;
+            __syncthreads();
+            int e = listed + inc - cnt;
+            for (int w = 0; w < wave; ++w) e += wsum[w];
+            if (cnt) {
+#pragma unroll
+                for (int j = 0; j < kRun; ++j)
+                    if (f[j] == 1) {
+                        p.EVL[base + e++] = cr + j;
+                        p.FLAG[base + cr + j] = 0;
+                    }
             }
-            __syncthreads();
-            if (tid == 0) nev += scan[255];
-            __syncthreads();
+            listed += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+            __syncthreads();  // wsum is rewritten by the next pass
         }
+        if (tid == 0) nev = listed;
+        __syncthreads();
         const int ne = nev;
         const float half = (float)(0.5 * (double)P->repulsion_range);
         for (int e = 0; e < ne; ++e) {
@@ -1775,12 +1793,27 @@ __device__ void divide_system(const Pool& p, int s, int32_t mode) {
         float e[12];
 #pragma unroll
         for (int a = 0; a < 12; a += 2) { e[a] = 1.0E8f; e[a + 1] = -1.0E8f; }
-        for (int c = tid; c < ncur; c += blockDim.x) {
-            const float* b = p.BBOXN + (int64_t)(base + c) * 6;
+        // the boxes of 8 cells per thread are loaded before any is reduced (min and max
+        // are exact in any order)
+        constexpr int kUnroll = 8;
+        for (int c0 = 0; c0 < ncur; c0 += kUnroll * 256) {
+            float bb[kUnroll][6];
 #pragma unroll
-            for (int a = 0; a < 6; a += 2) {
-                e[a] = fminf(e[a], b[a]);
-                e[a + 1] = fmaxf(e[a + 1], b[a + 1]);
+            for (int u = 0; u < kUnroll; ++u) {
+                const int c = c0 + u * 256 + tid;
+                const float* b = p.BBOXN + (int64_t)(base + min(c, ncur - 1)) * 6;
+#pragma unroll
+                for (int a = 0; a < 6; ++a) bb[u][a] = b[a];
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                if (c0 + u * 256 + tid < ncur) {
+#pragma unroll
+                    for (int a = 0; a < 6; a += 2) {
+                        e[a] = fminf(e[a], bb[u][a]);
+                        e[a + 1] = fmaxf(e[a + 1], bb[u][a + 1]);
+                    }
+                }
             }
         }
         block_minmax12(e, mpart, mres);
