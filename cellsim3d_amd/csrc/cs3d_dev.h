@@ -32,7 +32,9 @@ constexpr int kNodes = CS3D_NODES;   // 180
 constexpr int kPMax = 40;            // contact candidates / partners stored per node
 constexpr int kBCap = CS3D_BIN_CAP;  // nodes per search bin
 constexpr int kMaxSys = 4096;        // systems per context
-constexpr int kMaxPartial = 64;      // listed cells beyond which the next search is a full one
+constexpr int kMaxPartial = 192;     // listed cells beyond which the next search is a full one (at most the
+                                     // 192 lanes of a search workgroup, which test one listed cell each)
+static_assert(kMaxPartial <= kSlots, "the partial search tests one listed cell per lane");
 
 // Per-system state kept on the device.
 struct SysMeta {
