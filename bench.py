@@ -345,7 +345,7 @@ def ensemble_cpu(tissues, ens, steps, every=32, seconds=10.0, chunk=25):
     Each sampled tissue stands for `every` tissues of its size: the CPU time of the
     whole ensemble is `every` x the sum over the sample of (its cells x steps x 180,
     scaled by the ensemble's growth as counted on the GPU) / (its own CPU rate)."""
-    from cellsim3d_amd import configs, ensemble
+    from cellsim3d_amd import abi, configs, ensemble
     from cellsim3d_amd import mesh as meshmod
     from oracle import oracle as O
     m = configs.unit_mesh()
@@ -359,16 +359,20 @@ def ensemble_cpu(tissues, ens, steps, every=32, seconds=10.0, chunk=25):
     steps_run = []
     for i in idx:
         t = tissues[i]
-        sim = O.Sim(t["params"], om, t["X"], t["Y"], t["Z"], t["n"], max_cells=t["max_cells"], budget=1 << 16)
+        # room for several generations: the sample may run thousands of steps
+        sim = O.Sim(t["params"], om, t["X"], t["Y"], t["Z"], t["n"], max_cells=16 * t["n"] + 16, budget=1 << 16)
         sim.init()
         k = 0
         cs = 0
         t0 = time.perf_counter()
         while time.perf_counter() - t0 < share:
-            c0 = sim.n
-            sim.step(chunk)
-            cs += c0 * chunk  # cells at the start of the chunk (a lower bound while growing)
-            k += chunk
+            c0, s0 = sim.n, sim.stepno
+            st = sim.step(chunk)
+            done = sim.stepno - s0  # the oracle stops a call at a fatal status
+            cs += c0 * done  # cells at the start of the chunk (a lower bound while growing)
+            k += done
+            if st & abi.SYS_FATAL_MASK:
+                break
         dt = time.perf_counter() - t0
         t_all += dt
         cell_steps += cs
