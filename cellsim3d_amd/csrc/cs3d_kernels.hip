@@ -475,35 +475,21 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
         }
         if (node && (flags & I_CLEAR)) q->bins[MC->bbase + bin_of(MC, x, y, z)] = 0;
         if ((flags & I_BINS) && (!(flags & I_IFSEARCH) || MC->need_search)) {
-            // insertion into the search bins: the lanes of a wave that share a bin
-            // take one atomic between them (issued by the group's first lane), and
-            // each lane's slot is the group's base plus its rank in the group.  A
-            // partial search (after divisions) works on search positions: the nodes of
-            // cells that did not divide are binned at their last search position
+            // insertion into the search bins, one atomic per node (the order of a bin's
+            // entries does not matter: every candidate list is sorted).  Grouping the
+            // lanes of a wave that share a bin behind one atomic was slower: the loop
+            // over a wave's distinct bins cost more than the atomics it saved (the colony
+            // at 20,000 cells: 0.215 against 0.188 ms for this kernel).  A partial search
+            // (after divisions) works on search positions: the nodes of cells that did
+            // not divide are binned at their last search position
             float qx = x, qy = y, qz = z;
             if ((flags & I_IFSEARCH) && MC->need_search == 2 && node && !q->RESET[slot]) {
                 qx = q->XBX[gi]; qy = q->XBY[gi]; qz = q->XBZ[gi];
             }
             const int b = node ? bin_of(MC, qx, qy, qz) : -1;
-            const int lane = tid & 63;
-            uint64_t todo = __ballot(node);
-            int leader_of = 0, rank = 0, group = 0;
-            while (todo) {
-                const int leader = __ffsll((unsigned long long)todo) - 1;
-                const int lb = __shfl(b, leader);
-                const uint64_t same = __ballot(b == lb) & todo;
-                if (b == lb) {
-                    leader_of = leader;
-                    rank = __popcll(same & ((1ull << lane) - 1ull));
-                    group = __popcll(same);
-                }
-                todo &= ~same;
-            }
             int32_t* counts = q->bins + MC->bbase;
-            int k0 = 0;
-            if (node && lane == leader_of) k0 = atomicAdd(&counts[b], group);
-            const int k = __shfl(k0, leader_of) + rank;
             if (node) {
+                const int k = atomicAdd(&counts[b], 1);
                 if (k < kBCap) {
                     float4* ent = reinterpret_cast<float4*>(q->bins + MC->ebase);
                     ent[(int64_t)b * kBCap + k] = make_float4(qx, qy, qz, __int_as_float(c * kSlots + tid));
