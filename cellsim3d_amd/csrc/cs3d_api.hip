@@ -618,6 +618,22 @@ extern "C" cs3d_status cs3d_system_set_horizon(cs3d_ctx* c, cs3d_sys s, int64_t 
     return write_meta(c, s, m);
 }
 
+extern "C" cs3d_status cs3d_systems_set_horizon(cs3d_ctx* c, int32_t first, int32_t count, const int64_t* horizons) {
+    if (!c || first < 0 || count < 0 || (count > 0 && !horizons) || (int64_t)first + count > (int64_t)c->sys.size())
+        return fail(CS3D_INVALID_ARG, "bad horizon range");
+    if (count == 0) return CS3D_OK;
+    // the stream is idle while the metadata of the range makes its round trip
+    std::vector<SysMeta> m(count);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpyAsync(m.data(), c->pool.meta + first, sizeof(SysMeta) * count, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int32_t i = 0; i < count; ++i)
+        if (c->sys[first + i].used) m[i].horizon = horizons[i];
+    HIPCHK(hipMemcpyAsync(c->pool.meta + first, m.data(), sizeof(SysMeta) * count, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CS3D_OK;
+}
+
 extern "C" cs3d_status cs3d_system_info(cs3d_ctx* c, cs3d_sys s, int32_t* n_cells, int64_t* step, uint32_t* status) {
     if (!valid_sys(c, s)) return fail(CS3D_INVALID_ARG, "unknown system");
     SysMeta m;

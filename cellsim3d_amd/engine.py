@@ -59,6 +59,7 @@ SIGNATURES = {
     "cs3d_set_launch": ([vp, C.c_int32], st_t),
     "cs3d_systems_poll": ([vp, C.c_int32, C.c_int32, vp, vp, vp], st_t),
     "cs3d_systems_work": ([vp, C.c_int32, C.c_int32, vp], st_t),
+    "cs3d_systems_set_horizon": ([vp, C.c_int32, C.c_int32, vp], st_t),
     "cs3d_graph_prepare": ([vp, C.c_int32], st_t),
     "cs3d_step_timed": ([vp, C.c_int32, C.c_int32, vp], st_t),
     "cs3d_time_kernel": ([vp, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double)], st_t),
@@ -235,6 +236,13 @@ class Context:
         w = np.zeros(count, dtype=np.int64)
         check(lib().cs3d_systems_work(self.h, int(first), int(count), w.ctypes.data_as(vp)), "cs3d_systems_work")
         return w
+
+    def set_horizons(self, first, horizons):
+        """Horizons of systems first .. first+len(horizons)-1 (ids not in use are
+        ignored); a horizon at or below a system's step pauses it."""
+        h = np.ascontiguousarray(horizons, dtype=np.int64)
+        check(lib().cs3d_systems_set_horizon(self.h, int(first), int(h.size), h.ctypes.data_as(vp)),
+              "cs3d_systems_set_horizon")
 
     def prepare_graph(self, target=abi.ALL):
         """Captures and uploads the graph that step() replays for target, without stepping."""

@@ -76,8 +76,22 @@ def make_jobs(requests, grid_interval=10, phi=0.30):
 
 
 class Server:
+    """policy "least_work": before each chunk the resident jobs are ordered by their
+    remaining work (cells x steps still to go) and the chunk steps the shortest
+    ones, up to active_cells cells (at least one job); the others are paused on the
+    device (cs3d_systems_set_horizon) and stay resident.  A job that arrives with
+    little work therefore finishes without waiting behind long ones, while the batch
+    stays large enough to fill the GPU.  policy "all": every resident job steps in
+    every chunk (processor sharing by steps).  Results do not depend on the policy:
+    pausing leaves a system's state as it is."""
+
     def __init__(self, gm, hbm_budget_bytes, max_cells_total=None, chunk_steps=50, graph_steps=10,
-                 wg_per_cu=16, fields=(abi.F_X, abi.F_Y, abi.F_Z)):
+                 wg_per_cu=16, fields=(abi.F_X, abi.F_Y, abi.F_Z), policy="least_work", active_cells=16384):
+        if policy not in ("least_work", "all"):
+            raise ValueError("policy must be 'least_work' or 'all'")
+        self.policy = policy
+        self.active_cells = int(active_cells)
+        self.pauses = 0  # jobs paused by the scheduler (a job counts once per pause)
         per_cell = SLOT_BYTES + 4 * (VOX_WORDS_PER_CELL + BIN_WORDS_PER_CELL)
         cap = int(hbm_budget_bytes // per_cell)
         if max_cells_total:
@@ -122,6 +136,9 @@ class Server:
         the wall time.  Each job gets admitted/finished times and a result."""
         queue = sorted(jobs, key=lambda j: j.arrival)
         active = {}
+        steps_now = {}  # system id -> steps done at the last poll
+        running = None  # system ids the last horizon update let step
+        paused = set()  # system ids it paused
         t0 = time.perf_counter()
         qi = 0
         while qi < len(queue) or active:
@@ -134,23 +151,46 @@ class Server:
                                            (queue[qi].id, queue[qi].n, self.capacity))
                     break  # first in line waits for room (arrival order)
                 active[s.id] = (queue[qi], s)
+                steps_now[s.id] = 0
+                running = None  # the new job takes part in the next choice
                 qi += 1
             if not active:
                 if qi < len(queue):
                     time.sleep(max(0.0, queue[qi].arrival * time_scale - (time.perf_counter() - t0)))
                 continue
             self.peak_jobs = max(self.peak_jobs, len(active))
-            self.ctx.step(self.chunk)
             hi = max(active) + 1
+            if self.policy == "least_work":
+                order = sorted(active, key=lambda sid: (active[sid][0].n * (active[sid][0].horizon - steps_now[sid]),
+                                                        active[sid][0].arrival, sid))
+                chosen, cells = set(), 0
+                for sid in order:
+                    chosen.add(sid)
+                    cells += active[sid][0].n
+                    if cells >= self.active_cells:
+                        break
+                if chosen != running:
+                    hz = np.full(hi, -1, dtype=np.int64)
+                    for sid, (job, s) in active.items():
+                        hz[sid] = job.horizon if sid in chosen else steps_now[sid]
+                    held = set(active) - chosen
+                    self.pauses += len(held - paused)
+                    paused = held
+                    self.ctx.set_horizons(0, hz)
+                    running = chosen
+            self.ctx.step(self.chunk)
             ncell, steps, status = self.ctx.poll(0, hi)
             now = time.perf_counter() - t0
             for sid in list(active):
                 job, s = active[sid]
+                steps_now[sid] = int(steps[sid])
                 done = steps[sid] >= job.horizon or (status[sid] & abi.SYS_FATAL_MASK)
                 if done:
                     self.node_updates += float(job.n) * 180.0 * float(steps[sid])
                     self._finish(job, s, now)
                     del active[sid]
+                    del steps_now[sid]
+                    running = None
                     if log:
                         log("job %d done: %d cells, %d steps, latency %.3f s" %
                             (job.id, job.n, job.horizon, now - job.arrival * time_scale))
@@ -183,6 +223,8 @@ def summary(jobs, srv, wall):
         "offered_node_updates_per_s": offered / span if span > 0 else None,
         "rho": round((offered / span) / served, 3) if span > 0 and served > 0 else None,
         "peak_resident_jobs": srv.peak_jobs,
+        "policy": srv.policy if srv.policy == "all" else "%s (the jobs with the least work left step, until %d "
+                                                          "cells)" % (srv.policy, srv.active_cells),
     }
 
 
@@ -197,6 +239,9 @@ def main(argv=None):
     ap.add_argument("--max-cells", type=int, default=0, help="cap of the device pool in cells (0: the budget decides)")
     ap.add_argument("--chunk", type=int, default=50)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--policy", choices=("least_work", "all"), default="least_work")
+    ap.add_argument("--active-cells", type=int, default=16384,
+                    help="least_work: cells of the jobs that step in a chunk (the least work left first)")
     args = ap.parse_args(argv)
 
     def log(msg):
@@ -206,7 +251,8 @@ def main(argv=None):
     gm = engine.build_mesh(meshmod.tables(m), configs.rest_positions(m))
     reqs = configs.serving_requests(args.requests, args.seed, args.rate, 8, args.nmax, args.hmin, args.hmax)
     jobs = make_jobs(reqs)
-    srv = Server(gm, args.budget_gb * 1e9, max_cells_total=args.max_cells or None, chunk_steps=args.chunk)
+    srv = Server(gm, args.budget_gb * 1e9, max_cells_total=args.max_cells or None, chunk_steps=args.chunk,
+                 policy=args.policy, active_cells=args.active_cells)
     log("serving: %d requests, %d cells in total, pool of %d cells" % (len(jobs), sum(j.n for j in jobs),
                                                                       srv.capacity))
     wall = srv.run(jobs, log=log)

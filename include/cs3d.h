@@ -241,6 +241,14 @@ cs3d_status cs3d_system_grow(cs3d_ctx* ctx, cs3d_sys sys, int32_t max_cells, int
 cs3d_status cs3d_system_startup(cs3d_ctx* ctx, cs3d_sys sys);
 /* Limits a system to `horizon` total steps (stepping stops there; -1 = none). */
 cs3d_status cs3d_system_set_horizon(cs3d_ctx* ctx, cs3d_sys sys, int64_t horizon);
+/* Sets the horizons of systems first .. first+count-1 at once (entries of ids not in
+ * use are ignored).  A horizon at or below a system's current step pauses it: the
+ * step kernels skip it until a later horizon resumes it, and its state stays on the
+ * device unchanged, so a paused and resumed system ends where an uninterrupted run
+ * ends.  For schedulers that choose which resident systems advance (the serving
+ * loop steps the jobs with the least remaining work).  No counterpart in the
+ * reference, which steps one tissue.  Synchronous. */
+cs3d_status cs3d_systems_set_horizon(cs3d_ctx* ctx, int32_t first, int32_t count, const int64_t* horizons);
 cs3d_status cs3d_system_info(cs3d_ctx* ctx, cs3d_sys sys, int32_t* n_cells, int64_t* step, uint32_t* status);
 /* Sets the live cell count and step counter (state injection). */
 cs3d_status cs3d_system_set_counts(cs3d_ctx* ctx, cs3d_sys sys, int32_t n_cells, int64_t step);

@@ -121,7 +121,9 @@ def test_batched_divisions_match_oracle_and_alone():
 def test_served_requests_match_oracle():
     """Ten requests arriving at once; a 40-cell pool admits the first few and
     makes the others wait for finished ones, so later requests are admitted
-    while earlier ones are still being stepped."""
+    while earlier ones are still being stepped.  The scheduler steps only the
+    resident jobs with the least work left, up to 12 cells per chunk, and pauses
+    the others on the device, so most jobs are paused and resumed on the way."""
     from cellsim3d_amd import abi, serving
     gm = _mesh()
     specs = [(0.0, 1, 60), (0.0, 8, 140), (0.0, 20, 90), (0.0, 3, 230), (0.0, 12, 110), (0.0, 2, 75),
@@ -131,9 +133,10 @@ def test_served_requests_match_oracle():
     print("serving: %d requests into a 40-cell pool" % len(jobs), flush=True)
     fields = tuple(getattr(abi, f) for f in NODE)
     srv = serving.Server(gm, hbm_budget_bytes=1e12, max_cells_total=40, chunk_steps=20, graph_steps=10,
-                         fields=fields)
+                         fields=fields, policy="least_work", active_cells=12)
     srv.run(jobs)
     srv.close()
+    assert srv.pauses >= 3, "the scheduler paused too few jobs (%d)" % srv.pauses
     waited = sum(1 for j in jobs if j.admitted > j.arrival + 1e-3)
     print("serving: %d of %d requests waited for room" % (waited, len(jobs)), flush=True)
     assert waited >= 1
