@@ -370,6 +370,59 @@ def test_full_size_tissue_matches_oracle(oracle_mesh):
     ctx.close()
 
 
+def test_throughput_workload_at_length_matches_oracle(oracle_mesh):
+    """The throughput workload as the bench steps it (2000 cells, grid rebuilt every
+    10 steps, the captured graph) for 300 steps, through about a dozen candidate
+    searches, 30 grid rebuilds and the skin decisions between them: the state and the
+    voxel grid bit for bit at the end."""
+    A = _abi()
+    from cellsim3d_amd import configs
+    cfg = configs.single_tissue(2000, 0.40, seed=1, grid_interval=10, steps=300)
+    p, n = cfg["params"], cfg["n"]
+    sim, ctx, s = _pair(oracle_mesh, p, cfg["X"], cfg["Y"], cfg["Z"], n, n, budget=1 << 14)
+    ctx.set_graph_steps(100)
+    for k in range(3):
+        sim.step(100)
+        s.step(100)
+        ctx.sync()
+        n1, st1, status = s.info()
+        assert n1 == n and st1 == 100 * (k + 1) and status & A.SYS_FATAL_MASK == 0
+        print("2000 cells: %d steps" % st1, flush=True)
+    _cmp(sim, s, [A.F_X, A.F_Y, A.F_Z, A.F_VX, A.F_VY, A.F_VZ, A.F_FCX, A.F_FCY, A.F_FCZ, A.F_FDX, A.F_FDY,
+                  A.F_FDZ, A.F_CMX, A.F_CMY, A.F_CMZ], n, label="2000 cells, 300 steps")
+    _cmp_grid(sim, s)
+    ctx.close()
+
+
+def test_random_force_with_divisions_matches_oracle(oracle_mesh):
+    """add_rands=1 through divisions: the daughters' first Integrate and the random
+    force of the new cells, in the graph path, against the oracle."""
+    A = _abi()
+    from cellsim3d_amd import configs
+    cfg = configs.plain_small(seed=3)
+    p = cfg["params"]
+    p.division_vol = np.float32(1.02 * cfg["v0"])
+    p.add_rands = 1
+    p.rand_scale_factor = np.float32(0.05)
+    p.noise_seed = 7
+    n, mc = cfg["n"], 64
+    sim, ctx, s = _pair(oracle_mesh, p, cfg["X"], cfg["Y"], cfg["Z"], n, mc)
+    for k in range(6):
+        sim.step(50)
+        s.step(50)
+        ctx.sync()
+        n1, st1, status = s.info()
+        assert status & A.SYS_FATAL_MASK == 0
+        assert n1 == sim.n and st1 == sim.stepno, "after %d steps: gpu %d cells, oracle %d" % (st1, n1, sim.n)
+    assert sim.n > n, "no division happened"
+    nn = sim.n
+    for f in (A.F_FRX, A.F_FRY, A.F_FRZ):
+        assert np.array_equal(s.get(f)[:nn * 192], sim.get(f)[:nn * 192]), "random force field %d" % f
+    _cmp(sim, s, [A.F_X, A.F_Y, A.F_Z, A.F_VX, A.F_VY, A.F_VZ, A.F_PRESSURE, A.F_VOLUME], nn,
+         label="add_rands with divisions")
+    ctx.close()
+
+
 def _flags(st):
     A = _abi()
     return st & A.SYS_FATAL_MASK
