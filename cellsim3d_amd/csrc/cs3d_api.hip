@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <mutex>
 #include <string>
@@ -13,6 +14,7 @@
 #include <vector>
 
 #include "cs3d_dev.h"
+#include "cs3d_div.h"
 #include "cs3d_internal.h"
 
 namespace cs3d {
@@ -38,6 +40,7 @@ void launch_pressure_reset(const Pool& p, const Range& r, const int32_t* cells, 
 void launch_set_flags(const Pool& p, const Range& r, int rebuild, int search, hipStream_t st);
 void launch_copy_calibrate(const void* src, void* dst, int64_t n4, hipStream_t st);
 void launch_check_sqrt(unsigned long long* bad, hipStream_t st);
+void launch_check_div(unsigned long long* bad, uint32_t a_count, uint64_t scaled_pairs, hipStream_t st);
 void launch_acos(const double* x, double* y, int64_t n, hipStream_t st);
 }  // namespace cs3d
 
@@ -350,6 +353,10 @@ extern "C" cs3d_status cs3d_ctx_create(int device, const cs3d_mesh* mesh, int64_
     std::memcpy(md.sign, mesh->sign, sizeof(md.sign));
     std::memcpy(md.faces, mesh->faces, sizeof(md.faces));
     std::memcpy(md.R0, mesh->R0, sizeof(md.R0));
+    // reciprocals of the rest lengths for the kernels' quotients (IEEE division here;
+    // NaN where quot_ok fails sends those bonds to the kernels' division path)
+    for (int i = 0; i < 3 * kSlots; ++i)
+        md.R0inv[i] = quot_ok(mesh->R0[i]) ? 1.0f / mesh->R0[i] : std::numeric_limits<float>::quiet_NaN();
     std::memcpy(md.theta0, mesh->theta0, sizeof(md.theta0));
     for (int p = 0; p < kNodes; ++p) {
         for (int q = 0; q < 3; ++q) {
@@ -1421,6 +1428,26 @@ extern "C" cs3d_status cs3d_check_sqrt(cs3d_ctx* c, uint64_t* mismatches) {
     (void)hipFree(bad);
     if (e != hipSuccess) return fail(CS3D_HIP_ERROR, hipGetErrorString(e));
     *mismatches = (uint64_t)h;
+    return CS3D_OK;
+}
+
+extern "C" cs3d_status cs3d_check_div(cs3d_ctx* c, int32_t a_significands, int64_t scaled_pairs,
+                                      uint64_t* mismatches) {
+    if (!c || !mismatches || a_significands < 0 || a_significands > (1 << 23) || scaled_pairs < 0)
+        return fail(CS3D_INVALID_ARG, "bad self-check arguments");
+    unsigned long long* bad = nullptr;
+    HIPCHK(hipMalloc(&bad, 6 * sizeof(*bad)));
+    hipError_t e = hipMemsetAsync(bad, 0, 6 * sizeof(*bad), c->stream);
+    if (e == hipSuccess) {
+        launch_check_div(bad, (uint32_t)a_significands, (uint64_t)scaled_pairs, c->stream);
+        e = hipGetLastError();
+    }
+    unsigned long long h[6] = {0, 0, 0, 0, 0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(h, bad, sizeof(h), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(bad);
+    if (e != hipSuccess) return fail(CS3D_HIP_ERROR, hipGetErrorString(e));
+    for (int i = 0; i < 6; ++i) mismatches[i] = (uint64_t)h[i];
     return CS3D_OK;
 }
 

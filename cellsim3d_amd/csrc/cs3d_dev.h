@@ -88,6 +88,7 @@ struct MeshDev {
     int32_t sign[kSlots];
     int32_t faces[CS3D_FACES * 7];
     float R0[3 * kSlots];
+    float R0inv[3 * kSlots];  // RN(1/R0) where quot_ok(R0), NaN elsewhere
     float theta0[3 * kSlots];
     // angle gather table: for node p, six offsets (in floats) into the per-cell
     // angle-force array, in the summation order of CalculateAngleForce

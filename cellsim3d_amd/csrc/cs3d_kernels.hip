@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "cs3d_acos.h"
+#include "cs3d_div.h"
 #include "cs3d_dev.h"
 
 namespace cs3d {
@@ -79,6 +80,60 @@ __device__ __forceinline__ float sqrt_cr(float x) {
     float t = rm <= 0.f ? sm : s;
     t = rp > 0.f ? sp : t;
     return tiny ? t * 0x1p-16f : t;
+}
+
+// RN(1/b) for quot_ok(b): the hardware reciprocal estimate and one Newton step
+// with an fma residual.  Equal to 1.0f / b for every float b with quot_ok(b)
+// (cs3d_check_div).
+__device__ __forceinline__ float rcp_rn(float b) {
+    const float y0 = __builtin_amdgcn_rcpf(b);
+    const float e = fmaf(-b, y0, 1.0f);
+    return fmaf(e, y0, y0);
+}
+
+// Marks the division path of a quotient as a branch: the empty volatile asm
+// keeps the compiler from evaluating both paths and selecting (which it does for
+// a short division otherwise).
+__device__ __forceinline__ void slow_path() { asm volatile(""); }
+
+// 1.0f / b: from rcp_rn where quot_ok(b), by division otherwise (the same bits)
+__device__ __forceinline__ float inv_rn(float b) {
+    float y = rcp_rn(b);
+    if (!quot_ok(b)) {
+        slow_path();
+        y = 1.0f / b;
+    }
+    return y;
+}
+
+// One bond's spring term stiffness*(R-R0)/R0*d/R for d = dx, dy, dz
+// (propagate.cu:311-313).  spring_rcp takes the quotients from the reciprocals of
+// R and R0 (iR0 = RN(1/R0), NaN where R0 is outside quot_ok's range) and returns
+// whether every operand was in range (quot_ok, quot_num_ok); where it was, the
+// terms equal spring_div's, the reference's divisions, bit for bit.  The callers
+// sum a node's three bonds with spring_rcp and, where some operand was out of
+// range, sum them again with spring_div (one branch per node).
+__device__ __forceinline__ bool spring_rcp(float stiffness, float R, float R0, float iR0, float dx, float dy, float dz,
+                                           float& tx, float& ty, float& tz) {
+    const float a = stiffness * (R - R0);
+    const float c = quot_rn(a, R0, iR0);
+    const float yR = rcp_rn(R);
+    const float nx = c * dx, ny = c * dy, nz = c * dz;
+    tx = quot_rn(nx, R, yR);
+    ty = quot_rn(ny, R, yR);
+    tz = quot_rn(nz, R, yR);
+    bool ok = quot_ok(R);  // every compare evaluated, no short-circuit branches
+    ok &= quot_num_ok(a);
+    ok &= quot_num_ok(nx);
+    ok &= quot_num_ok(ny);
+    ok &= quot_num_ok(nz);
+    return ok;
+}
+__device__ __forceinline__ void spring_div(float stiffness, float R, float R0, float dx, float dy, float dz, float& tx,
+                                           float& ty, float& tz) {
+    tx = stiffness * (R - R0) / R0 * dx / R;
+    ty = stiffness * (R - R0) / R0 * dy / R;
+    tz = stiffness * (R - R0) / R0 * dz / R;
 }
 
 // rGrowth of the population model, GPUbounce.cu:1077-1088
@@ -633,6 +688,23 @@ __device__ __forceinline__ RuleCtx rule_ctx(MP M, PP P) {
     return R;
 }
 
+// The angle force's quotients idk/ra2 and idk/rb2 from the reciprocals ya2, yb2
+// of ra2, rb2 (cs3d_div.h) where the operands are in range (quot_ok, quot_num_ok),
+// by division otherwise: the same bits either way.  (-1/sqrt(inner) is -inv_rn.)
+__device__ __forceinline__ void angle_quotients(float idk, float ra2, float ya2, float rb2, float yb2, float& ca,
+                                                float& cb) {
+    ca = quot_rn(idk, ra2, ya2);
+    cb = quot_rn(idk, rb2, yb2);
+    bool ok = quot_num_ok(idk);
+    ok &= quot_ok(ra2);
+    ok &= quot_ok(rb2);
+    if (!ok) {
+        slow_path();
+        ca = idk / ra2;
+        cb = idk / rb2;
+    }
+}
+
 // Shared GetAngleForce pair (propagate.cu:26-60): Fa = F(a-v, b-v), Fb = F(b-v, a-v).
 // dot, |.|^2, c1 and theta are symmetric in the two arguments, so both come from
 // one evaluation with bit-identical values.
@@ -642,11 +714,12 @@ __device__ __forceinline__ uint32_t angle_pair(float ax, float ay, float az, flo
     const float ra2 = ax * ax + ay * ay + az * az;
     const float rb2 = bx * bx + by * by + bz * bz;
     const float inner = (float)((double)(ra2 * rb2 - idk * idk) + 1e-3);
-    const float c1 = -1.0f / sqrt_cr(inner);
+    const float c1 = -inv_rn(sqrt_cr(inner));
     const float den = sqrt_cr(ra2) * sqrt_cr(rb2);
     const float theta = (float)cs3d_acos((double)idk / ((double)den + 1e-3));
     const float s = ((-k) * c1) * (theta - th0);
-    const float ca = idk / ra2, cb = idk / rb2;
+    float ca, cb;
+    angle_quotients(idk, ra2, rcp_rn(ra2), rb2, rcp_rn(rb2), ca, cb);
     out[0] = s * (bx - ca * ax); out[1] = s * (by - ca * ay); out[2] = s * (bz - ca * az);
     out[3] = s * (ax - cb * bx); out[4] = s * (ay - cb * by); out[5] = s * (az - cb * bz);
     bool ok = isfinite(out[0]) && isfinite(out[1]) && isfinite(out[2]) && isfinite(out[3]) && isfinite(out[4]) &&
@@ -657,7 +730,10 @@ __device__ __forceinline__ uint32_t angle_pair(float ax, float ay, float az, flo
 // GetAngleForce pair (as angle_pair) with the squared lengths ra2, rb2 of the two
 // bond vectors and their square roots la, lb computed by the caller: the same
 // operations on the same vectors as in angle_pair (the springs of the node use
-// them too), so every bit is the same.
+// them too), so every bit is the same.  Its three quotients stay divisions here:
+// from reciprocals, with the division path kept for operands out of range, they
+// would keep more values live across the acos than the six-wave register budget
+// of k_con_dis holds.
 __device__ __forceinline__ uint32_t angle_pair_l(float ax, float ay, float az, float ra2, float la, float bx, float by,
                                                  float bz, float rb2, float lb, float th0, float k, float* out) {
     const float idk = ax * bx + ay * by + az * bz;
@@ -680,7 +756,7 @@ __device__ __forceinline__ void node_normal(const float* sx, const float* sy, co
     const float Ax = sx[N2] - sx[N1], Ay = sy[N2] - sy[N1], Az = sz[N2] - sz[N1];
     const float Bx = sx[N3] - sx[N1], By = sy[N3] - sy[N1], Bz = sz[N3] - sz[N1];
     const float cx = Ay * Bz - Az * By, cy = Az * Bx - Ax * Bz, cz = Ax * By - Ay * Bx;
-    const float inv = 1.0f / sqrt_cr(cx * cx + cy * cy + cz * cz);
+    const float inv = inv_rn(sqrt_cr(cx * cx + cy * cy + cz * cz));
     const float sg = (float)ldu(mesh->sign, lane_index());
     nx = sg * (inv * cx);
     ny = sg * (inv * cy);
@@ -1004,13 +1080,29 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
                 }
                 // ---- springs, propagate.cu:298-314
                 const float stiffness = cld(p.E + slot);
+                bool ok = true;
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
-                    const float R = bl[i];
-                    const float R0 = ldu(mesh->R0, (uint32_t)(i * kSlots) + lane_index());
-                    FX += stiffness * (R - R0) / R0 * bx[i] / R;
-                    FY += stiffness * (R - R0) / R0 * by[i] / R;
-                    FZ += stiffness * (R - R0) / R0 * bz[i] / R;
+                    const uint32_t li = (uint32_t)(i * kSlots) + lane_index();
+                    float tx, ty, tz;
+                    ok &= spring_rcp(stiffness, bl[i], ldu(mesh->R0, li), ldu(mesh->R0inv, li), bx[i], by[i], bz[i],
+                                     tx, ty, tz);
+                    FX += tx;
+                    FY += ty;
+                    FZ += tz;
+                }
+                if (!ok) {  // an operand out of range: the three springs by division
+                    slow_path();
+                    FX = 0.f; FY = 0.f; FZ = 0.f;
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) {
+                        float tx, ty, tz;
+                        spring_div(stiffness, bl[i], ldu(mesh->R0, (uint32_t)(i * kSlots) + lane_index()), bx[i], by[i],
+                                   bz[i], tx, ty, tz);
+                        FX += tx;
+                        FY += ty;
+                        FZ += tz;
+                    }
                 }
             }
             if (angles) __syncthreads();  // the angle forces of every node are in LDS
@@ -1018,7 +1110,7 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
                 const float X = lds_reread(sx, tid), Y = lds_reread(sy, tid), Z = lds_reread(sz, tid);
                 // ---- pressure, propagate.cu:318-330 (centre of mass from the last CenterOfMass)
                 const float rx = X - cld(p.CMX + slot), ry = Y - cld(p.CMY + slot), rz = Z - cld(p.CMZ + slot);
-                const float inv = 1.0f / sqrt_cr(rx * rx + ry * ry + rz * rz);
+                const float inv = inv_rn(sqrt_cr(rx * rx + ry * ry + rz * rz));
                 const float g = 3.0f * cld(p.P + slot);
                 FX += g * (inv * rx);
                 FY += g * (inv * ry);
@@ -1158,12 +1250,13 @@ __global__ __launch_bounds__(192) void k_bonded(Pool p, Range r) {
     const bool node = tid < kNodes;
     const MeshDev* mesh = p.mesh;
     int nb[3] = {0, 0, 0};
-    float r0[3] = {1.f, 1.f, 1.f}, th[3] = {0.f, 0.f, 0.f};
+    float r0[3] = {1.f, 1.f, 1.f}, ir0[3] = {1.f, 1.f, 1.f}, th[3] = {0.f, 0.f, 0.f};
     if (node) {
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             nb[i] = mesh->nn[i * kSlots + tid];
             r0[i] = mesh->R0[i * kSlots + tid];
+            ir0[i] = mesh->R0inv[i * kSlots + tid];
             if (ANGLES) th[i] = mesh->theta0[3 * tid + i];
         }
     }
@@ -1209,18 +1302,33 @@ __global__ __launch_bounds__(192) void k_bonded(Pool p, Range r) {
                 if (isnan(X) || isnan(Y) || isnan(Z)) st |= CS3D_SYS_NAN;
                 float FX = 0.f, FY = 0.f, FZ = 0.f;
                 const float stiffness = p.E[slot];
+                float dX[3], dY[3], dZ[3], R[3];
+                bool ok = true;
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
                     const int N1 = nb[i];
-                    const float dX = sx[N1] - X, dY = sy[N1] - Y, dZ = sz[N1] - Z;
-                    const float R = sqrt_cr(dX * dX + dY * dY + dZ * dZ);
-                    const float R0 = r0[i];
-                    FX += stiffness * (R - R0) / R0 * dX / R;
-                    FY += stiffness * (R - R0) / R0 * dY / R;
-                    FZ += stiffness * (R - R0) / R0 * dZ / R;
+                    dX[i] = sx[N1] - X; dY[i] = sy[N1] - Y; dZ[i] = sz[N1] - Z;
+                    R[i] = sqrt_cr(dX[i] * dX[i] + dY[i] * dY[i] + dZ[i] * dZ[i]);
+                    float tx, ty, tz;
+                    ok &= spring_rcp(stiffness, R[i], r0[i], ir0[i], dX[i], dY[i], dZ[i], tx, ty, tz);
+                    FX += tx;
+                    FY += ty;
+                    FZ += tz;
+                }
+                if (!ok) {  // an operand out of range: the three springs by division
+                    slow_path();
+                    FX = 0.f; FY = 0.f; FZ = 0.f;
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) {
+                        float tx, ty, tz;
+                        spring_div(stiffness, R[i], r0[i], dX[i], dY[i], dZ[i], tx, ty, tz);
+                        FX += tx;
+                        FY += ty;
+                        FZ += tz;
+                    }
                 }
                 const float rx = X - p.CMX[slot], ry = Y - p.CMY[slot], rz = Z - p.CMZ[slot];
-                const float inv = 1.0f / sqrt_cr(rx * rx + ry * ry + rz * rz);
+                const float inv = inv_rn(sqrt_cr(rx * rx + ry * ry + rz * rz));
                 const float g = 3.0f * p.P[slot];
                 FX += g * (inv * rx);
                 FY += g * (inv * ry);
@@ -2233,6 +2341,65 @@ __global__ __launch_bounds__(256) void k_check_sqrt(unsigned long long* bad) {
     }
     if (nb) atomicAdd(bad, nb);
 }
+// Each check also counts what shows that it ran and can see a difference: the
+// reciprocal check the floats it compared, the quotient checks the pairs where
+// the uncorrected product RN(a * y) differs from a / b (about a quarter).
+// rcp_rn against 1.0f / b for every float b with quot_ok(b): out[0] differing,
+// out[1] compared
+__global__ __launch_bounds__(256) void k_check_rcp(unsigned long long* out) {
+    unsigned long long nb = 0, nc = 0;
+    for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < (1ull << 32);
+         u += (uint64_t)gridDim.x * blockDim.x) {
+        const float b = __uint_as_float((uint32_t)u);
+        if (quot_ok(b)) {
+            nb += __float_as_uint(rcp_rn(b)) != __float_as_uint(1.0f / b);
+            ++nc;
+        }
+    }
+    if (nb) atomicAdd(out, nb);
+    atomicAdd(out + 1, nc);
+}
+// quot_rn(a, b, rcp_rn(b)) against a / b: b in [1, 2), one significand per thread
+// (the grid is exactly 2^23 threads), a in [1, 2) with the a_count significands
+// from a_first (a_first + a_count <= 2^23, checked by the caller), and a = +-0
+// with the first chunk; out[0] differing, out[1] the control
+__global__ __launch_bounds__(256) void k_check_quot(unsigned long long* out, uint32_t a_first, uint32_t a_count) {
+    const uint32_t bs = blockIdx.x * blockDim.x + threadIdx.x;
+    const float b = __uint_as_float(0x3f800000u | bs);
+    const float y = rcp_rn(b);
+    uint32_t nb = 0, nc = 0;
+    if (a_first == 0) {  // a = +0 and -0 keep their sign
+        const float zp = __uint_as_float(0u), zm = __uint_as_float(0x80000000u);
+        nb += __float_as_uint(quot_rn(zp, b, y)) != __float_as_uint(zp / b);
+        nb += __float_as_uint(quot_rn(zm, b, y)) != __float_as_uint(zm / b);
+    }
+    for (uint32_t i = 0; i < a_count; ++i) {
+        const float a = __uint_as_float(0x3f800000u | (a_first + i));
+        const uint32_t t = __float_as_uint(a / b);
+        nb += __float_as_uint(quot_rn(a, b, y)) != t;
+        nc += __float_as_uint(a * y) != t;
+    }
+    if (nb) atomicAdd(out, (unsigned long long)nb);
+    atomicAdd(out + 1, (unsigned long long)nc);
+}
+// the same on n pseudo-random pairs over the whole range quot_ok admits: a of
+// either sign, both with exponents in [-40, 39] and random significands
+__global__ __launch_bounds__(256) void k_check_quot_scaled(unsigned long long* out, uint64_t seed, uint64_t n) {
+    unsigned long long nb = 0, nc = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t h = (i + seed) * 0x9E3779B97F4A7C15ull;
+        h ^= h >> 31; h *= 0xBF58476D1CE4E5B9ull; h ^= h >> 29; h *= 0x94D049BB133111EBull; h ^= h >> 32;
+        const uint32_t ea = (uint32_t)(h % 80), eb = (uint32_t)((h >> 8) % 80);
+        const float a = __uint_as_float((uint32_t)((h >> 63) << 31) | ((87u + ea) << 23) | (uint32_t)((h >> 16) & 0x7fffff));
+        const float b = __uint_as_float(((87u + eb) << 23) | (uint32_t)((h >> 39) & 0x7fffff));
+        const float y = rcp_rn(b);
+        const uint32_t t = __float_as_uint(a / b);
+        nb += __float_as_uint(quot_rn(a, b, y)) != t;
+        nc += __float_as_uint(a * y) != t;
+    }
+    if (nb) atomicAdd(out, nb);
+    atomicAdd(out + 1, nc);
+}
 __global__ __launch_bounds__(256) void k_acos(const double* __restrict__ x, double* __restrict__ y, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         y[i] = cs3d_acos(x[i]);
@@ -2390,6 +2557,14 @@ void launch_copy_calibrate(const void* src, void* dst, int64_t n4, hipStream_t s
 }
 void launch_check_sqrt(unsigned long long* bad, hipStream_t st) {
     launch(k_check_sqrt, dim3(16384), dim3(256), st, bad);
+}
+void launch_check_div(unsigned long long* bad, uint32_t a_count, uint64_t scaled_pairs, hipStream_t st) {
+    launch(k_check_rcp, dim3(16384), dim3(256), st, bad);
+    constexpr uint32_t kChunkA = 1024;  // about 5 ms per launch
+    for (uint32_t a0 = 0; a0 < a_count; a0 += kChunkA)
+        launch(k_check_quot, dim3((1u << 23) / 256), dim3(256), st, bad + 2, a0,
+               a_count - a0 < kChunkA ? a_count - a0 : kChunkA);
+    if (scaled_pairs > 0) launch(k_check_quot_scaled, dim3(16384), dim3(256), st, bad + 4, (uint64_t)12345, scaled_pairs);
 }
 void launch_acos(const double* x, double* y, int64_t n, hipStream_t st) {
     launch(k_acos, dim3(n < 1 ? 1 : (n + 255) / 256 < 16384 ? (n + 255) / 256 : 16384), dim3(256), st, x, y, n);

@@ -82,6 +82,7 @@ SIGNATURES = {
     "cs3d_k_divide": ([vp, C.c_int32, C.POINTER(C.c_int32)], st_t),
     "cs3d_copy_calibrate": ([vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)], st_t),
     "cs3d_check_sqrt": ([vp, C.POINTER(C.c_uint64)], st_t),
+    "cs3d_check_div": ([vp, C.c_int32, C.c_int64, C.POINTER(C.c_uint64)], st_t),
     "cs3d_device_acos": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int64], st_t),
     "cs3d_traj_open": ([C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(vp)], st_t),
     "cs3d_traj_append": ([vp, C.c_int32, C.c_int32, C.c_int32, vp, vp, vp, vp], st_t),
@@ -254,6 +255,17 @@ class Context:
         out = C.c_uint64(0)
         check(lib().cs3d_check_sqrt(self.h, C.byref(out)), "cs3d_check_sqrt")
         return int(out.value)
+
+    def check_div(self, a_significands=1 << 23, scaled_pairs=1 << 30):
+        """The kernels' reciprocal and quotients against IEEE division, as six counts:
+        reciprocal differing / compared (every float with |b| in [2^-40, 2^40]);
+        quotient differing / control over every b in [1, 2) times a_significands
+        values of a in [1, 2); the same over scaled_pairs random pairs across the
+        admitted range.  A control counts the pairs where the uncorrected product
+        RN(a * RN(1/b)) differs from a / b."""
+        out = (C.c_uint64 * 6)()
+        check(lib().cs3d_check_div(self.h, int(a_significands), int(scaled_pairs), out), "cs3d_check_div")
+        return tuple(int(v) for v in out)
 
     def device_acos(self, x):
         """The kernels' double acos (the angle force's) of each element of x, computed on the device."""

@@ -397,8 +397,18 @@ cs3d_status cs3d_copy_calibrate(cs3d_ctx* ctx, int64_t bytes, int32_t reps, doub
  * the number of differing results (NaN equals NaN).
  * cs3d_device_acos: the kernels' double acos of the angle force
  * (cellsim3d_amd/csrc/cs3d_acos.h, propagate.cu:36) of n host arguments, into
- * host y.  Both allocate and free device buffers. */
+ * host y.
+ * cs3d_check_div: the kernels' quotients from a reciprocal
+ * (cellsim3d_amd/csrc/cs3d_div.h) against IEEE division, as (differing,
+ * control) pairs.  out[0..1]: the reciprocal against 1.0f/b over every float b
+ * with |b| in [2^-40, 2^40], and the number of such b compared; out[2..3]: the
+ * quotient for every b in [1, 2) and the first a_significands significands of a
+ * in [1, 2) (2^23 covers all 2^46 pairs); out[4..5]: scaled_pairs pseudo-random
+ * pairs over the whole admitted range.  The control of a quotient check counts
+ * the pairs where the uncorrected product RN(a * RN(1/b)) differs from a/b.
+ * All three allocate and free device buffers. */
 cs3d_status cs3d_check_sqrt(cs3d_ctx* ctx, uint64_t* mismatches);
+cs3d_status cs3d_check_div(cs3d_ctx* ctx, int32_t a_significands, int64_t scaled_pairs, uint64_t out[6]);
 cs3d_status cs3d_device_acos(cs3d_ctx* ctx, const double* x, double* y, int64_t n);
 
 /* ---- trajectory (binary format of GPUbounce.cu:1007-1018, 2446-2477) ----- */
