@@ -17,11 +17,13 @@ FP32 FLOP fraction (the kernel is limited by vector-ALU issue, not bytes).
 `cpu_baseline` is the CPU oracle (a C++/OpenMP restatement of the reference
 kernels) timed on this host on a bounded sample of the same workload.
 Further objects in the same line:
+  search_full_launch_ms  one full contact-candidate search of every cell (the
+                   search launch of the steps that search), HIP-event timed
   fp64_error       the FP32 engine against the oracle's FP64 mode on the same
                    2000-cell input (forces, centres of mass, volumes)
   bonded_roofline  the standalone bonded-force kernel on 20,000 cells (3.6 M
-                   nodes), springs + pressure (HBM-bound, 24 B per node) and
-                   with bending (VALU-bound), HIP-event timed
+                   nodes), springs + pressure (24 B per node) and with bending,
+                   HIP-event timed
   ensemble         256 independent tissues for 5000 steps with growth and
                    division, batched in one context, against a size-stratified
                    sample of the same tissues run one after another (plain path)
@@ -520,6 +522,10 @@ def main():
         roofline["valu_frac"] = round(tf / FP32_PEAK_TFLOPS, 5)
         roofline["valu_peak_tflops"] = FP32_PEAK_TFLOPS
     log("bench: phases ms/step %s" % json.dumps({k: round(v, 4) for k, v in phases.items()}))
+    # one full candidate search of every cell (the search launch of the steps that
+    # search; the next step searches again)
+    search_ms = s.time_kernel(2, reps=10)
+    log("bench: full candidate search %.4f ms" % search_ms)
 
     calib = None
     if args.calib_copy_mb > 0:
@@ -562,6 +568,7 @@ def main():
             "grid_interval": args.grid_interval,
         },
         "phases_ms_per_step": {k: round(v, 5) for k, v in phases.items()},
+        "search_full_launch_ms": round(search_ms, 5),
         "contacts": contacts,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -1398,16 +1398,26 @@ extern "C" cs3d_status cs3d_k_divide(cs3d_ctx* c, cs3d_sys s, int32_t* n_events)
 }
 
 extern "C" cs3d_status cs3d_time_kernel(cs3d_ctx* c, cs3d_sys s, int32_t which, int32_t reps, double* avg_ms) {
-    if (!valid_sys(c, s) || reps < 1 || !avg_ms || which < 0 || which > 1)
+    if (!valid_sys(c, s) || reps < 1 || !avg_ms || which < 0 || which > 2)
         return fail(CS3D_INVALID_ARG, "bad kernel timing arguments");
     const Range r = range_of(c, s);
     auto launch = [&]() {
         if (which == 0) launch_bonded(c->pool, r, c->sys[s].prm.constrain_angles ? 1 : 0, c->stream);
-        else launch_dis(c->pool, r, M_COM, 0, 0, false, c->stream);
+        else if (which == 1) launch_dis(c->pool, r, M_COM, 0, 0, false, c->stream);
+        else launch_search(c->pool, r, 1, c->stream);
     };
+    // the full search reads the search bins: filled from the current positions here,
+    // cleared after, and the next step searches again (as after cs3d_k_con_force)
+    if (which == 2) launch_integrate(c->pool, r, I_BINS, 0, c->stream);
     launch();  // warm-up
     double ms = 0.0;
-    const cs3d_status st = timed_groups(c, 1, [&](int) { for (int q = 0; q < reps; ++q) launch(); }, &ms);
+    cs3d_status st = timed_groups(c, 1, [&](int) { for (int q = 0; q < reps; ++q) launch(); }, &ms);
+    if (which == 2) {
+        launch_integrate(c->pool, r, I_CLEAR, 0, c->stream);
+        launch_set_flags(c->pool, r, -1, 1, c->stream);
+        const cs3d_status st2 = finish(c);
+        if (st == CS3D_OK) st = st2;
+    }
     if (st != CS3D_OK) return st;
     *avg_ms = ms / reps;
     return CS3D_OK;

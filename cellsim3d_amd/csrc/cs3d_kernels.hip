@@ -606,16 +606,32 @@ __device__ int find_candidates(const Pool& p, MP M, PP P, int c, float X, float 
         ++nc;
     };
     // 9 rows of 3 x-adjacent bins; the first 4 entries of each bin of a row are
-    // loaded together (independent loads), longer bins finish in a tail loop
+    // loaded together (independent loads), longer bins finish in a tail loop; the
+    // next row's counts load while a row is scanned
     constexpr int kHead = 4;
+    int bin_n[3], cnt_n[3];
+    {
+        const int y3 = wrap1(by - 1, bdy), z3 = wrap1(bz - 1, bdz);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            bin_n[q] = (z3 * bdy + y3) * bdx + wrap1(bx + q - 1, bdx);
+            cnt_n[q] = counts[bin_n[q]];
+        }
+    }
     for (int row = 0; row < 9; ++row) {
-        const int y3 = wrap1(by + row % 3 - 1, bdy), z3 = wrap1(bz + row / 3 - 1, bdz);
         int bi[3], cn[3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            const int x3 = wrap1(bx + q - 1, bdx);
-            bi[q] = (z3 * bdy + y3) * bdx + x3;
-            cn[q] = min(counts[bi[q]], kBCap);
+            bi[q] = bin_n[q];
+            cn[q] = min(cnt_n[q], kBCap);
+        }
+        if (row < 8) {
+            const int y3 = wrap1(by + (row + 1) % 3 - 1, bdy), z3 = wrap1(bz + (row + 1) / 3 - 1, bdz);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                bin_n[q] = (z3 * bdy + y3) * bdx + wrap1(bx + q - 1, bdx);
+                cnt_n[q] = counts[bin_n[q]];
+            }
         }
         float4 head[3][kHead];  // loads of empty slots are masked off
 #pragma unroll

@@ -360,7 +360,8 @@ class System:
         self.ctx.step(n_steps, self.id)
 
     def time_kernel(self, which, reps=20):
-        """Average ms of one launch: which 0 = bonded force, 1 = centre of mass."""
+        """Average ms of one launch: which 0 = bonded force, 1 = centre of mass, 2 = a full
+        contact-candidate search (the next step searches again)."""
         out = C.c_double()
         check(lib().cs3d_time_kernel(self.ctx.h, self.id, int(which), int(reps), C.byref(out)), "cs3d_time_kernel")
         return out.value

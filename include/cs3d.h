@@ -322,8 +322,10 @@ cs3d_status cs3d_graph_prepare(cs3d_ctx* ctx, cs3d_sys sys);
 cs3d_status cs3d_step_timed(cs3d_ctx* ctx, cs3d_sys sys, int32_t n_steps, double* phase_ms);
 /* Average milliseconds of one launch of a kernel on the system's state, from
  * per-launch start/stop HIP events over reps launches on the context's stream: which = 0 the
- * bonded force (cs3d_k_con_force_bonded), 1 CenterOfMass (cs3d_k_com).  For
- * roofline measurements; no counterpart in the reference. */
+ * bonded force (cs3d_k_con_force_bonded), 1 CenterOfMass (cs3d_k_com), 2 a full
+ * contact-candidate search of every cell (with the search bins filled from the current
+ * positions before and cleared after; the next step searches again).  For roofline
+ * measurements; no counterpart in the reference. */
 cs3d_status cs3d_time_kernel(cs3d_ctx* ctx, cs3d_sys sys, int32_t which, int32_t reps, double* avg_ms);
 
 /* ---- per-kernel entry points (state injection / parity) ------------------
