@@ -230,6 +230,30 @@ def test_graph_equals_eager(oracle_mesh):
     c2.close()
 
 
+def test_timed_search_leaves_results_unchanged(oracle_mesh):
+    """The kernel timer's full searches (cs3d_time_kernel, which = 2) fill the search
+    bins, search every cell, clear the bins and make the next step search again: a
+    run with them between steps ends bit-identical to one without."""
+    A, E = _abi(), _engine()
+    p, X, Y, Z, n, mc = _case("tissue", n=27, seed=6)
+    p.grid_interval = 10
+    c1 = E.Context(oracle_mesh, capacity_cells=256, voxel_budget=1 << 22)
+    c2 = E.Context(oracle_mesh, capacity_cells=256, voxel_budget=1 << 22)
+    s1 = c1.add_system(p, X, Y, Z, n, max_cells=mc)
+    s2 = c2.add_system(p, X, Y, Z, n, max_cells=mc)
+    for k in (7, 13, 20):
+        s1.step(k)
+        assert s1.time_kernel(2, reps=3) > 0
+        s2.step(k)
+    c1.sync()
+    c2.sync()
+    assert s1.info()[1] == s2.info()[1] == 40
+    for f in (A.F_X, A.F_Y, A.F_Z, A.F_VX, A.F_VY, A.F_VZ, A.F_FCX, A.F_FDX):
+        assert np.array_equal(s1.get(f)[:n * 192], s2.get(f)[:n * 192]), "field %d differs" % f
+    c1.close()
+    c2.close()
+
+
 @pytest.mark.parametrize("graph", [False, True])
 def test_random_force_matches_oracle(oracle_mesh, graph):
     """add_rands=1: the counter-based normals (Philox4x32-10 + polar method) are
