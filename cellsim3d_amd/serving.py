@@ -200,7 +200,7 @@ class Server:
 def summary(jobs, srv, wall):
     """p50/p99 completion latency (result on the host minus arrival), aggregate
     node-updates/s and the utilisation rho = offered node-updates/s over served
-    node-updates/s (above 1 the requests queue, as SURVEY-size streams do)."""
+    node-updates/s (above 1 the requests queue, as streams of the configured size do)."""
     lat = np.array([j.finished - j.arrival for j in jobs])
     span = max(j.arrival for j in jobs) if jobs else 0.0
     offered = sum(j.n * 180.0 * j.horizon for j in jobs)

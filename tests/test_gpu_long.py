@@ -2,8 +2,8 @@
 unit-radius cells on a 2x2x2 lattice, growth and division at 2 V0, 10^4 steps
 (the 8 cells grow to 287).  The GPU engine and the CPU oracle step side by side
 and are compared every 100 steps: cell counts, steps and status exactly;
-positions, velocities and pressures bit for bit (1e-5 per
-element, is checked first so that a failure states the size of a difference).
+positions, velocities and pressures bit for bit (the 1e-5 per-element force
+tolerance is checked first so that a failure states the size of a difference).
 The kernels and the oracle share the angle force's double acos (cs3d_acos.h):
 with the device and host math libraries' acos instead, one angle near a float
 midpoint (step 8277) made a few values differ by an ulp from there on.

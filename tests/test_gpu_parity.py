@@ -7,7 +7,7 @@ correctly rounded float division and square root, the reference's double
 promotions), so positions, velocities, forces, pressures, volumes and
 centres of mass are compared bit for bit as well; where a test states a
 tolerance instead, it is per element: |gpu - oracle| <= tol * max(|oracle|,
-1e-6 * max|oracle|) with tol = 1e-5, the north-star force bar.
+1e-6 * max|oracle|) with tol = 1e-5, the force tolerance of this engine.
 """
 import ctypes as C
 
