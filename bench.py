@@ -13,7 +13,8 @@ Prints ONE JSON line.  The captured step graph is instantiated and uploaded
 before the warm-up, so the timed region holds graph replays only.  `roofline`
 covers the dominant kernel (the fused contact/friction kernel), timed live with
 HIP events on the engine's stream; its HBM fraction is reported next to its
-FP32 FLOP fraction (the kernel is limited by vector-ALU issue, not bytes).
+FP32 FLOP fraction and the vector-ALU busy fraction measured with rocprofv3
+counters (the kernel is limited by vector-ALU issue, not bytes).
 `cpu_baseline` is the CPU oracle (a C++/OpenMP restatement of the reference
 kernels) timed on this host on a bounded sample of the same workload.
 Further objects in the same line:
@@ -521,6 +522,10 @@ def main():
         roofline["achieved_tflops"] = round(tf, 3)
         roofline["valu_frac"] = round(tf / FP32_PEAK_TFLOPS, 5)
         roofline["valu_peak_tflops"] = FP32_PEAK_TFLOPS
+        if tr and dom in tr.get("valu_busy", {}):
+            # the fraction of vector-ALU issue slots the kernel used, from rocprofv3
+            # counters (a run of its own: counters cannot be read in this run)
+            roofline["valu_busy_counters"] = tr["valu_busy"][dom]
     log("bench: phases ms/step %s" % json.dumps({k: round(v, 4) for k, v in phases.items()}))
     # one full candidate search of every cell (the search launch of the steps that
     # search; the next step searches again)
